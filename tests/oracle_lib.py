@@ -1,0 +1,258 @@
+"""ctypes access to the CPU oracle (oracle/liboracle.so) and, where it was built, to the
+reference's own header-only numerics (oracle/_ref/libphare_ref.so).
+
+Test infrastructure only: the product (phare_amd/) never imports this module.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
+REF_SO = os.path.join(ROOT, "oracle", "_ref", "libphare_ref.so")
+
+# field ids shared with include/phx.h
+F_E, F_B, F_EPRED, F_BPRED, F_EAVG, F_BAVG, F_J, F_VE, F_PE, F_RHO, F_MASSD, F_VI = (
+    0, 3, 6, 9, 12, 15, 18, 21, 24, 25, 26, 27)
+F_POP0 = 32
+
+
+def f_pop(pop, k):
+    """field id of population `pop` moment k (0 n, 1 rho, 2..4 flux x,y,z)"""
+    return F_POP0 + 8 * pop + k
+
+
+# quantity ids of the stand-alone operators: 0..2 B, 3..5 E, 6..8 J, 9 moments
+Q_BX, Q_BY, Q_BZ, Q_EX, Q_EY, Q_EZ, Q_JX, Q_JY, Q_JZ, Q_MOM = range(10)
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_lp = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+
+
+def _ptrs(arrs):
+    return (C.POINTER(C.c_double) * len(arrs))(*[a.ctypes.data_as(C.POINTER(C.c_double)) for a in arrs])
+
+
+def _i3(v):
+    v = list(v) + [0] * (3 - len(v))
+    return np.array(v, dtype=np.int32)
+
+
+def _d3(v):
+    v = list(v) + [1.0] * (3 - len(v))
+    return np.array(v, dtype=np.float64)
+
+
+_lib = None
+_ref = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            raise RuntimeError(f"oracle not built: {ORACLE_SO} (run `make -C oracle`)")
+        L = C.CDLL(ORACLE_SO)
+        L.ora_create.restype = C.c_void_p
+        L.ora_create.argtypes = [C.c_int, C.c_int, _ip, _dp, _ip, C.c_int, C.c_int, _dp,
+                                 C.c_double, C.c_double, C.c_double, C.c_int]
+        L.ora_destroy.argtypes = [C.c_void_p]
+        for name in ("ora_field_shape",):
+            getattr(L, name).argtypes = [C.c_void_p, C.c_int, _ip]
+        L.ora_set_field.argtypes = [C.c_void_p, C.c_int, _dp]
+        L.ora_get_field.argtypes = [C.c_void_p, C.c_int, _dp]
+        L.ora_num_particles.restype = C.c_long
+        L.ora_num_particles.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.ora_set_particles.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_long, _ip, _dp, _dp, _dp, _dp]
+        L.ora_get_particles.argtypes = [C.c_void_p, C.c_int, C.c_int, _ip, _dp, _dp, _dp, _dp]
+        L.ora_load_maxwellian.argtypes = [C.c_void_p, C.c_int] + [_dp] * 7 + [
+            C.c_double, C.c_uint, C.c_ulonglong, C.c_double]
+        for name in ("ora_initialize", "ora_update_ions", "ora_fill_moment_borders",
+                     "ora_fill_ion_borders", "ora_electrons", "ora_average"):
+            getattr(L, name).argtypes = [C.c_void_p]
+        L.ora_update_populations.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int]
+        L.ora_move_ions.argtypes = [C.c_void_p, C.c_double, C.c_int]
+        L.ora_advance.argtypes = [C.c_void_p, C.c_double, C.c_int]
+        L.ora_faraday.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double]
+        L.ora_ampere.argtypes = [C.c_void_p, C.c_int]
+        L.ora_ohm.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.ora_fill_ghosts.argtypes = [C.c_void_p, C.c_int]
+        L.ora_gather.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, _dp, _dp]
+        L.ora_push_error.argtypes = [C.c_void_p, _dp]
+        L.ora_diagnostics.argtypes = [C.c_void_p, _dp]
+        L.ora_weights.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.POINTER(C.c_int), _dp]
+        L.ora_layout_shape.argtypes = [C.c_int, C.c_int, _ip, C.c_int, _ip]
+        L.ora_s_deposit.argtypes = [C.c_int, C.c_int, _ip, _dp, _ip, C.c_long, _ip, _dp, _dp, _dp,
+                                    _dp, C.c_double, _dp, _dp, C.c_void_p]
+        L.ora_s_laplacian.argtypes = [C.c_int, C.c_int, _ip, _dp, _ip, C.c_int, _dp, _dp]
+        L.ora_s_project.argtypes = [C.c_int, C.c_int, _ip, _dp, _ip, C.c_int, C.c_int, C.c_int, _dp, _dp]
+        L.ora_s_faraday.argtypes = [C.c_int, C.c_int, _ip, _dp, _ip, C.c_void_p, C.c_void_p,
+                                    C.c_void_p, C.c_double]
+        L.ora_s_ampere.argtypes = [C.c_int, C.c_int, _ip, _dp, _ip, C.c_void_p, C.c_void_p]
+        L.ora_partition.restype = C.c_long
+        L.ora_partition.argtypes = [C.c_int, _ip, _ip, C.c_long, _ip, _ip, _ip, _ip, _lp]
+        L.ora_boris_trajectory.argtypes = [C.c_int, C.c_long, C.c_double, _dp, C.c_double, C.c_double,
+                                           _dp, _dp, _ip, _dp, _dp, _dp]
+        _lib = L
+    return _lib
+
+
+def ref():
+    """the reference's own numerics, or None where oracle/_ref was not built"""
+    global _ref
+    if _ref is None:
+        if not os.path.exists(REF_SO):
+            return None
+        R = C.CDLL(REF_SO)
+        R.ref_layout_info.argtypes = [C.c_int, C.c_int, _ip, C.c_int, _ip]
+        R.ref_weights.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.POINTER(C.c_int), _dp]
+        R.ref_gather.argtypes = [C.c_int, C.c_int, _ip, _dp, _ip, C.c_void_p, C.c_void_p, C.c_long,
+                                 _ip, _dp, _dp, _dp]
+        R.ref_deposit.argtypes = [C.c_int, C.c_int, _ip, _dp, _ip, C.c_long, _ip, _dp, _dp, _dp, _dp,
+                                  C.c_double, _dp, _dp, C.c_void_p]
+        R.ref_faraday.argtypes = [C.c_int, C.c_int, _ip, _dp, _ip, C.c_void_p, C.c_void_p, C.c_void_p,
+                                  C.c_double]
+        R.ref_ampere.argtypes = [C.c_int, C.c_int, _ip, _dp, _ip, C.c_void_p, C.c_void_p]
+        R.ref_laplacian.argtypes = [C.c_int, C.c_int, _ip, _dp, _ip, C.c_int, _dp, _dp]
+        R.ref_project.argtypes = [C.c_int, C.c_int, _ip, _dp, _ip, C.c_int, C.c_int, C.c_int, _dp, _dp]
+        R.ref_partition.restype = C.c_long
+        R.ref_partition.argtypes = [C.c_int, _ip, _ip, C.c_long, _ip, _ip, _ip, _ip, _lp]
+        _ref = R
+    return _ref
+
+
+def weights(order, dual, icell_local, delta, use_ref=False):
+    L = ref() if use_ref else lib()
+    start = C.c_int(0)
+    w = np.zeros(4)
+    (L.ref_weights if use_ref else L.ora_weights)(order, int(dual), int(icell_local), float(delta),
+                                                  C.byref(start), w)
+    return start.value, w[: order + 1].copy()
+
+
+def layout_shape(dim, order, ncells, q):
+    s = np.zeros(3, dtype=np.int32)
+    lib().ora_layout_shape(dim, order, _i3(ncells), q, s)
+    return tuple(int(x) for x in s[:dim])
+
+
+class Sim:
+    """One periodic patch driven by the oracle's SolverPPC restatement."""
+
+    def __init__(self, dim, order, ncells, dx, lower=None, masses=(1.0,), Te=0.1, eta=0.0, nu=1e-4,
+                 hyper_mode=0, level=0):
+        self.dim, self.order = dim, order
+        self.ncells = list(ncells)
+        self.dx = list(dx)
+        lower = lower or [0] * dim
+        self.npop = len(masses)
+        self.h = lib().ora_create(dim, order, _i3(ncells), _d3(dx), _i3(lower), level, len(masses),
+                                  np.array(masses, dtype=np.float64), Te, eta, nu, hyper_mode)
+        if not self.h:
+            raise RuntimeError("ora_create failed")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().ora_destroy(self.h)
+            self.h = None
+
+    def shape(self, fid):
+        s = np.zeros(3, dtype=np.int32)
+        if lib().ora_field_shape(self.h, fid, s):
+            raise KeyError(fid)
+        return tuple(int(x) for x in s[: self.dim])
+
+    def get(self, fid):
+        a = np.zeros(self.shape(fid))
+        lib().ora_get_field(self.h, fid, a.reshape(-1))
+        return a
+
+    def set(self, fid, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        assert a.shape == self.shape(fid), (a.shape, self.shape(fid))
+        lib().ora_set_field(self.h, fid, a.reshape(-1))
+
+    def set_particles(self, pop, kind, p):
+        n = len(p["weight"])
+        lib().ora_set_particles(self.h, pop, kind, n,
+                                np.ascontiguousarray(p["icell"], dtype=np.int32).reshape(-1),
+                                np.ascontiguousarray(p["delta"], dtype=np.float64).reshape(-1),
+                                np.ascontiguousarray(p["v"], dtype=np.float64).reshape(-1),
+                                np.ascontiguousarray(p["weight"], dtype=np.float64),
+                                np.ascontiguousarray(p["charge"], dtype=np.float64))
+
+    def count(self, pop=0, kind=0):
+        return int(lib().ora_num_particles(self.h, pop, kind))
+
+    def get_particles(self, pop=0, kind=0):
+        n = self.count(pop, kind)
+        p = dict(icell=np.zeros((n, self.dim), dtype=np.int32), delta=np.zeros((n, self.dim)),
+                 v=np.zeros((n, 3)), weight=np.zeros(n), charge=np.zeros(n))
+        if n:
+            lib().ora_get_particles(self.h, pop, kind, p["icell"].reshape(-1), p["delta"].reshape(-1),
+                                    p["v"].reshape(-1), p["weight"], p["charge"])
+        return p
+
+    def load_maxwellian(self, pop, n, V, Vth, charge, ppc, seed, cutoff=1e-16):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64).reshape(-1) for a in [n, *V, *Vth]]
+        rc = lib().ora_load_maxwellian(self.h, pop, *arrs, charge, ppc, seed, cutoff)
+        assert rc == 0
+
+    def _rc(self, rc):
+        if rc:
+            e = np.zeros(4)
+            lib().ora_push_error(self.h, e)
+            raise RuntimeError(f"oracle error {rc}: {e}")
+
+    def initialize(self):
+        self._rc(lib().ora_initialize(self.h))
+
+    def update_populations(self, dt, mode, emset=2):
+        self._rc(lib().ora_update_populations(self.h, emset, dt, mode))
+
+    def update_ions(self):
+        self._rc(lib().ora_update_ions(self.h))
+
+    def fill_moment_borders(self):
+        self._rc(lib().ora_fill_moment_borders(self.h))
+
+    def fill_ion_borders(self):
+        self._rc(lib().ora_fill_ion_borders(self.h))
+
+    def move_ions(self, dt, mode):
+        self._rc(lib().ora_move_ions(self.h, dt, mode))
+
+    def advance(self, dt, nsteps=1):
+        self._rc(lib().ora_advance(self.h, dt, nsteps))
+
+    def faraday(self, bset, eset, bnewset, dt):
+        self._rc(lib().ora_faraday(self.h, bset, eset, bnewset, dt))
+
+    def ampere(self, bset):
+        self._rc(lib().ora_ampere(self.h, bset))
+
+    def electrons(self):
+        self._rc(lib().ora_electrons(self.h))
+
+    def ohm(self, bset, eset):
+        self._rc(lib().ora_ohm(self.h, bset, eset))
+
+    def average(self):
+        self._rc(lib().ora_average(self.h))
+
+    def fill_ghosts(self, fid):
+        self._rc(lib().ora_fill_ghosts(self.h, fid))
+
+    def gather(self, emset, pop=0, kind=0):
+        n = self.count(pop, kind)
+        E = np.zeros((n, 3))
+        B = np.zeros((n, 3))
+        lib().ora_gather(self.h, emset, pop, kind, E.reshape(-1), B.reshape(-1))
+        return E, B
+
+    def diagnostics(self):
+        out = np.zeros(4)
+        lib().ora_diagnostics(self.h, out)
+        return dict(kinetic=out[0], magnetic=out[1], charge=out[2], count=out[3])
