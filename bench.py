@@ -1,0 +1,176 @@
+#!/usr/bin/env python3
+"""Benchmark of the hybrid-PIC ion update + field solve on one MI355X.
+
+A step is one full predictor-corrector step (SolverPPC::advanceLevel: two updatePopulations
+passes with push + gather + deposit, and the Faraday/Ampere/Ohm field solve), replayed from a
+captured hipGraph, over one periodic level resident in HBM. Default workload: the 2D Harris
+sheet, 256x128 cells, 100 particles per cell (3.28M particles), interpolation order 2, plain
+FP64.
+
+Prints one JSON line: particle-steps/s, the roofline of the dominant kernel (the fused
+push+gather+deposit kernel, algorithmic bytes per launch / its mean duration from HIP events)
+and the CPU baseline (the FP64 oracle restatement of the reference method, 1 thread, timed on
+a bounded sample of the same workload).
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from phare_amd import phx  # noqa: E402
+from phare_amd import workloads as W  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+
+def _hip():
+    for name in ("libamdhip64.so", "/opt/rocm/lib/libamdhip64.so"):
+        try:
+            return ctypes.CDLL(name)
+        except OSError:
+            continue
+    raise RuntimeError("HIP runtime not found")
+
+
+def device_sync():
+    rc = _hip().hipDeviceSynchronize()
+    if rc != 0:
+        raise RuntimeError(f"hipDeviceSynchronize failed: {rc}")
+
+
+def workload(name):
+    if name == "harris2d":
+        return W.harris2d()
+    if name == "uniform1d":
+        return W.uniform1d()
+    if name.startswith("uniform3d"):
+        # uniform3d[-o<order>][-n<cells per side>]
+        order, n = 2, 128
+        for tok in name.split("-")[1:]:
+            if tok.startswith("o"):
+                order = int(tok[1:])
+            if tok.startswith("n"):
+                n = int(tok[1:])
+        return W.uniform3d(ncells=(n, n, n), order=order)
+    raise SystemExit(f"unknown workload {name}")
+
+
+def push_bytes(wl, mode):
+    """algorithmic HBM bytes of one fused push+deposit launch (FP64 SoA record R = 12D+40 B per
+    particle read; mode all also writes the mutable part M = 12D+24 B; plus 6 E/B component
+    reads and 5 moment writes per cell)"""
+    D = wl.dim
+    R, M = 12 * D + 40, 12 * D + 24
+    per_particle = R + (M if mode == phx.MODE_ALL else 0)
+    return wl.n_particles * per_particle + wl.n_cells * 11 * 8
+
+
+def step_bytes(wl):
+    """algorithmic bytes of one PPC step: N_p (2R+M) + N_c * 8 * (189 + 20 npop)"""
+    D = wl.dim
+    R, M = 12 * D + 40, 12 * D + 24
+    return wl.n_particles * (2 * R + M) + wl.n_cells * 8 * (189 + 20 * len(wl.masses))
+
+
+def cpu_baseline(wl, steps):
+    """the oracle (FP64 restatement of the reference method: AoS particles, CellMap, the same
+    loops) on one host thread, on `steps` PPC steps of the same workload after one warm-up step"""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+
+    sim = O.Sim(wl.dim, wl.order, wl.ncells, wl.dx, masses=wl.masses, Te=wl.Te)
+    wl.setup(sim, O.F_B)
+    sim.advance(wl.dt, 1)
+    t0 = time.perf_counter()
+    sim.advance(wl.dt, steps)
+    el = time.perf_counter() - t0
+    return dict(value=wl.n_particles * steps / el, unit="particle-steps/s", cores=1, kind="port",
+                sample=f"{wl.name} {'x'.join(map(str, wl.ncells))} o{wl.order} {wl.ppc} ppc: "
+                       f"{steps} PPC steps after 1 warm-up step ({el:.1f} s), 1 thread, no MPI",
+                seconds=el)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="harris2d")
+    ap.add_argument("--cpu-steps", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--kernel-reps", type=int, default=20)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU: this build has no multi-GPU path (patch sharding over "
+              "xGMI is a design, not built); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    wl = workload(args.workload)
+    lvl = phx.Level(wl.dim, wl.order, wl.ncells, wl.dx, masses=wl.masses, Te=wl.Te,
+                    capacity=wl.n_particles)
+    wl.setup(lvl, phx.F_B)
+    diag0 = lvl.diagnostics()
+
+    lvl.advance(wl.dt, args.warmup)
+    lvl.sync()
+    device_sync()
+    t0 = time.perf_counter()
+    lvl.advance(wl.dt, args.steps)
+    lvl.sync()
+    device_sync()
+    elapsed = time.perf_counter() - t0
+    diag1 = lvl.diagnostics()
+
+    ms_step = 1e3 * elapsed / args.steps
+    value = wl.n_particles * args.steps / elapsed
+
+    # dominant kernel: the fused push + gather + deposit, timed with HIP events on its stream
+    ms_all = lvl.profile_push(wl.dt, phx.MODE_ALL, args.kernel_reps)
+    ms_dom = lvl.profile_push(wl.dt, phx.MODE_DOMAIN_ONLY, args.kernel_reps)
+    bytes_all = push_bytes(wl, phx.MODE_ALL)
+    achieved = bytes_all / (ms_all * 1e-3) / 1e9
+    lvl.close()
+
+    e0 = diag0["kinetic"] + diag0["magnetic"]
+    e1 = diag1["kinetic"] + diag1["magnetic"]
+    out = {
+        "metric": "particle-steps/s (push+gather+deposit)",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{wl.name} {'x'.join(map(str, wl.ncells))} cells, order {wl.order}, "
+                               f"{wl.ppc} ppc, {wl.n_particles} particles, dt={wl.dt}, plain FP64, "
+                               f"one periodic patch",
+                   "particles": wl.n_particles, "cells": wl.n_cells, "interp_order": wl.order,
+                   "precision": "fp64", "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_push (fused prePush+gather+Boris+postPush+deposit, mode all)",
+                     "kernel_ms": ms_all, "kernel_ms_domain_only": ms_dom,
+                     "algorithmic_bytes_per_launch": bytes_all},
+        "step_hbm_fraction": step_bytes(wl) / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "energy_drift": (e1 - e0) / e0,
+        "charge_drift": (diag1["charge"] - diag0["charge"]) / diag0["charge"],
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(wl, args.cpu_steps)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

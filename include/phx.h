@@ -1,0 +1,187 @@
+/* phx.h -- C ABI of the MI355X (gfx950) hybrid-PIC ion update and field solve.
+ *
+ * One context = one AMR patch level held resident in HBM (round-one scope: a level made of one
+ * patch; a periodic level is the patch covering the whole domain). Plain pointers and sizes only;
+ * no torch or C++ types cross this boundary. All entry points return a phx_status; on failure
+ * phx_last_error(ctx) describes the error (for a two-cell move: the particle, delta, velocity
+ * and stage, as the reference's DictionaryException does).
+ *
+ * Each entry point names the reference interface it replaces (paths relative to the PHARE tree).
+ * Data formats are the reference's:
+ *  - fields: C-order FP64 arrays with the GridLayout's ghost nodes, sizes of
+ *    GridLayout::allocSize (src/core/data/grid/gridlayout.hpp:849-864): per direction
+ *    nbrCells + 1 + 2*ghosts for primal, one less for dual; ghosts = {2,4,4}[order-1];
+ *  - particles: Particle<dim> (src/core/data/particles/particle.hpp:37-61), either as the
+ *    reference's AoS records (56/64/80 B for dim 1/2/3) or as SoA arrays; iCell is in AMR index
+ *    space, delta in [0,1).
+ * Thread safety: a context is used by one host thread at a time (the reference is
+ * single-threaded per rank); different contexts are independent.
+ */
+#ifndef PHX_H
+#define PHX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct phx_ctx phx_ctx;
+
+typedef enum {
+    PHX_OK                 = 0,
+    PHX_ERR_ARG            = 1, /* bad argument (id, size, dimension) */
+    PHX_ERR_HIP            = 2, /* HIP runtime error (message in phx_last_error) */
+    PHX_ERR_MOVE_TWO_CELLS = 3, /* Boris moved a particle more than 2 cells in a half step
+                                   (boris.hpp:164-165, rethrown as "Updater::updatePopulations") */
+    PHX_ERR_OUT_OF_GHOST   = 4, /* a particle's interpolation stencil left the ghost box */
+    PHX_ERR_CAPACITY       = 5, /* particle capacity exceeded */
+    PHX_ERR_STATE          = 6  /* call not valid in the current state */
+} phx_status;
+
+/* arithmetic of the particle path */
+enum { PHX_PRECISION_FP64 = 0, /* plain: the reference method in FP64 */
+       PHX_PRECISION_FP32 = 1  /* mixed: FP32 particles and moments, FP64 fields */ };
+
+/* UpdaterMode (src/core/numerics/ion_updater/ion_updater.hpp:22) */
+enum { PHX_MODE_DOMAIN_ONLY = 1, PHX_MODE_ALL = 2 };
+
+/* HyperMode (src/core/numerics/ohm/ohm.hpp:15) */
+enum { PHX_HYPER_CONSTANT = 0, PHX_HYPER_SPATIAL = 1 };
+
+/* particle arrays of a population: ParticlesPack (src/core/data/ions/particle_pack.hpp:20-114) */
+enum { PHX_DOMAIN = 0, PHX_PATCH_GHOST = 1, PHX_LEVEL_GHOST = 2 };
+
+/* electromagnetic field sets of SolverPPC (solver_ppc.hpp:52-53): state, predicted, averaged */
+enum { PHX_EM = 0, PHX_EM_PRED = 1, PHX_EM_AVG = 2 };
+
+/* field ids: E/B of each set, J, electron Ve/Pe, ion totals, per-population moments */
+enum {
+    PHX_F_EX = 0, PHX_F_EY, PHX_F_EZ, PHX_F_BX, PHX_F_BY, PHX_F_BZ,   /* state E, B */
+    PHX_F_EPRED_X = 6,  /* .. 11: predicted E, B */
+    PHX_F_EAVG_X  = 12, /* .. 17: averaged E, B */
+    PHX_F_JX      = 18, PHX_F_JY, PHX_F_JZ,
+    PHX_F_VEX     = 21, PHX_F_VEY, PHX_F_VEZ,
+    PHX_F_PE      = 24,
+    PHX_F_RHO     = 25, /* ion charge density (Ions::chargeDensity) */
+    PHX_F_MASSD   = 26, /* ion mass density */
+    PHX_F_VIX     = 27, PHX_F_VIY, PHX_F_VIZ,
+    PHX_F_POP0    = 32  /* + 8*pop + {0: particleDensity, 1: chargeDensity, 2..4: flux x,y,z} */
+};
+#define PHX_F_POP(pop, k) (PHX_F_POP0 + 8 * (pop) + (k))
+
+#define PHX_MAX_POP 4
+
+typedef struct {
+    int dim;          /* 1, 2, 3 */
+    int interp_order; /* 1, 2, 3 */
+    int precision;    /* PHX_PRECISION_* */
+    int n_pop;
+    double mass[PHX_MAX_POP];  /* IonPopulation::mass */
+    double eta;                /* OhmInfo::eta (resistivity) */
+    double nu;                 /* OhmInfo::nu (hyper-resistivity) */
+    int hyper_mode;            /* PHX_HYPER_* */
+    double Te;                 /* isothermal electron temperature (electrons.hpp:202-212) */
+} phx_config;
+
+typedef struct {
+    int lower[3];     /* AMR box of the patch, cells, inclusive */
+    int upper[3];
+    double dx[3];     /* GridLayout::meshSize */
+    int level;        /* GridLayout::levelNumber */
+    int periodic;     /* 1: the patch covers a periodic domain (same-level fills wrap onto itself) */
+    /* UpdaterSelectionBoxing::nonLevelGhostBox (ion_updater.hpp:128; amr_utils.hpp:233-251);
+       n_nonlevel_ghost_boxes == 0 means {domain box, domain box grown by the particle ghost width} */
+    int n_nonlevel_ghost_boxes;
+    int nonlevel_ghost_lower[27][3];
+    int nonlevel_ghost_upper[27][3];
+} phx_patch;
+
+/* ---- context ------------------------------------------------------------------------------ */
+/* replaces constructing IonUpdater{dict} (ion_updater.hpp:53-56) + SolverPPC{dict}
+   (solver_ppc.hpp:69-75) + allocating the patch data the ResourcesManager would bind */
+phx_status phx_create(const phx_config* cfg, const phx_patch* patch, size_t particle_capacity,
+                      phx_ctx** out);
+void phx_destroy(phx_ctx* ctx);
+const char* phx_last_error(const phx_ctx* ctx);
+phx_status phx_synchronize(phx_ctx* ctx);
+
+/* ---- data movement (host <-> HBM) ---------------------------------------------------------- */
+/* shape of a field id as GridLayout::allocSize gives it (unused dimensions are 1) */
+phx_status phx_field_shape(const phx_ctx* ctx, int field_id, int shape[3]);
+phx_status phx_set_field(phx_ctx* ctx, int field_id, const double* host);
+phx_status phx_get_field(phx_ctx* ctx, int field_id, double* host);
+
+/* SoA particle arrays: icell[n*dim], delta[n*dim], v[n*3], weight[n], charge[n] */
+phx_status phx_set_particles(phx_ctx* ctx, int pop, int kind, size_t n, const int32_t* icell,
+                             const double* delta, const double* v, const double* weight,
+                             const double* charge);
+phx_status phx_particle_count(phx_ctx* ctx, int pop, int kind, size_t* n);
+phx_status phx_get_particles(phx_ctx* ctx, int pop, int kind, int32_t* icell, double* delta,
+                             double* v, double* weight, double* charge);
+/* reference AoS records Particle<dim> (particle.hpp:37-61), e.g. ParticleArray::vector().data() */
+phx_status phx_set_particles_aos(phx_ctx* ctx, int pop, int kind, size_t n, const void* records);
+phx_status phx_get_particles_aos(phx_ctx* ctx, int pop, int kind, void* records);
+
+/* MaxwellianParticleInitializer::loadParticles (maxwellian_particle_initializer.hpp:137-198):
+   per-cell profiles in the row-major AMR cell order of layout.indices(AMRBox); the particles
+   are appended to the population's domain array */
+phx_status phx_load_maxwellian(phx_ctx* ctx, int pop, const double* density, const double* vx,
+                               const double* vy, const double* vz, const double* vthx,
+                               const double* vthy, const double* vthz, double charge,
+                               uint32_t ppc, uint64_t seed, double density_cut_off);
+
+/* ---- hot path --------------------------------------------------------------------------- */
+/* IonUpdater::updatePopulations(ions, em, boxing, dt, mode) (ion_updater.hpp:58-60, 88-107):
+   resets the population moments, pushes (Boris: pusher/boris.hpp:93-138) with the selection
+   semantics of updateAndDepositDomain_ / updateAndDepositAll_ (:169-295) and deposits
+   (Interpolator::operator() interpolator.hpp:468-510). em_set selects the E/B set. */
+phx_status phx_update_populations(phx_ctx* ctx, int em_set, double dt, int mode);
+/* IonUpdater::updateIons (ion_updater.hpp:111-116) */
+phx_status phx_update_ions(phx_ctx* ctx);
+/* same-level fills of HybridHybridMessengerStrategy (hybrid_hybrid_messenger_strategy.hpp):
+   fillFluxBorders + fillDensityBorders (:419-483), fillIonBorders (:486-497),
+   fillIonGhostParticles (:404-415), field ghost fills (:376-401) */
+phx_status phx_fill_moment_borders(phx_ctx* ctx);
+phx_status phx_fill_ion_borders(phx_ctx* ctx);
+phx_status phx_fill_ion_ghost_particles(phx_ctx* ctx);
+phx_status phx_fill_ghosts(phx_ctx* ctx, int field_id_x); /* the 3 components from _x */
+
+/* Faraday<GL>{layout}(B, E, Bnew, dt) (faraday.hpp:27-48) with E/B sets */
+phx_status phx_faraday(phx_ctx* ctx, int b_set, int e_set, int bnew_set, double dt);
+/* Ampere<GL>{layout}(B, J) (ampere.hpp:26-47) */
+phx_status phx_ampere(phx_ctx* ctx, int b_set);
+/* Electrons::update (electrons.hpp:304-314): Ve and Pe from J, Vi and the charge density */
+phx_status phx_electrons_update(phx_ctx* ctx);
+/* Ohm<GL>{info, layout}(n, Ve, Pe, B, J, Enew) (ohm.hpp:48-64) */
+phx_status phx_ohm(phx_ctx* ctx, int b_set, int e_out_set);
+/* SolverPPC::average_ (solver_ppc.hpp:483-510) including the Eavg ghost fill */
+phx_status phx_average(phx_ctx* ctx);
+
+/* SolverPPC::moveIons_ (solver_ppc.hpp:537-598): updatePopulations + fills + updateIons */
+phx_status phx_move_ions(phx_ctx* ctx, double dt, int mode);
+/* HybridLevelInitializer::initialize root branch (hybrid_level_initializer.hpp:103-171):
+   moments from the domain particles, then J by Ampere and E by Ohm */
+phx_status phx_initialize(phx_ctx* ctx);
+/* SolverPPC::advanceLevel (solver_ppc.hpp:315-341), nsteps times; the whole step is a
+   captured hipGraph replayed per step (periodic level only) */
+phx_status phx_advance(phx_ctx* ctx, double dt, int nsteps);
+
+/* FP64 diagnostics: [kinetic energy, magnetic energy, total charge, particle count] */
+phx_status phx_diagnostics(phx_ctx* ctx, double out[4]);
+
+/* the HIP stream the context launches on (a hipStream_t), for timing with HIP events */
+void* phx_stream(phx_ctx* ctx);
+
+/* ---- measurement (no counterpart in the reference: a benchmark hook) ------------------------ */
+/* launches only the fused push+gather+deposit kernel of a periodic level (population 0, E/B of
+   the averaged set) `reps` times, each between two HIP events on the context stream, and returns
+   the mean kernel time in milliseconds. Mode PHX_MODE_ALL advances the particles in place. */
+phx_status phx_profile_push(phx_ctx* ctx, double dt, int mode, int reps, double* avg_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PHX_H */

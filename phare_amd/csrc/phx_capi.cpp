@@ -1,0 +1,1282 @@
+// phx_capi.cpp -- host side of the C ABI declared in include/phx.h.
+//
+// A context holds one patch level resident in HBM: the Yee fields of SolverPPC (E/B state,
+// predicted and averaged sets, J, electron Ve/Pe), the ion totals and per-population moments,
+// and the particle arrays of each population in structure-of-arrays form. Every kernel runs on
+// the context's own HIP stream; SolverPPC::advanceLevel is captured once into a hipGraph and
+// replayed per step. Device memory is allocated at creation (and when a particle array must
+// grow); the hot path allocates nothing.
+#include "../../include/phx.h"
+#include "phx_kernels.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <random>
+#include <sstream>
+#include <string>
+#include <vector>
+
+using namespace phx;
+
+namespace
+{
+
+struct Store
+{
+    size_t n = 0, cap = 0;
+    int32_t* ic[3] = {nullptr, nullptr, nullptr};
+    double* de[3]  = {nullptr, nullptr, nullptr};
+    double* v[3]   = {nullptr, nullptr, nullptr};
+    double* w      = nullptr;
+    double* q      = nullptr;
+
+    PView view() const
+    {
+        PView p;
+        for (int d = 0; d < 3; ++d)
+        {
+            p.ic[d] = ic[d];
+            p.de[d] = de[d];
+            p.v[d]  = v[d];
+        }
+        p.w = w;
+        p.q = q;
+        return p;
+    }
+};
+
+struct Pop
+{
+    double mass = 1.;
+    Store arr[3]; // domain, patch ghost, level ghost
+    double* mom[5] = {};
+};
+
+} // namespace
+
+struct phx_ctx
+{
+    phx_config cfg{};
+    phx_patch patch{};
+    Geom G{};
+    Boxes boxes{};
+    hipStream_t stream = nullptr;
+
+    std::vector<void*> allocs;
+    double* em[3][6] = {};
+    double* J[3]     = {};
+    double* Ve[3]    = {};
+    double* Pe = nullptr, *rho = nullptr, *massD = nullptr, *tmp = nullptr;
+    double* Vi[3] = {};
+    Pop pops[PHX_MAX_POP];
+    Store scratch;
+
+    DevError* err           = nullptr;
+    unsigned int* dropped   = nullptr;
+    uint8_t* status         = nullptr;
+    size_t statusCap        = 0;
+    unsigned int* blockCnt  = nullptr;
+    size_t blockCap         = 0;
+    unsigned long long* tot = nullptr;
+
+    hipGraphExec_t graph = nullptr;
+    double graphDt       = std::numeric_limits<double>::quiet_NaN();
+    std::string lastError;
+};
+
+namespace
+{
+
+phx_status fail(phx_ctx* c, phx_status s, std::string const& msg)
+{
+    if (c)
+        c->lastError = msg;
+    return s;
+}
+
+#define PHX_HIP(ctx, call)                                                                         \
+    do                                                                                             \
+    {                                                                                              \
+        hipError_t e_ = (call);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+            return fail(ctx, PHX_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));      \
+    } while (0)
+
+#define PHX_TRY(expr)                                                                              \
+    do                                                                                             \
+    {                                                                                              \
+        phx_status s_ = (expr);                                                                    \
+        if (s_ != PHX_OK)                                                                          \
+            return s_;                                                                             \
+    } while (0)
+
+phx_status devAlloc(phx_ctx* c, void** p, size_t bytes)
+{
+    *p = nullptr;
+    if (bytes == 0)
+        bytes = 8;
+    PHX_HIP(c, hipMalloc(p, bytes));
+    c->allocs.push_back(*p);
+    return PHX_OK;
+}
+
+void devFree(phx_ctx* c, void* p)
+{
+    if (!p)
+        return;
+    auto it = std::find(c->allocs.begin(), c->allocs.end(), p);
+    if (it != c->allocs.end())
+        c->allocs.erase(it);
+    (void)hipFree(p);
+}
+
+phx_status storeAlloc(phx_ctx* c, Store& s, size_t cap)
+{
+    int const D = c->cfg.dim;
+    for (int d = 0; d < 3; ++d)
+    {
+        PHX_TRY(devAlloc(c, (void**)&s.ic[d], d < D ? cap * 4 : 8));
+        PHX_TRY(devAlloc(c, (void**)&s.de[d], d < D ? cap * 8 : 8));
+        PHX_TRY(devAlloc(c, (void**)&s.v[d], cap * 8));
+    }
+    PHX_TRY(devAlloc(c, (void**)&s.w, cap * 8));
+    PHX_TRY(devAlloc(c, (void**)&s.q, cap * 8));
+    s.cap = cap;
+    return PHX_OK;
+}
+
+void storeFree(phx_ctx* c, Store& s)
+{
+    for (int d = 0; d < 3; ++d)
+    {
+        devFree(c, s.ic[d]);
+        devFree(c, s.de[d]);
+        devFree(c, s.v[d]);
+        s.ic[d] = nullptr;
+        s.de[d] = nullptr;
+        s.v[d]  = nullptr;
+    }
+    devFree(c, s.w);
+    devFree(c, s.q);
+    s.w = s.q = nullptr;
+    s.cap = s.n = 0;
+}
+
+// grow a store to hold at least `need` particles, keeping its content
+phx_status storeReserve(phx_ctx* c, Store& s, size_t need)
+{
+    if (need <= s.cap)
+        return PHX_OK;
+    size_t cap = std::max(need, s.cap + s.cap / 2 + 1024);
+    Store ns;
+    PHX_TRY(storeAlloc(c, ns, cap));
+    int const D = c->cfg.dim;
+    if (s.n)
+    {
+        for (int d = 0; d < D; ++d)
+        {
+            PHX_HIP(c, hipMemcpyAsync(ns.ic[d], s.ic[d], s.n * 4, hipMemcpyDeviceToDevice, c->stream));
+            PHX_HIP(c, hipMemcpyAsync(ns.de[d], s.de[d], s.n * 8, hipMemcpyDeviceToDevice, c->stream));
+        }
+        for (int d = 0; d < 3; ++d)
+            PHX_HIP(c, hipMemcpyAsync(ns.v[d], s.v[d], s.n * 8, hipMemcpyDeviceToDevice, c->stream));
+        PHX_HIP(c, hipMemcpyAsync(ns.w, s.w, s.n * 8, hipMemcpyDeviceToDevice, c->stream));
+        PHX_HIP(c, hipMemcpyAsync(ns.q, s.q, s.n * 8, hipMemcpyDeviceToDevice, c->stream));
+        PHX_HIP(c, hipStreamSynchronize(c->stream));
+    }
+    ns.n = s.n;
+    storeFree(c, s);
+    s = ns;
+    return PHX_OK;
+}
+
+phx_status scratchReserve(phx_ctx* c, size_t n)
+{
+    PHX_TRY(storeReserve(c, c->scratch, n));
+    if (n > c->statusCap)
+    {
+        devFree(c, c->status);
+        size_t cap = std::max(n, c->statusCap * 2);
+        PHX_TRY(devAlloc(c, (void**)&c->status, cap));
+        c->statusCap = cap;
+    }
+    size_t nb = compactBlocks(n) + 1;
+    if (nb > c->blockCap)
+    {
+        devFree(c, c->blockCnt);
+        PHX_TRY(devAlloc(c, (void**)&c->blockCnt, nb * 4 * 2));
+        c->blockCap = nb * 2;
+    }
+    return PHX_OK;
+}
+
+// field id -> (pointer, quantity)
+bool fieldOf(phx_ctx const* c, int fid, double*& p, int& q)
+{
+    if (fid >= PHX_F_POP0)
+    {
+        int pop = (fid - PHX_F_POP0) / 8, k = (fid - PHX_F_POP0) % 8;
+        if (pop >= c->cfg.n_pop || k > 4)
+            return false;
+        p = c->pops[pop].mom[k];
+        q = QMOM;
+        return true;
+    }
+    if (fid >= 0 && fid < 18)
+    {
+        int set = fid / 6, r = fid % 6;
+        p       = c->em[set][r];
+        q       = r < 3 ? QEX + r : QBX + (r - 3);
+        return true;
+    }
+    if (fid >= PHX_F_JX && fid <= PHX_F_JZ)
+    {
+        p = c->J[fid - PHX_F_JX];
+        q = QJX + (fid - PHX_F_JX);
+        return true;
+    }
+    q = QMOM;
+    if (fid >= PHX_F_VEX && fid <= PHX_F_VEZ)
+        p = c->Ve[fid - PHX_F_VEX];
+    else if (fid == PHX_F_PE)
+        p = c->Pe;
+    else if (fid == PHX_F_RHO)
+        p = c->rho;
+    else if (fid == PHX_F_MASSD)
+        p = c->massD;
+    else if (fid >= PHX_F_VIX && fid <= PHX_F_VIZ)
+        p = c->Vi[fid - PHX_F_VIX];
+    else
+        return false;
+    return true;
+}
+
+CVecPtr cvec(double* const* p) { return CVecPtr{{p[0], p[1], p[2]}}; }
+VecPtr vec(double* const* p) { return VecPtr{{p[0], p[1], p[2]}}; }
+
+phx_status checkLaunch(phx_ctx* c, char const* what)
+{
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+        return fail(c, PHX_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+    return PHX_OK;
+}
+
+// read the device error record; raise the reference's error semantics
+phx_status checkDeviceError(phx_ctx* c)
+{
+    PHX_HIP(c, hipStreamSynchronize(c->stream));
+    DevError e{};
+    PHX_HIP(c, hipMemcpy(&e, c->err, sizeof(e), hipMemcpyDeviceToHost));
+    if (e.code == 0)
+        return PHX_OK;
+    DevError z{};
+    PHX_HIP(c, hipMemcpy(c->err, &z, sizeof(z), hipMemcpyHostToDevice));
+    std::ostringstream os;
+    if (e.code == PHX_ERR_MOVE_TWO_CELLS)
+        os << "Updater::updatePopulations: " << (e.stage == 1 ? "PrePush" : "PostPush")
+           << " Particle moved 2 cells with delta/vel: " << e.delta << "/" << e.v << " (pop " << e.pop
+           << ", array " << e.kind << ", particle " << e.particle << ")";
+    else
+        os << "Updater::updatePopulations: particle " << e.particle << " (pop " << e.pop << ", array "
+           << e.kind << ") has an interpolation stencil outside the ghost box at stage " << e.stage;
+    return fail(c, static_cast<phx_status>(e.code), os.str());
+}
+
+PushArgs pushArgs(phx_ctx* c, int pop, int kind, int mode, int emSet, double dt)
+{
+    PushArgs A{};
+    A.G = c->G;
+    for (int i = 0; i < 3; ++i)
+    {
+        A.em.E[i] = c->em[emSet][i];
+        A.em.B[i] = c->em[emSet][3 + i];
+    }
+    for (int m = 0; m < 5; ++m)
+        A.mom[m] = c->pops[pop].mom[m];
+    for (int d = 0; d < 3; ++d)
+        A.hdtdx[d] = d < c->cfg.dim ? 0.5 * dt / c->G.dx[d] : 0.;
+    A.dto2m        = 0.5 * dt / c->pops[pop].mass;
+    A.coef         = 1.;
+    A.box          = c->boxes;
+    A.kind         = kind;
+    A.mode         = mode;
+    A.periodicWrap = 0;
+    A.pop          = pop;
+    A.n            = c->pops[pop].arr[kind].n;
+    A.err          = c->err;
+    A.status       = nullptr;
+    A.dropped      = c->dropped;
+    return A;
+}
+
+long count(phx_ctx const* c, int q) { return c->G.size(q); }
+
+void zeroMoments(phx_ctx* c)
+{
+    for (int p = 0; p < c->cfg.n_pop; ++p)
+        for (int m = 0; m < 5; ++m)
+            launchZero(c->pops[p].mom[m], count(c, QMOM), c->stream);
+}
+
+// fillFluxBorders + fillDensityBorders: sum = copy(moment); sum += periodic images; copy back
+phx_status fillMomentBorders(phx_ctx* c)
+{
+    if (!c->patch.periodic)
+        return PHX_OK;
+    for (int p = 0; p < c->cfg.n_pop; ++p)
+        for (int m = 0; m < 5; ++m)
+        {
+            launchCopy(c->pops[p].mom[m], c->tmp, count(c, QMOM), c->stream);
+            launchBorderSum(c->G, c->tmp, c->pops[p].mom[m], c->stream);
+        }
+    return checkLaunch(c, "fillMomentBorders");
+}
+
+phx_status fillIonBorders(phx_ctx* c)
+{
+    if (!c->patch.periodic)
+        return PHX_OK;
+    double* f[5] = {c->rho, c->massD, c->Vi[0], c->Vi[1], c->Vi[2]};
+    for (auto* x : f)
+    {
+        launchCopy(x, c->tmp, count(c, QMOM), c->stream);
+        launchBorderMax(c->G, c->tmp, x, c->stream);
+    }
+    return checkLaunch(c, "fillIonBorders");
+}
+
+void fillGhosts(phx_ctx* c, double* const* comps, int q0)
+{
+    if (!c->patch.periodic)
+        return;
+    for (int k = 0; k < 3; ++k)
+        launchFillGhosts(c->G, q0 + k, comps[k], c->stream);
+}
+
+void updateIons(phx_ctx* c)
+{
+    PopMoments P{};
+    P.npop = c->cfg.n_pop;
+    for (int p = 0; p < P.npop; ++p)
+    {
+        P.n[p]   = c->pops[p].mom[0];
+        P.rho[p] = c->pops[p].mom[1];
+        for (int k = 0; k < 3; ++k)
+            P.F[p][k] = c->pops[p].mom[2 + k];
+        P.mass[p] = c->pops[p].mass;
+    }
+    launchIonTotals(P, c->rho, c->massD, vec(c->Vi), count(c, QMOM), c->stream);
+}
+
+void electrons(phx_ctx* c)
+{
+    launchElectrons(c->G, c->cfg.Te, cvec(c->J), cvec(c->Vi), c->rho, vec(c->Ve), c->Pe, c->stream);
+}
+
+void ohm(phx_ctx* c, int bset, int eset)
+{
+    launchOhm(c->G, c->cfg.eta, c->cfg.nu, c->cfg.hyper_mode, c->patch.level, c->rho, cvec(c->Ve),
+              c->Pe, cvec(c->em[bset] + 3), cvec(c->J), vec(c->em[eset]), c->stream);
+}
+
+// the fast path of a periodic level: the `all` push wraps leavers into the domain in place
+bool fastPath(phx_ctx* c)
+{
+    if (!c->patch.periodic)
+        return false;
+    for (int p = 0; p < c->cfg.n_pop; ++p)
+        if (c->pops[p].arr[PHX_PATCH_GHOST].n || c->pops[p].arr[PHX_LEVEL_GHOST].n)
+            return false;
+    return true;
+}
+
+// enqueue the particle stage of SolverPPC::moveIons_ on the fast path (no host sync)
+phx_status enqueueMoveIonsFast(phx_ctx* c, double dt, int mode)
+{
+    zeroMoments(c);
+    for (int p = 0; p < c->cfg.n_pop; ++p)
+    {
+        PushArgs A     = pushArgs(c, p, KIND_DOMAIN, mode, PHX_EM_AVG, dt);
+        A.periodicWrap = 1;
+        PView v        = c->pops[p].arr[PHX_DOMAIN].view();
+        launchPush(A, v, v, c->stream);
+    }
+    PHX_TRY(checkLaunch(c, "push"));
+    PHX_TRY(fillMomentBorders(c));
+    updateIons(c);
+    return fillIonBorders(c);
+}
+
+// SolverPPC predictor1_/predictor2_ (solver_ppc.hpp:346-428)
+void enqueuePredictor(phx_ctx* c, double dt, int eset)
+{
+    launchFaraday(c->G, cvec(c->em[PHX_EM] + 3), cvec(c->em[eset]), vec(c->em[PHX_EM_PRED] + 3), dt,
+                  c->stream);
+    fillGhosts(c, c->em[PHX_EM_PRED] + 3, QBX);
+    launchAmpere(c->G, cvec(c->em[PHX_EM_PRED] + 3), vec(c->J), c->stream);
+    fillGhosts(c, c->J, QJX);
+    electrons(c);
+    ohm(c, PHX_EM_PRED, PHX_EM_PRED);
+}
+
+// SolverPPC::average_ (solver_ppc.hpp:483-510)
+void enqueueAverage(phx_ctx* c)
+{
+    for (int k = 0; k < 6; ++k)
+        launchAverage(c->em[PHX_EM][k], c->em[PHX_EM_PRED][k], c->em[PHX_EM_AVG][k],
+                      count(c, k < 3 ? QEX + k : QBX + k - 3), c->stream);
+    fillGhosts(c, c->em[PHX_EM_AVG], QEX);
+}
+
+// SolverPPC::corrector_ (solver_ppc.hpp:433-479)
+void enqueueCorrector(phx_ctx* c, double dt)
+{
+    launchFaraday(c->G, cvec(c->em[PHX_EM] + 3), cvec(c->em[PHX_EM_AVG]), vec(c->em[PHX_EM] + 3), dt,
+                  c->stream);
+    fillGhosts(c, c->em[PHX_EM] + 3, QBX);
+    launchAmpere(c->G, cvec(c->em[PHX_EM] + 3), vec(c->J), c->stream);
+    fillGhosts(c, c->J, QJX);
+    electrons(c);
+    ohm(c, PHX_EM, PHX_EM);
+    fillGhosts(c, c->em[PHX_EM], QEX);
+}
+
+phx_status enqueueAdvanceFast(phx_ctx* c, double dt)
+{
+    enqueuePredictor(c, dt, PHX_EM);
+    enqueueAverage(c);
+    PHX_TRY(enqueueMoveIonsFast(c, dt, PHX_MODE_DOMAIN_ONLY));
+    enqueuePredictor(c, dt, PHX_EM_AVG);
+    enqueueAverage(c);
+    PHX_TRY(enqueueMoveIonsFast(c, dt, PHX_MODE_ALL));
+    enqueueCorrector(c, dt);
+    return checkLaunch(c, "advance");
+}
+
+phx_status validPop(phx_ctx* c, int pop, int kind)
+{
+    if (!c)
+        return PHX_ERR_ARG;
+    if (pop < 0 || pop >= c->cfg.n_pop || kind < 0 || kind > 2)
+        return fail(c, PHX_ERR_ARG, "bad population or particle array kind");
+    return PHX_OK;
+}
+
+phx_status upload(phx_ctx* c, Store& s, size_t n, int32_t const* ic, double const* de, double const* v,
+                  double const* w, double const* q, size_t offset)
+{
+    int const D = c->cfg.dim;
+    PHX_TRY(storeReserve(c, s, offset + n));
+    std::vector<int32_t> ib(n);
+    std::vector<double> db(n);
+    for (int d = 0; d < D; ++d)
+    {
+        for (size_t i = 0; i < n; ++i)
+            ib[i] = ic[i * D + d];
+        PHX_HIP(c, hipMemcpy(s.ic[d] + offset, ib.data(), n * 4, hipMemcpyHostToDevice));
+        for (size_t i = 0; i < n; ++i)
+            db[i] = de[i * D + d];
+        PHX_HIP(c, hipMemcpy(s.de[d] + offset, db.data(), n * 8, hipMemcpyHostToDevice));
+    }
+    for (int d = 0; d < 3; ++d)
+    {
+        for (size_t i = 0; i < n; ++i)
+            db[i] = v[i * 3 + d];
+        PHX_HIP(c, hipMemcpy(s.v[d] + offset, db.data(), n * 8, hipMemcpyHostToDevice));
+    }
+    PHX_HIP(c, hipMemcpy(s.w + offset, w, n * 8, hipMemcpyHostToDevice));
+    PHX_HIP(c, hipMemcpy(s.q + offset, q, n * 8, hipMemcpyHostToDevice));
+    s.n = offset + n;
+    return PHX_OK;
+}
+
+// live (not lost) particles of a store, downloaded to host SoA buffers
+phx_status download(phx_ctx* c, Store const& s, std::vector<int32_t>& ic, std::vector<double>& de,
+                    std::vector<double>& v, std::vector<double>& w, std::vector<double>& q)
+{
+    int const D  = c->cfg.dim;
+    size_t const n = s.n;
+    std::vector<int32_t> icd(n * D);
+    std::vector<double> ded(n * D), vd(n * 3), wd(n), qd(n);
+    std::vector<int32_t> ib(n);
+    std::vector<double> db(n);
+    PHX_HIP(c, hipStreamSynchronize(c->stream));
+    for (int d = 0; d < D; ++d)
+    {
+        PHX_HIP(c, hipMemcpy(ib.data(), s.ic[d], n * 4, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n; ++i)
+            icd[i * D + d] = ib[i];
+        PHX_HIP(c, hipMemcpy(db.data(), s.de[d], n * 8, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n; ++i)
+            ded[i * D + d] = db[i];
+    }
+    for (int d = 0; d < 3; ++d)
+    {
+        PHX_HIP(c, hipMemcpy(db.data(), s.v[d], n * 8, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n; ++i)
+            vd[i * 3 + d] = db[i];
+    }
+    PHX_HIP(c, hipMemcpy(wd.data(), s.w, n * 8, hipMemcpyDeviceToHost));
+    PHX_HIP(c, hipMemcpy(qd.data(), s.q, n * 8, hipMemcpyDeviceToHost));
+    ic.clear();
+    de.clear();
+    v.clear();
+    w.clear();
+    q.clear();
+    for (size_t i = 0; i < n; ++i)
+    {
+        if (icd[i * D] == DEAD_CELL)
+            continue;
+        for (int d = 0; d < D; ++d)
+        {
+            ic.push_back(icd[i * D + d]);
+            de.push_back(ded[i * D + d]);
+        }
+        for (int d = 0; d < 3; ++d)
+            v.push_back(vd[i * 3 + d]);
+        w.push_back(wd[i]);
+        q.push_back(qd[i]);
+    }
+    return PHX_OK;
+}
+
+size_t recordSize(int dim) { return dim == 1 ? 56 : (dim == 2 ? 64 : 80); }
+
+} // namespace
+
+// =============================================================================================
+// C ABI
+// =============================================================================================
+extern "C" {
+
+phx_status phx_create(phx_config const* cfg, phx_patch const* patch, size_t particle_capacity,
+                      phx_ctx** out)
+{
+    if (!cfg || !patch || !out)
+        return PHX_ERR_ARG;
+    *out = nullptr;
+    if (cfg->dim < 1 || cfg->dim > 3 || cfg->interp_order < 1 || cfg->interp_order > 3
+        || cfg->n_pop < 1 || cfg->n_pop > PHX_MAX_POP || cfg->precision != PHX_PRECISION_FP64)
+        return PHX_ERR_ARG;
+    if (patch->n_nonlevel_ghost_boxes < 0 || patch->n_nonlevel_ghost_boxes > MAX_NLGB)
+        return PHX_ERR_ARG;
+    auto* c  = new phx_ctx;
+    c->cfg   = *cfg;
+    c->patch = *patch;
+    int const D = cfg->dim, O = cfg->interp_order;
+    Geom& G = c->G;
+    G.dim   = D;
+    G.order = O;
+    G.g     = O == 1 ? 2 : 4;
+    G.pg    = O == 1 ? 1 : 2;
+    for (int d = 0; d < 3; ++d)
+    {
+        G.n[d]     = d < D ? patch->upper[d] - patch->lower[d] + 1 : 1;
+        G.lower[d] = d < D ? patch->lower[d] : 0;
+        G.dx[d]    = d < D ? patch->dx[d] : 1.;
+        G.inv[d]   = 1. / G.dx[d];
+        if (d < D && patch->periodic && G.n[d] <= 2 * G.g)
+        {
+            delete c;
+            return PHX_ERR_ARG; // a periodic patch must be wider than both ghost layers
+        }
+    }
+    for (int q = 0; q < NQ; ++q)
+        for (int d = 0; d < 3; ++d)
+            G.shape[q][d] = d < D ? G.n[d] + 1 + 2 * G.g - centering(q, d) : 1;
+
+    Boxes& B = c->boxes;
+    for (int d = 0; d < 3; ++d)
+    {
+        B.dom_lo[d] = d < D ? patch->lower[d] : 0;
+        B.dom_hi[d] = d < D ? patch->upper[d] : 0;
+        B.gh_lo[d]  = d < D ? patch->lower[d] - G.pg : 0;
+        B.gh_hi[d]  = d < D ? patch->upper[d] + G.pg : 0;
+    }
+    if (patch->n_nonlevel_ghost_boxes == 0)
+    {
+        B.nnlgb = 2;
+        for (int d = 0; d < 3; ++d)
+        {
+            B.nlgb_lo[0][d] = B.dom_lo[d];
+            B.nlgb_hi[0][d] = B.dom_hi[d];
+            B.nlgb_lo[1][d] = B.gh_lo[d];
+            B.nlgb_hi[1][d] = B.gh_hi[d];
+        }
+    }
+    else
+    {
+        B.nnlgb = patch->n_nonlevel_ghost_boxes;
+        for (int b = 0; b < B.nnlgb; ++b)
+            for (int d = 0; d < 3; ++d)
+            {
+                B.nlgb_lo[b][d] = d < D ? patch->nonlevel_ghost_lower[b][d] : 0;
+                B.nlgb_hi[b][d] = d < D ? patch->nonlevel_ghost_upper[b][d] : 0;
+            }
+    }
+
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess)
+    {
+        delete c;
+        return PHX_ERR_HIP;
+    }
+    auto alloc = [&](double** p, int q) { return devAlloc(c, (void**)p, static_cast<size_t>(G.size(q)) * 8); };
+    phx_status s = PHX_OK;
+    for (int set = 0; set < 3 && s == PHX_OK; ++set)
+        for (int k = 0; k < 6 && s == PHX_OK; ++k)
+            s = alloc(&c->em[set][k], k < 3 ? QEX + k : QBX + k - 3);
+    for (int k = 0; k < 3 && s == PHX_OK; ++k)
+    {
+        s = alloc(&c->J[k], QJX + k);
+        if (s == PHX_OK)
+            s = alloc(&c->Ve[k], QMOM);
+        if (s == PHX_OK)
+            s = alloc(&c->Vi[k], QMOM);
+    }
+    if (s == PHX_OK)
+        s = alloc(&c->Pe, QMOM);
+    if (s == PHX_OK)
+        s = alloc(&c->rho, QMOM);
+    if (s == PHX_OK)
+        s = alloc(&c->massD, QMOM);
+    if (s == PHX_OK)
+        s = alloc(&c->tmp, QMOM);
+    for (int p = 0; p < cfg->n_pop && s == PHX_OK; ++p)
+    {
+        c->pops[p].mass = cfg->mass[p];
+        for (int m = 0; m < 5 && s == PHX_OK; ++m)
+            s = alloc(&c->pops[p].mom[m], QMOM);
+        for (int k = 0; k < 3 && s == PHX_OK; ++k)
+            s = storeAlloc(c, c->pops[p].arr[k], k == 0 ? std::max<size_t>(particle_capacity, 1024) : 1024);
+    }
+    if (s == PHX_OK)
+        s = devAlloc(c, (void**)&c->err, sizeof(DevError));
+    if (s == PHX_OK)
+        s = devAlloc(c, (void**)&c->dropped, 64);
+    if (s == PHX_OK)
+        s = devAlloc(c, (void**)&c->tot, 64);
+    if (s != PHX_OK)
+    {
+        phx_destroy(c);
+        return s;
+    }
+    // every field starts at zero
+    for (int set = 0; set < 3; ++set)
+        for (int k = 0; k < 6; ++k)
+            launchZero(c->em[set][k], G.size(k < 3 ? QEX + k : QBX + k - 3), c->stream);
+    for (int k = 0; k < 3; ++k)
+    {
+        launchZero(c->J[k], G.size(QJX + k), c->stream);
+        launchZero(c->Ve[k], G.size(QMOM), c->stream);
+        launchZero(c->Vi[k], G.size(QMOM), c->stream);
+    }
+    for (double* p : {c->Pe, c->rho, c->massD, c->tmp})
+        launchZero(p, G.size(QMOM), c->stream);
+    zeroMoments(c);
+    DevError z{};
+    (void)hipMemcpy(c->err, &z, sizeof(z), hipMemcpyHostToDevice);
+    unsigned int zz[16] = {};
+    (void)hipMemcpy(c->dropped, zz, 64, hipMemcpyHostToDevice);
+    if (hipStreamSynchronize(c->stream) != hipSuccess)
+    {
+        phx_destroy(c);
+        return PHX_ERR_HIP;
+    }
+    *out = c;
+    return PHX_OK;
+}
+
+void phx_destroy(phx_ctx* c)
+{
+    if (!c)
+        return;
+    if (c->stream)
+        (void)hipStreamSynchronize(c->stream);
+    if (c->graph)
+        (void)hipGraphExecDestroy(c->graph);
+    for (void* p : c->allocs)
+        (void)hipFree(p);
+    if (c->stream)
+        (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+char const* phx_last_error(phx_ctx const* c) { return c ? c->lastError.c_str() : "null context"; }
+
+phx_status phx_synchronize(phx_ctx* c)
+{
+    if (!c)
+        return PHX_ERR_ARG;
+    PHX_HIP(c, hipStreamSynchronize(c->stream));
+    return PHX_OK;
+}
+
+void* phx_stream(phx_ctx* c) { return c ? static_cast<void*>(c->stream) : nullptr; }
+
+phx_status phx_field_shape(phx_ctx const* c, int fid, int shape[3])
+{
+    double* p;
+    int q;
+    if (!c || !shape || !fieldOf(c, fid, p, q))
+        return PHX_ERR_ARG;
+    for (int d = 0; d < 3; ++d)
+        shape[d] = c->G.shape[q][d];
+    return PHX_OK;
+}
+
+phx_status phx_set_field(phx_ctx* c, int fid, double const* host)
+{
+    double* p;
+    int q;
+    if (!c || !host || !fieldOf(c, fid, p, q))
+        return fail(c, PHX_ERR_ARG, "bad field id");
+    PHX_HIP(c, hipStreamSynchronize(c->stream));
+    PHX_HIP(c, hipMemcpy(p, host, static_cast<size_t>(c->G.size(q)) * 8, hipMemcpyHostToDevice));
+    return PHX_OK;
+}
+
+phx_status phx_get_field(phx_ctx* c, int fid, double* host)
+{
+    double* p;
+    int q;
+    if (!c || !host || !fieldOf(c, fid, p, q))
+        return fail(c, PHX_ERR_ARG, "bad field id");
+    PHX_HIP(c, hipStreamSynchronize(c->stream));
+    PHX_HIP(c, hipMemcpy(host, p, static_cast<size_t>(c->G.size(q)) * 8, hipMemcpyDeviceToHost));
+    return PHX_OK;
+}
+
+phx_status phx_set_particles(phx_ctx* c, int pop, int kind, size_t n, int32_t const* icell,
+                             double const* delta, double const* v, double const* weight,
+                             double const* charge)
+{
+    PHX_TRY(validPop(c, pop, kind));
+    if (n && (!icell || !delta || !v || !weight || !charge))
+        return fail(c, PHX_ERR_ARG, "null particle buffer");
+    PHX_HIP(c, hipStreamSynchronize(c->stream));
+    return upload(c, c->pops[pop].arr[kind], n, icell, delta, v, weight, charge, 0);
+}
+
+phx_status phx_particle_count(phx_ctx* c, int pop, int kind, size_t* n)
+{
+    PHX_TRY(validPop(c, pop, kind));
+    if (!n)
+        return PHX_ERR_ARG;
+    Store const& s = c->pops[pop].arr[kind];
+    if (kind == PHX_DOMAIN)
+    {
+        unsigned int dropped = 0;
+        PHX_HIP(c, hipStreamSynchronize(c->stream));
+        PHX_HIP(c, hipMemcpy(&dropped, c->dropped, 4, hipMemcpyDeviceToHost));
+        if (dropped)
+        {
+            std::vector<int32_t> ic;
+            std::vector<double> de, v, w, q;
+            PHX_TRY(download(c, s, ic, de, v, w, q));
+            *n = w.size();
+            return PHX_OK;
+        }
+    }
+    *n = s.n;
+    return PHX_OK;
+}
+
+phx_status phx_get_particles(phx_ctx* c, int pop, int kind, int32_t* icell, double* delta, double* v,
+                             double* weight, double* charge)
+{
+    PHX_TRY(validPop(c, pop, kind));
+    std::vector<int32_t> ic;
+    std::vector<double> de, vv, w, q;
+    PHX_TRY(download(c, c->pops[pop].arr[kind], ic, de, vv, w, q));
+    std::copy(ic.begin(), ic.end(), icell);
+    std::copy(de.begin(), de.end(), delta);
+    std::copy(vv.begin(), vv.end(), v);
+    std::copy(w.begin(), w.end(), weight);
+    std::copy(q.begin(), q.end(), charge);
+    return PHX_OK;
+}
+
+// Particle<dim> records: double weight, charge; int iCell[dim]; double delta[dim]; double v[3]
+phx_status phx_set_particles_aos(phx_ctx* c, int pop, int kind, size_t n, void const* records)
+{
+    PHX_TRY(validPop(c, pop, kind));
+    int const D       = c->cfg.dim;
+    size_t const rs   = recordSize(D);
+    size_t const oIc  = 16;
+    size_t const oDe  = D == 1 ? 24 : (D == 2 ? 24 : 32);
+    size_t const oV   = oDe + 8 * D;
+    auto const* bytes = static_cast<unsigned char const*>(records);
+    std::vector<int32_t> ic(n * D);
+    std::vector<double> de(n * D), v(n * 3), w(n), q(n);
+    for (size_t i = 0; i < n; ++i)
+    {
+        unsigned char const* r = bytes + i * rs;
+        std::memcpy(&w[i], r, 8);
+        std::memcpy(&q[i], r + 8, 8);
+        std::memcpy(&ic[i * D], r + oIc, 4 * D);
+        std::memcpy(&de[i * D], r + oDe, 8 * D);
+        std::memcpy(&v[i * 3], r + oV, 24);
+    }
+    return phx_set_particles(c, pop, kind, n, ic.data(), de.data(), v.data(), w.data(), q.data());
+}
+
+phx_status phx_get_particles_aos(phx_ctx* c, int pop, int kind, void* records)
+{
+    PHX_TRY(validPop(c, pop, kind));
+    int const D     = c->cfg.dim;
+    size_t const rs = recordSize(D);
+    size_t const oDe = D == 1 ? 24 : (D == 2 ? 24 : 32);
+    size_t const oV  = oDe + 8 * D;
+    std::vector<int32_t> ic;
+    std::vector<double> de, v, w, q;
+    PHX_TRY(download(c, c->pops[pop].arr[kind], ic, de, v, w, q));
+    auto* bytes = static_cast<unsigned char*>(records);
+    for (size_t i = 0; i < w.size(); ++i)
+    {
+        unsigned char* r = bytes + i * rs;
+        std::memset(r, 0, rs);
+        std::memcpy(r, &w[i], 8);
+        std::memcpy(r + 8, &q[i], 8);
+        std::memcpy(r + 16, &ic[i * D], 4 * D);
+        std::memcpy(r + oDe, &de[i * D], 8 * D);
+        std::memcpy(r + oV, &v[i * 3], 24);
+    }
+    return PHX_OK;
+}
+
+phx_status phx_load_maxwellian(phx_ctx* c, int pop, double const* n, double const* vx,
+                               double const* vy, double const* vz, double const* tx,
+                               double const* ty, double const* tz, double charge, uint32_t ppc,
+                               uint64_t seed, double cutoff)
+{
+    PHX_TRY(validPop(c, pop, PHX_DOMAIN));
+    int const D = c->cfg.dim;
+    // MaxwellianParticleInitializer::loadParticles: cells in row-major AMR order, per particle
+    // three normal draws (fresh distributions each) then dim uniform deltas in [0, 1-eps)
+    std::mt19937_64 gen(seed);
+    std::uniform_real_distribution<double> deltaDistrib{0, 1. - std::numeric_limits<double>::epsilon()};
+    std::vector<int32_t> ic;
+    std::vector<double> de, v, w, q;
+    int cell[3] = {c->patch.lower[0], c->patch.lower[1], c->patch.lower[2]};
+    size_t ncell = 1;
+    for (int d = 0; d < D; ++d)
+        ncell *= static_cast<size_t>(c->G.n[d]);
+    for (size_t flat = 0; flat < ncell; ++flat)
+    {
+        if (!(n[flat] < cutoff))
+        {
+            double const cw = n[flat] / ppc;
+            for (uint32_t ip = 0; ip < ppc; ++ip)
+            {
+                std::normal_distribution<> mx(vx[flat], tx[flat]);
+                std::normal_distribution<> my(vy[flat], ty[flat]);
+                std::normal_distribution<> mz(vz[flat], tz[flat]);
+                double const a = mx(gen);
+                double const b = my(gen);
+                double const cc = mz(gen);
+                w.push_back(cw);
+                q.push_back(charge);
+                for (int d = 0; d < D; ++d)
+                    ic.push_back(cell[d]);
+                for (int d = 0; d < D; ++d)
+                    de.push_back(deltaDistrib(gen));
+                v.push_back(a);
+                v.push_back(b);
+                v.push_back(cc);
+            }
+        }
+        // next cell, last dimension fastest
+        for (int d = D - 1; d >= 0; --d)
+        {
+            if (++cell[d] <= c->patch.upper[d])
+                break;
+            cell[d] = c->patch.lower[d];
+        }
+    }
+    Store& s = c->pops[pop].arr[PHX_DOMAIN];
+    PHX_HIP(c, hipStreamSynchronize(c->stream));
+    return upload(c, s, w.size(), ic.data(), de.data(), v.data(), w.data(), q.data(), s.n);
+}
+
+phx_status phx_update_populations(phx_ctx* c, int em_set, double dt, int mode)
+{
+    if (!c)
+        return PHX_ERR_ARG;
+    if (em_set < 0 || em_set > 2 || (mode != PHX_MODE_DOMAIN_ONLY && mode != PHX_MODE_ALL))
+        return fail(c, PHX_ERR_ARG, "bad em set or mode");
+    zeroMoments(c);
+    int const D = c->cfg.dim;
+    Wrap noWrap{};
+    for (int p = 0; p < c->cfg.n_pop; ++p)
+    {
+        Pop& P = c->pops[p];
+        Store& dom = P.arr[PHX_DOMAIN];
+        Store& pg  = P.arr[PHX_PATCH_GHOST];
+        Store& lg  = P.arr[PHX_LEVEL_GHOST];
+        if (mode == PHX_MODE_DOMAIN_ONLY)
+        {
+            // pushed copies: nothing written back
+            PushArgs A = pushArgs(c, p, KIND_DOMAIN, mode, em_set, dt);
+            launchPush(A, dom.view(), dom.view(), c->stream);
+            if (lg.n)
+            {
+                PushArgs L = pushArgs(c, p, KIND_LEVEL_GHOST, mode, em_set, dt);
+                launchPush(L, lg.view(), lg.view(), c->stream);
+            }
+            PHX_TRY(checkLaunch(c, "push"));
+            continue;
+        }
+        // mode all: existing patch ghosts are deposited unpushed
+        if (pg.n)
+        {
+            PushArgs G = pushArgs(c, p, KIND_PATCH_GHOST, mode, em_set, dt);
+            launchDeposit(G, pg.view(), c->stream);
+        }
+        size_t const nd = dom.n, nl = lg.n;
+        PHX_TRY(scratchReserve(c, std::max(nd, nl)));
+        // worst-case room: every domain particle may leave, every level ghost may enter
+        PHX_TRY(storeReserve(c, pg, pg.n + nd));
+        // domain: push in place with a fate per particle, then split keep / leave
+        PushArgs A = pushArgs(c, p, KIND_DOMAIN, mode, em_set, dt);
+        A.status   = c->status;
+        launchPush(A, dom.view(), dom.view(), c->stream);
+        PHX_TRY(checkLaunch(c, "push"));
+        PHX_TRY(checkDeviceError(c));
+        // leavers appended to patchGhost (updateAndDepositAll_ :248-254)
+        unsigned long long got = 0;
+        launchCompact(dom.view(), pg.view(), c->status, nd, ST_LEAVE, c->blockCnt, c->tot, pg.n, D,
+                      noWrap, c->stream);
+        PHX_HIP(c, hipStreamSynchronize(c->stream));
+        PHX_HIP(c, hipMemcpy(&got, c->tot, 8, hipMemcpyDeviceToHost));
+        pg.n += got;
+        // kept domain particles -> scratch, then swap (erase of the leavers, :273)
+        launchCompact(dom.view(), c->scratch.view(), c->status, nd, ST_KEEP, c->blockCnt, c->tot, 0,
+                      D, noWrap, c->stream);
+        PHX_HIP(c, hipStreamSynchronize(c->stream));
+        PHX_HIP(c, hipMemcpy(&got, c->tot, 8, hipMemcpyDeviceToHost));
+        std::swap(dom, c->scratch);
+        dom.n        = got;
+        c->scratch.n = 0;
+        if (nl)
+        {
+            // level ghosts (:275-288): pushed with inGhostBox / inGhostLayer selection; those that
+            // entered the domain were deposited by the push and join the domain array
+            PHX_TRY(storeReserve(c, dom, dom.n + nl));
+            PHX_TRY(scratchReserve(c, nl));
+            PushArgs L = pushArgs(c, p, KIND_LEVEL_GHOST, mode, em_set, dt);
+            L.status   = c->status;
+            launchPush(L, lg.view(), lg.view(), c->stream);
+            PHX_TRY(checkDeviceError(c));
+            launchCompact(lg.view(), dom.view(), c->status, nl, ST_EXPORT, c->blockCnt, c->tot, dom.n,
+                          D, noWrap, c->stream);
+            PHX_HIP(c, hipStreamSynchronize(c->stream));
+            PHX_HIP(c, hipMemcpy(&got, c->tot, 8, hipMemcpyDeviceToHost));
+            dom.n += got;
+            launchCompact(lg.view(), c->scratch.view(), c->status, nl, ST_KEEP, c->blockCnt, c->tot, 0,
+                          D, noWrap, c->stream);
+            PHX_HIP(c, hipStreamSynchronize(c->stream));
+            PHX_HIP(c, hipMemcpy(&got, c->tot, 8, hipMemcpyDeviceToHost));
+            std::swap(lg, c->scratch);
+            lg.n         = got;
+            c->scratch.n = 0;
+        }
+    }
+    return checkDeviceError(c);
+}
+
+phx_status phx_update_ions(phx_ctx* c)
+{
+    if (!c)
+        return PHX_ERR_ARG;
+    updateIons(c);
+    return checkLaunch(c, "updateIons");
+}
+
+phx_status phx_fill_moment_borders(phx_ctx* c)
+{
+    if (!c)
+        return PHX_ERR_ARG;
+    return fillMomentBorders(c);
+}
+
+phx_status phx_fill_ion_borders(phx_ctx* c)
+{
+    if (!c)
+        return PHX_ERR_ARG;
+    return fillIonBorders(c);
+}
+
+phx_status phx_fill_ion_ghost_particles(phx_ctx* c)
+{
+    if (!c)
+        return PHX_ERR_ARG;
+    int const D = c->cfg.dim;
+    for (int p = 0; p < c->cfg.n_pop; ++p)
+    {
+        Store& pg  = c->pops[p].arr[PHX_PATCH_GHOST];
+        Store& dom = c->pops[p].arr[PHX_DOMAIN];
+        if (c->patch.periodic && pg.n)
+        {
+            PHX_TRY(storeReserve(c, dom, dom.n + pg.n));
+            PHX_TRY(scratchReserve(c, pg.n));
+            PHX_HIP(c, hipMemsetAsync(c->status, ST_KEEP, pg.n, c->stream));
+            Wrap wrap{};
+            wrap.on = 1;
+            for (int d = 0; d < 3; ++d)
+            {
+                wrap.n[d]  = c->G.n[d];
+                wrap.lo[d] = c->boxes.dom_lo[d];
+                wrap.hi[d] = c->boxes.dom_hi[d];
+            }
+            launchCompact(pg.view(), dom.view(), c->status, pg.n, ST_KEEP, c->blockCnt, c->tot, dom.n,
+                          D, wrap, c->stream);
+            dom.n += pg.n;
+        }
+        pg.n = 0;
+    }
+    return checkLaunch(c, "fillIonGhostParticles");
+}
+
+phx_status phx_fill_ghosts(phx_ctx* c, int fid)
+{
+    double* p;
+    int q;
+    if (!c || !fieldOf(c, fid, p, q))
+        return fail(c, PHX_ERR_ARG, "bad field id");
+    if (!c->patch.periodic)
+        return PHX_OK;
+    for (int k = 0; k < 3; ++k)
+    {
+        double* pk;
+        int qk;
+        if (!fieldOf(c, fid + k, pk, qk))
+            return fail(c, PHX_ERR_ARG, "bad vector field id");
+        launchFillGhosts(c->G, qk, pk, c->stream);
+    }
+    return checkLaunch(c, "fillGhosts");
+}
+
+phx_status phx_faraday(phx_ctx* c, int bset, int eset, int bnewset, double dt)
+{
+    if (!c || bset < 0 || bset > 2 || eset < 0 || eset > 2 || bnewset < 0 || bnewset > 2)
+        return fail(c, PHX_ERR_ARG, "bad field set");
+    launchFaraday(c->G, cvec(c->em[bset] + 3), cvec(c->em[eset]), vec(c->em[bnewset] + 3), dt, c->stream);
+    return checkLaunch(c, "faraday");
+}
+
+phx_status phx_ampere(phx_ctx* c, int bset)
+{
+    if (!c || bset < 0 || bset > 2)
+        return fail(c, PHX_ERR_ARG, "bad field set");
+    launchAmpere(c->G, cvec(c->em[bset] + 3), vec(c->J), c->stream);
+    return checkLaunch(c, "ampere");
+}
+
+phx_status phx_electrons_update(phx_ctx* c)
+{
+    if (!c)
+        return PHX_ERR_ARG;
+    electrons(c);
+    return checkLaunch(c, "electrons");
+}
+
+phx_status phx_ohm(phx_ctx* c, int bset, int eset)
+{
+    if (!c || bset < 0 || bset > 2 || eset < 0 || eset > 2)
+        return fail(c, PHX_ERR_ARG, "bad field set");
+    ohm(c, bset, eset);
+    return checkLaunch(c, "ohm");
+}
+
+phx_status phx_average(phx_ctx* c)
+{
+    if (!c)
+        return PHX_ERR_ARG;
+    enqueueAverage(c);
+    return checkLaunch(c, "average");
+}
+
+phx_status phx_move_ions(phx_ctx* c, double dt, int mode)
+{
+    if (!c)
+        return PHX_ERR_ARG;
+    if (mode != PHX_MODE_DOMAIN_ONLY && mode != PHX_MODE_ALL)
+        return fail(c, PHX_ERR_ARG, "bad mode");
+    if (fastPath(c))
+    {
+        PHX_TRY(enqueueMoveIonsFast(c, dt, mode));
+        return checkDeviceError(c);
+    }
+    PHX_TRY(phx_update_populations(c, PHX_EM_AVG, dt, mode));
+    PHX_TRY(fillMomentBorders(c));
+    if (mode == PHX_MODE_ALL)
+        PHX_TRY(phx_fill_ion_ghost_particles(c));
+    updateIons(c);
+    PHX_TRY(fillIonBorders(c));
+    return checkDeviceError(c);
+}
+
+phx_status phx_initialize(phx_ctx* c)
+{
+    if (!c)
+        return PHX_ERR_ARG;
+    fillGhosts(c, c->em[PHX_EM] + 3, QBX);
+    zeroMoments(c);
+    for (int p = 0; p < c->cfg.n_pop; ++p)
+    {
+        PushArgs A = pushArgs(c, p, KIND_DOMAIN, PHX_MODE_ALL, PHX_EM, 0.);
+        launchDeposit(A, c->pops[p].arr[PHX_DOMAIN].view(), c->stream);
+    }
+    PHX_TRY(checkLaunch(c, "initial deposit"));
+    PHX_TRY(fillMomentBorders(c));
+    updateIons(c);
+    PHX_TRY(fillIonBorders(c));
+    launchAmpere(c->G, cvec(c->em[PHX_EM] + 3), vec(c->J), c->stream);
+    fillGhosts(c, c->J, QJX);
+    electrons(c);
+    ohm(c, PHX_EM, PHX_EM);
+    fillGhosts(c, c->em[PHX_EM], QEX);
+    PHX_TRY(checkLaunch(c, "initialize"));
+    return checkDeviceError(c);
+}
+
+phx_status phx_advance(phx_ctx* c, double dt, int nsteps)
+{
+    if (!c || nsteps < 0)
+        return PHX_ERR_ARG;
+    if (!fastPath(c))
+    {
+        // general level: the staged sequence of SolverPPC::advanceLevel
+        for (int s = 0; s < nsteps; ++s)
+        {
+            enqueuePredictor(c, dt, PHX_EM);
+            enqueueAverage(c);
+            PHX_TRY(phx_move_ions(c, dt, PHX_MODE_DOMAIN_ONLY));
+            enqueuePredictor(c, dt, PHX_EM_AVG);
+            enqueueAverage(c);
+            PHX_TRY(phx_move_ions(c, dt, PHX_MODE_ALL));
+            enqueueCorrector(c, dt);
+        }
+        return checkDeviceError(c);
+    }
+    if (!c->graph || c->graphDt != dt)
+    {
+        if (c->graph)
+        {
+            (void)hipGraphExecDestroy(c->graph);
+            c->graph = nullptr;
+        }
+        PHX_HIP(c, hipStreamSynchronize(c->stream));
+        hipGraph_t g = nullptr;
+        PHX_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        phx_status s = enqueueAdvanceFast(c, dt);
+        hipError_t e = hipStreamEndCapture(c->stream, &g);
+        if (s != PHX_OK)
+        {
+            if (g)
+                (void)hipGraphDestroy(g);
+            return s;
+        }
+        if (e != hipSuccess)
+            return fail(c, PHX_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        e = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess)
+        {
+            c->graph = nullptr;
+            return fail(c, PHX_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        }
+        c->graphDt = dt;
+    }
+    for (int s = 0; s < nsteps; ++s)
+        PHX_HIP(c, hipGraphLaunch(c->graph, c->stream));
+    return PHX_OK;
+}
+
+phx_status phx_diagnostics(phx_ctx* c, double out[4])
+{
+    if (!c || !out)
+        return PHX_ERR_ARG;
+    PHX_TRY(checkDeviceError(c));
+    int const D = c->cfg.dim;
+    double vol  = 1.;
+    for (int d = 0; d < D; ++d)
+        vol *= c->G.dx[d];
+    double kin = 0., charge = 0., cnt = 0.;
+    for (int p = 0; p < c->cfg.n_pop; ++p)
+    {
+        std::vector<int32_t> ic;
+        std::vector<double> de, v, w, q;
+        PHX_TRY(download(c, c->pops[p].arr[PHX_DOMAIN], ic, de, v, w, q));
+        double k = 0.;
+        for (size_t i = 0; i < w.size(); ++i)
+        {
+            k += w[i] * (v[i * 3] * v[i * 3] + v[i * 3 + 1] * v[i * 3 + 1] + v[i * 3 + 2] * v[i * 3 + 2]);
+            charge += q[i] * w[i];
+        }
+        kin += 0.5 * c->pops[p].mass * k;
+        cnt += static_cast<double>(w.size());
+    }
+    double mag = 0.;
+    for (int k = 0; k < 3; ++k)
+    {
+        int q = QBX + k;
+        std::vector<double> f(static_cast<size_t>(c->G.size(q)));
+        PHX_HIP(c, hipMemcpy(f.data(), c->em[PHX_EM][3 + k], f.size() * 8, hipMemcpyDeviceToHost));
+        int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+        for (int d = 0; d < D; ++d)
+        {
+            lo[d] = c->G.g;
+            hi[d] = c->G.g + c->G.n[d] - 1;
+        }
+        for (int i = lo[0]; i <= hi[0]; ++i)
+            for (int j = lo[1]; j <= hi[1]; ++j)
+                for (int kk = lo[2]; kk <= hi[2]; ++kk)
+                {
+                    double b = f[static_cast<size_t>(c->G.idx(q, i, j, kk))];
+                    mag += b * b;
+                }
+    }
+    out[0] = kin * vol;
+    out[1] = 0.5 * mag * vol;
+    out[2] = charge * vol;
+    out[3] = cnt;
+    return PHX_OK;
+}
+
+phx_status phx_profile_push(phx_ctx* c, double dt, int mode, int reps, double* avg_ms)
+{
+    if (!c || !avg_ms || reps < 1 || (mode != PHX_MODE_DOMAIN_ONLY && mode != PHX_MODE_ALL))
+        return PHX_ERR_ARG;
+    if (!fastPath(c))
+        return fail(c, PHX_ERR_STATE, "phx_profile_push needs a periodic level without ghost arrays");
+    hipEvent_t a = nullptr, b = nullptr;
+    PHX_HIP(c, hipEventCreate(&a));
+    PHX_HIP(c, hipEventCreate(&b));
+    double total = 0.;
+    for (int r = 0; r < reps; ++r)
+    {
+        PushArgs A     = pushArgs(c, 0, KIND_DOMAIN, mode, PHX_EM_AVG, dt);
+        A.periodicWrap = 1;
+        PView v        = c->pops[0].arr[PHX_DOMAIN].view();
+        PHX_HIP(c, hipEventRecord(a, c->stream));
+        launchPush(A, v, v, c->stream);
+        PHX_HIP(c, hipEventRecord(b, c->stream));
+        PHX_HIP(c, hipEventSynchronize(b));
+        float ms = 0.f;
+        PHX_HIP(c, hipEventElapsedTime(&ms, a, b));
+        total += ms;
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    *avg_ms = total / reps;
+    return checkDeviceError(c);
+}
+
+} // extern "C"

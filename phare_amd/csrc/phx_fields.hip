@@ -1,0 +1,522 @@
+// phx_fields.hip -- Yee-grid kernels for gfx950: Faraday (faraday.hpp:27-95), Ampere
+// (ampere.hpp:26-94), the electron closure (electrons.hpp:102-128, 202-212), Ohm
+// (ohm.hpp:48-269), the predictor average (algorithm.hpp:68-76), the ion totals
+// (ions.hpp:75-144) and the same-level fills of one periodic patch (ghost copy, border sum of
+// population moments, border max of the totals; hybrid_hybrid_messenger_strategy.hpp:376-497).
+//
+// One thread per node, FP64, arrays in the reference's C-order layout with ghost nodes. The
+// stencils read neighbours straight from HBM through the L1/L2 caches.
+#include "phx_kernels.hpp"
+
+namespace phx
+{
+
+namespace
+{
+constexpr int FB = 256;
+
+__device__ __forceinline__ bool nodeOf(Geom const& G, int q, long t, int& i, int& j, int& k)
+{
+    int const s1 = G.shape[q][1], s2 = G.shape[q][2];
+    long const plane = static_cast<long>(s1) * s2;
+    i                = static_cast<int>(t / plane);
+    long r           = t - static_cast<long>(i) * plane;
+    j                = static_cast<int>(r / s2);
+    k                = static_cast<int>(r - static_cast<long>(j) * s2);
+    return i < G.shape[q][0];
+}
+
+__device__ __forceinline__ int coord(int d, int i, int j, int k) { return d == 0 ? i : (d == 1 ? j : k); }
+
+// inside [lo, hi] of quantity q in the directions < dim
+__device__ __forceinline__ bool inRange(Geom const& G, int i, int j, int k, int const* lo, int const* hi)
+{
+    for (int d = 0; d < G.dim; ++d)
+    {
+        int c = coord(d, i, j, k);
+        if (c < lo[d] || c > hi[d])
+            return false;
+    }
+    return true;
+}
+
+// GridLayout::deriv (gridlayout.hpp:571-632): operand centering decides next/prev
+__device__ __forceinline__ double deriv(Geom const& G, double const* f, int q, int dir, int i, int j,
+                                        int k)
+{
+    int const c = centering(q, dir);
+    int in = i, jn = j, kn = k, ip = i, jp = j, kp = k;
+    int const up = c == CP ? 1 : 0;
+    int const dn = c == CP ? 0 : -1;
+    if (dir == 0)
+    {
+        in = i + up;
+        ip = i + dn;
+    }
+    else if (dir == 1)
+    {
+        jn = j + up;
+        jp = j + dn;
+    }
+    else
+    {
+        kn = k + up;
+        kp = k + dn;
+    }
+    return G.inv[dir] * (f[G.idx(q, in, jn, kn)] - f[G.idx(q, ip, jp, kp)]);
+}
+
+// GridLayout::laplacian (gridlayout.hpp:638-700)
+__device__ __forceinline__ double laplacian(Geom const& G, double const* f, int q, int i, int j, int k)
+{
+    double res    = 0.;
+    double const h = f[G.idx(q, i, j, k)];
+    for (int d = 0; d < G.dim; ++d)
+    {
+        double nx = f[G.idx(q, i + (d == 0), j + (d == 1), k + (d == 2))];
+        double px = f[G.idx(q, i - (d == 0), j - (d == 1), k - (d == 2))];
+        double lap = G.inv[d] * G.inv[d] * (nx - 2.0 * h + px);
+        res        = d == 0 ? lap : res + lap;
+    }
+    return res;
+}
+
+// GridLayout::project with the Yee tables (gridlayout_hybrid_yee.hpp:351-446): per direction
+// none (identity), P2D (primal->dual: offsets 0,+1) or D2P (dual->primal: -1,0), coefficients
+// 0.5 each, tensor order X outer, Z inner, accumulated from 0.
+enum : int
+{
+    PN  = 0,
+    P2D = 1,
+    D2P = 2
+};
+__device__ __forceinline__ double project(Geom const& G, double const* f, int q, int i, int j, int k,
+                                          int mx, int my, int mz)
+{
+    int m[3] = {mx, my, mz};
+    int off[3][2];
+    double cf[3][2];
+    int np[3];
+    for (int d = 0; d < 3; ++d)
+    {
+        int mode = d < G.dim ? m[d] : PN;
+        if (mode == PN)
+        {
+            np[d]     = 1;
+            off[d][0] = 0;
+            cf[d][0]  = 1.0;
+        }
+        else
+        {
+            np[d]     = 2;
+            int base  = mode == P2D ? 1 : 0;
+            off[d][0] = base - 1;
+            off[d][1] = base;
+            cf[d][0]  = 0.5;
+            cf[d][1]  = 0.5;
+        }
+    }
+    double r = 0.;
+    for (int a = 0; a < np[0]; ++a)
+        for (int b = 0; b < np[1]; ++b)
+            for (int c = 0; c < np[2]; ++c)
+                r += cf[0][a] * cf[1][b] * cf[2][c]
+                     * f[G.idx(q, i + off[0][a], j + off[1][b], k + off[2][c])];
+    return r;
+}
+
+__device__ __forceinline__ void physBox(Geom const& G, int q, int* lo, int* hi)
+{
+    for (int d = 0; d < 3; ++d)
+    {
+        lo[d] = d < G.dim ? G.g : 0;
+        hi[d] = d < G.dim ? G.physEnd(q, d) : 0;
+    }
+}
+
+// ---- Faraday ------------------------------------------------------------------------------
+__global__ __launch_bounds__(FB) void k_faraday(Geom G, CVecPtr B, CVecPtr E, VecPtr Bn, double dt)
+{
+    long const t = static_cast<long>(blockIdx.x) * FB + threadIdx.x;
+    int i, j, k;
+    // iterate over the primal-in-every-direction shape (the largest), each component on its box
+    if (!nodeOf(G, QMOM, t, i, j, k))
+        return;
+    int const D = G.dim;
+    int lo[3], hi[3];
+    physBox(G, QBX, lo, hi);
+    if (inRange(G, i, j, k, lo, hi))
+    {
+        int const n = G.idx(QBX, i, j, k);
+        double r;
+        if (D == 1)
+            r = B.c[0][n];
+        else if (D == 2)
+            r = B.c[0][n] - dt * deriv(G, E.c[2], QEZ, 1, i, j, k);
+        else
+            r = B.c[0][n] - dt * deriv(G, E.c[2], QEZ, 1, i, j, k) + dt * deriv(G, E.c[1], QEY, 2, i, j, k);
+        Bn.c[0][n] = r;
+    }
+    physBox(G, QBY, lo, hi);
+    if (inRange(G, i, j, k, lo, hi))
+    {
+        int const n = G.idx(QBY, i, j, k);
+        double r;
+        if (D < 3)
+            r = B.c[1][n] + dt * deriv(G, E.c[2], QEZ, 0, i, j, k);
+        else
+            r = B.c[1][n] - dt * deriv(G, E.c[0], QEX, 2, i, j, k) + dt * deriv(G, E.c[2], QEZ, 0, i, j, k);
+        Bn.c[1][n] = r;
+    }
+    physBox(G, QBZ, lo, hi);
+    if (inRange(G, i, j, k, lo, hi))
+    {
+        int const n = G.idx(QBZ, i, j, k);
+        double r;
+        if (D == 1)
+            r = B.c[2][n] - dt * deriv(G, E.c[1], QEY, 0, i, j, k);
+        else
+            r = B.c[2][n] - dt * deriv(G, E.c[1], QEY, 0, i, j, k) + dt * deriv(G, E.c[0], QEX, 1, i, j, k);
+        Bn.c[2][n] = r;
+    }
+}
+
+// ---- Ampere (ghost box shrunk by one) -------------------------------------------------------
+__global__ __launch_bounds__(FB) void k_ampere(Geom G, CVecPtr B, VecPtr J)
+{
+    long const t = static_cast<long>(blockIdx.x) * FB + threadIdx.x;
+    int i, j, k;
+    if (!nodeOf(G, QMOM, t, i, j, k))
+        return;
+    int const D = G.dim;
+    for (int c = 0; c < 3; ++c)
+    {
+        int q = QJX + c;
+        int lo[3], hi[3];
+        for (int d = 0; d < 3; ++d)
+        {
+            lo[d] = d < D ? 1 : 0;
+            hi[d] = d < D ? G.ghostEnd(q, d) - 1 : 0;
+        }
+        if (!inRange(G, i, j, k, lo, hi))
+            continue;
+        double r;
+        if (c == 0)
+        {
+            if (D == 1)
+                r = 0.0;
+            else if (D == 2)
+                r = deriv(G, B.c[2], QBZ, 1, i, j, k);
+            else
+                r = deriv(G, B.c[2], QBZ, 1, i, j, k) - deriv(G, B.c[1], QBY, 2, i, j, k);
+        }
+        else if (c == 1)
+        {
+            if (D < 3)
+                r = -deriv(G, B.c[2], QBZ, 0, i, j, k);
+            else
+                r = deriv(G, B.c[0], QBX, 2, i, j, k) - deriv(G, B.c[2], QBZ, 0, i, j, k);
+        }
+        else
+        {
+            if (D == 1)
+                r = deriv(G, B.c[1], QBY, 0, i, j, k);
+            else
+                r = deriv(G, B.c[1], QBY, 0, i, j, k) - deriv(G, B.c[0], QBX, 1, i, j, k);
+        }
+        J.c[c][G.idx(q, i, j, k)] = r;
+    }
+}
+
+// ---- electrons: Ve on the physical primal box, Pe everywhere ---------------------------------
+__global__ __launch_bounds__(FB) void k_electrons(Geom G, double Te, CVecPtr J, CVecPtr Vi,
+                                                   double const* Ne, VecPtr Ve, double* Pe)
+{
+    long const t = static_cast<long>(blockIdx.x) * FB + threadIdx.x;
+    int i, j, k;
+    if (!nodeOf(G, QMOM, t, i, j, k))
+        return;
+    int const n = G.idx(QMOM, i, j, k);
+    Pe[n]       = Ne[n] * Te;
+    int lo[3], hi[3];
+    physBox(G, QMOM, lo, hi);
+    if (!inRange(G, i, j, k, lo, hi))
+        return;
+    // JxToMoments / JyToMoments / JzToMoments: dual->primal along the component's direction
+    double const jx = project(G, J.c[0], QJX, i, j, k, D2P, PN, PN);
+    double const jy = project(G, J.c[1], QJY, i, j, k, PN, D2P, PN);
+    double const jz = project(G, J.c[2], QJZ, i, j, k, PN, PN, D2P);
+    Ve.c[0][n]      = Vi.c[0][n] - jx / Ne[n];
+    Ve.c[1][n]      = Vi.c[1][n] - jy / Ne[n];
+    Ve.c[2][n]      = Vi.c[2][n] - jz / Ne[n];
+}
+
+// ---- Ohm --------------------------------------------------------------------------------------
+__global__ __launch_bounds__(FB) void k_ohm(Geom G, double eta, double nu, int hyper, double lvlCoef,
+                                            double const* nn, CVecPtr Ve, double const* Pe, CVecPtr B,
+                                            CVecPtr J, VecPtr E)
+{
+    long const t = static_cast<long>(blockIdx.x) * FB + threadIdx.x;
+    int i, j, k;
+    if (!nodeOf(G, QMOM, t, i, j, k))
+        return;
+    int const D = G.dim;
+    for (int c = 0; c < 3; ++c)
+    {
+        int const q = QEX + c;
+        int lo[3], hi[3];
+        physBox(G, q, lo, hi);
+        if (!inRange(G, i, j, k, lo, hi))
+            continue;
+        // moments -> E_c: primal->dual along c
+        int const mx = c == 0 ? P2D : PN, my = c == 1 ? P2D : PN, mz = c == 2 ? P2D : PN;
+        double ideal, press = 0., bxE, byE, bzE;
+        if (c == 0)
+        {
+            double const vy = project(G, Ve.c[1], QMOM, i, j, k, mx, my, mz);
+            double const vz = project(G, Ve.c[2], QMOM, i, j, k, mx, my, mz);
+            byE             = project(G, B.c[1], QBY, i, j, k, PN, PN, D2P); // ByToEx
+            bzE             = project(G, B.c[2], QBZ, i, j, k, PN, D2P, PN); // BzToEx
+            ideal           = -vy * bzE + vz * byE;
+        }
+        else if (c == 1)
+        {
+            double const vx = project(G, Ve.c[0], QMOM, i, j, k, mx, my, mz);
+            double const vz = project(G, Ve.c[2], QMOM, i, j, k, mx, my, mz);
+            bxE             = project(G, B.c[0], QBX, i, j, k, PN, PN, D2P); // BxToEy
+            bzE             = project(G, B.c[2], QBZ, i, j, k, D2P, PN, PN); // BzToEy
+            ideal           = -vz * bxE + vx * bzE;
+        }
+        else
+        {
+            double const vx = project(G, Ve.c[0], QMOM, i, j, k, mx, my, mz);
+            double const vy = project(G, Ve.c[1], QMOM, i, j, k, mx, my, mz);
+            bxE             = project(G, B.c[0], QBX, i, j, k, PN, D2P, PN); // BxToEz
+            byE             = project(G, B.c[1], QBY, i, j, k, D2P, PN, PN); // ByToEz
+            ideal           = -vx * byE + vy * bxE;
+        }
+        if (c < D)
+        {
+            double const nOnE = project(G, nn, QMOM, i, j, k, mx, my, mz);
+            double const grad = deriv(G, Pe, QMOM, c, i, j, k);
+            press             = -grad / nOnE;
+        }
+        int const n       = G.idx(q, i, j, k);
+        double const res  = eta * (0. + 1.0 * J.c[c][n]);
+        double const lapJ = laplacian(G, J.c[c], QJX + c, i, j, k);
+        double hyp;
+        if (hyper == 0)
+            hyp = -nu * lapJ;
+        else
+        {
+            // BxToE_c, ByToE_c, BzToE_c and momentsToE_c (gridlayout_hybrid_yee.hpp:450-745)
+            double bx, by, bz;
+            if (c == 0)
+            {
+                bx = project(G, B.c[0], QBX, i, j, k, P2D, D2P, D2P);
+                by = project(G, B.c[1], QBY, i, j, k, PN, PN, D2P);
+                bz = project(G, B.c[2], QBZ, i, j, k, PN, D2P, PN);
+            }
+            else if (c == 1)
+            {
+                bx = project(G, B.c[0], QBX, i, j, k, PN, PN, D2P);
+                by = project(G, B.c[1], QBY, i, j, k, D2P, P2D, D2P);
+                bz = project(G, B.c[2], QBZ, i, j, k, D2P, PN, PN);
+            }
+            else
+            {
+                bx = project(G, B.c[0], QBX, i, j, k, PN, D2P, PN);
+                by = project(G, B.c[1], QBY, i, j, k, D2P, PN, PN);
+                bz = project(G, B.c[2], QBZ, i, j, k, D2P, D2P, P2D);
+            }
+            double const nOnE = project(G, nn, QMOM, i, j, k, mx, my, mz);
+            double const b    = sqrt(bx * bx + by * by + bz * bz);
+            hyp               = -nu * (b / (nOnE + 0.1) + 1) * lvlCoef * lapJ;
+        }
+        E.c[c][n] = ideal + press + res + hyp;
+    }
+}
+
+// ---- elementwise ------------------------------------------------------------------------------
+__global__ __launch_bounds__(FB) void k_average(double const* a, double const* b, double* o, long n)
+{
+    long const t = static_cast<long>(blockIdx.x) * FB + threadIdx.x;
+    if (t < n)
+        o[t] = (a[t] + b[t]) * .5;
+}
+
+__global__ __launch_bounds__(FB) void k_zero(double* p, long n)
+{
+    long const t = static_cast<long>(blockIdx.x) * FB + threadIdx.x;
+    if (t < n)
+        p[t] = 0.;
+}
+
+__global__ __launch_bounds__(FB) void k_copy(double const* s, double* d, long n)
+{
+    long const t = static_cast<long>(blockIdx.x) * FB + threadIdx.x;
+    if (t < n)
+        d[t] = s[t];
+}
+
+__global__ __launch_bounds__(FB) void k_ion_totals(PopMoments P, double* rho, double* md, VecPtr Vi,
+                                                   long n)
+{
+    long const t = static_cast<long>(blockIdx.x) * FB + threadIdx.x;
+    if (t >= n)
+        return;
+    double r = 0., m = 0., vx = 0., vy = 0., vz = 0.;
+    for (int p = 0; p < P.npop; ++p)
+        r = r + P.rho[p][t];
+    for (int p = 0; p < P.npop; ++p)
+        m = m + P.n[p][t] * P.mass[p];
+    for (int p = 0; p < P.npop; ++p)
+    {
+        vx = vx + P.F[p][0][t] * P.mass[p];
+        vy = vy + P.F[p][1][t] * P.mass[p];
+        vz = vz + P.F[p][2][t] * P.mass[p];
+    }
+    rho[t]     = r;
+    md[t]      = m;
+    Vi.c[0][t] = vx / m;
+    Vi.c[1][t] = vy / m;
+    Vi.c[2][t] = vz / m;
+}
+
+// ---- periodic same-level fills ----------------------------------------------------------------
+// pure ghost nodes take the value of the periodic interior node (interior not overwritten)
+__global__ __launch_bounds__(FB) void k_fill_ghosts(Geom G, int q, double* f)
+{
+    long const t = static_cast<long>(blockIdx.x) * FB + threadIdx.x;
+    int i, j, k;
+    if (!nodeOf(G, q, t, i, j, k))
+        return;
+    int src[3] = {i, j, k};
+    bool ghost = false;
+    for (int d = 0; d < G.dim; ++d)
+    {
+        int c = src[d];
+        if (c < G.g)
+        {
+            src[d] = c + G.n[d];
+            ghost  = true;
+        }
+        else if (c > G.physEnd(q, d))
+        {
+            src[d] = c - G.n[d];
+            ghost  = true;
+        }
+    }
+    if (ghost)
+        f[G.idx(q, i, j, k)] = f[G.idx(q, src[0], src[1], src[2])];
+}
+
+// image k of {-1,0,1}^dim (lexicographic, zero excluded) overlaps the all-primal ghost box on
+// [max(0, k*N), min(end, end + k*N)]; node n receives orig[n - k*N]
+template<bool MAX>
+__global__ __launch_bounds__(FB) void k_border(Geom G, double const* orig, double* out)
+{
+    long const t = static_cast<long>(blockIdx.x) * FB + threadIdx.x;
+    int i, j, k;
+    if (!nodeOf(G, QMOM, t, i, j, k))
+        return;
+    int const c[3] = {i, j, k};
+    int total      = 1;
+    for (int d = 0; d < G.dim; ++d)
+        total *= 3;
+    int const n = G.idx(QMOM, i, j, k);
+    double v    = orig[n];
+    for (int im = 0; im < total; ++im)
+    {
+        int kk[3] = {0, 0, 0};
+        int r     = im;
+        bool zero = true;
+        for (int d = G.dim - 1; d >= 0; --d)
+        {
+            kk[d] = (r % 3) - 1;
+            r /= 3;
+            if (kk[d])
+                zero = false;
+        }
+        if (zero)
+            continue;
+        bool in = true;
+        int s[3] = {i, j, k};
+        for (int d = 0; d < G.dim; ++d)
+        {
+            int const end = G.ghostEnd(QMOM, d);
+            int const sh  = kk[d] * G.n[d];
+            int const lo  = sh > 0 ? sh : 0;
+            int const hi  = end + sh < end ? end + sh : end;
+            if (c[d] < lo || c[d] > hi)
+            {
+                in = false;
+                break;
+            }
+            s[d] = c[d] - sh;
+        }
+        if (!in)
+            continue;
+        double const o = orig[G.idx(QMOM, s[0], s[1], s[2])];
+        if (MAX)
+            v = v < o ? o : v;
+        else
+            v += o;
+    }
+    out[n] = v;
+}
+
+unsigned int gridFor(long n) { return static_cast<unsigned int>((n + FB - 1) / FB); }
+
+} // namespace
+
+void launchFaraday(Geom const& G, CVecPtr B, CVecPtr E, VecPtr Bnew, double dt, hipStream_t s)
+{
+    k_faraday<<<gridFor(G.size(QMOM)), FB, 0, s>>>(G, B, E, Bnew, dt);
+}
+void launchAmpere(Geom const& G, CVecPtr B, VecPtr J, hipStream_t s)
+{
+    k_ampere<<<gridFor(G.size(QMOM)), FB, 0, s>>>(G, B, J);
+}
+void launchElectrons(Geom const& G, double Te, CVecPtr J, CVecPtr Vi, double const* Ne, VecPtr Ve,
+                     double* Pe, hipStream_t s)
+{
+    k_electrons<<<gridFor(G.size(QMOM)), FB, 0, s>>>(G, Te, J, Vi, Ne, Ve, Pe);
+}
+void launchOhm(Geom const& G, double eta, double nu, int hyper, int level, double const* n,
+               CVecPtr Ve, double const* Pe, CVecPtr B, CVecPtr J, VecPtr E, hipStream_t s)
+{
+    double lvlCoef = 1.;
+    for (int l = 0; l < level; ++l)
+        lvlCoef *= 4.;
+    lvlCoef = 1. / lvlCoef;
+    k_ohm<<<gridFor(G.size(QMOM)), FB, 0, s>>>(G, eta, nu, hyper, lvlCoef, n, Ve, Pe, B, J, E);
+}
+void launchAverage(double const* a, double const* b, double* out, long count, hipStream_t s)
+{
+    k_average<<<gridFor(count), FB, 0, s>>>(a, b, out, count);
+}
+void launchZero(double* p, long count, hipStream_t s) { k_zero<<<gridFor(count), FB, 0, s>>>(p, count); }
+void launchCopy(double const* src, double* dst, long count, hipStream_t s)
+{
+    k_copy<<<gridFor(count), FB, 0, s>>>(src, dst, count);
+}
+void launchFillGhosts(Geom const& G, int q, double* f, hipStream_t s)
+{
+    k_fill_ghosts<<<gridFor(G.size(q)), FB, 0, s>>>(G, q, f);
+}
+void launchBorderSum(Geom const& G, double const* orig, double* out, hipStream_t s)
+{
+    k_border<false><<<gridFor(G.size(QMOM)), FB, 0, s>>>(G, orig, out);
+}
+void launchBorderMax(Geom const& G, double const* orig, double* inout, hipStream_t s)
+{
+    k_border<true><<<gridFor(G.size(QMOM)), FB, 0, s>>>(G, orig, inout);
+}
+void launchIonTotals(PopMoments const& P, double* rho, double* massDensity, VecPtr Vi, long count,
+                     hipStream_t s)
+{
+    k_ion_totals<<<gridFor(count), FB, 0, s>>>(P, rho, massDensity, Vi, count);
+}
+
+} // namespace phx

@@ -1,0 +1,116 @@
+// phx_kernels.hpp -- kernel argument blocks and launchers shared by the HIP translation units.
+#pragma once
+
+#include "phx_device.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace phx
+{
+
+constexpr int DEAD_CELL = -0x40000000; // iCell[0] of a particle lost beyond the ghost box
+constexpr int MAX_NLGB  = 27;
+
+enum : int
+{
+    KIND_DOMAIN = 0,
+    KIND_PATCH_GHOST = 1,
+    KIND_LEVEL_GHOST = 2
+};
+enum : int
+{
+    MODE_DOMAIN_ONLY = 1,
+    MODE_ALL         = 2
+};
+// per-particle fate after a push in mode all
+enum : uint8_t
+{
+    ST_KEEP   = 0, // stays in its array
+    ST_LEAVE  = 1, // domain particle now in the patch ghost layer -> patchGhost
+    ST_DROP   = 2, // outside every box that keeps it
+    ST_EXPORT = 3  // level-ghost particle that entered the domain -> domain
+};
+
+// structure-of-arrays view of a particle array
+struct PView
+{
+    int32_t* ic[3];
+    double* de[3];
+    double* v[3];
+    double* w;
+    double* q;
+};
+
+struct Boxes
+{
+    int dom_lo[3], dom_hi[3]; // AMR domain box
+    int gh_lo[3], gh_hi[3];   // domain grown by the particle ghost width
+    int nnlgb;
+    int nlgb_lo[MAX_NLGB][3], nlgb_hi[MAX_NLGB][3];
+};
+
+struct PushArgs
+{
+    Geom G;
+    EMPtrs em;
+    double* mom[5]; // n, rho, Fx, Fy, Fz of the population
+    double hdtdx[3];
+    double dto2m;
+    double coef;
+    Boxes box;
+    int kind, mode, periodicWrap, pop;
+    unsigned long long n;
+    DevError* err;
+    uint8_t* status;
+    unsigned int* dropped;
+};
+
+struct Wrap
+{
+    int on;
+    int n[3], lo[3], hi[3];
+};
+
+void launchPush(PushArgs const& A, PView const& in, PView const& out, hipStream_t s);
+void launchDeposit(PushArgs const& A, PView const& in, hipStream_t s);
+unsigned int compactBlocks(unsigned long long n);
+void launchCompact(PView const& src, PView const& dst, uint8_t const* status, unsigned long long n,
+                   uint8_t want, unsigned int* blockCounts, unsigned long long* total,
+                   unsigned long long dstOffset, int dim, Wrap const& wrap, hipStream_t s);
+
+// ---- fields ----------------------------------------------------------------------------------
+struct VecPtr
+{
+    double* c[3];
+};
+struct CVecPtr
+{
+    double const* c[3];
+};
+
+void launchFaraday(Geom const& G, CVecPtr B, CVecPtr E, VecPtr Bnew, double dt, hipStream_t s);
+void launchAmpere(Geom const& G, CVecPtr B, VecPtr J, hipStream_t s);
+void launchElectrons(Geom const& G, double Te, CVecPtr J, CVecPtr Vi, double const* Ne, VecPtr Ve,
+                     double* Pe, hipStream_t s);
+void launchOhm(Geom const& G, double eta, double nu, int hyper, int level, double const* n,
+               CVecPtr Ve, double const* Pe, CVecPtr B, CVecPtr J, VecPtr E, hipStream_t s);
+void launchAverage(double const* a, double const* b, double* out, long count, hipStream_t s);
+void launchFillGhosts(Geom const& G, int q, double* f, hipStream_t s);
+void launchBorderSum(Geom const& G, double const* orig, double* out, hipStream_t s);
+void launchBorderMax(Geom const& G, double const* orig, double* inout, hipStream_t s);
+void launchZero(double* p, long count, hipStream_t s);
+void launchCopy(double const* src, double* dst, long count, hipStream_t s);
+struct PopMoments
+{
+    double const* n[4];
+    double const* rho[4];
+    double const* F[4][3];
+    double mass[4];
+    int npop;
+};
+void launchIonTotals(PopMoments const& P, double* rho, double* massDensity, VecPtr Vi, long count,
+                     hipStream_t s);
+
+} // namespace phx

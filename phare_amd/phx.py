@@ -1,0 +1,260 @@
+"""ctypes binding of include/phx.h (libphx.so, built by phare_amd/build.py).
+
+Loading fails loudly when the HIP library is missing: there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libphx.so")
+
+PHX_OK = 0
+PRECISION_FP64, PRECISION_FP32 = 0, 1
+MODE_DOMAIN_ONLY, MODE_ALL = 1, 2
+DOMAIN, PATCH_GHOST, LEVEL_GHOST = 0, 1, 2
+EM, EM_PRED, EM_AVG = 0, 1, 2
+F_E, F_B, F_EPRED, F_BPRED, F_EAVG, F_BAVG = 0, 3, 6, 9, 12, 15
+F_J, F_VE, F_PE, F_RHO, F_MASSD, F_VI, F_POP0 = 18, 21, 24, 25, 26, 27, 32
+MAX_POP = 4
+
+
+def f_pop(pop, k):
+    return F_POP0 + 8 * pop + k
+
+
+class PhxError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"phx status {status}: {msg}")
+        self.status = status
+
+
+class Config(C.Structure):
+    _fields_ = [("dim", C.c_int), ("interp_order", C.c_int), ("precision", C.c_int),
+                ("n_pop", C.c_int), ("mass", C.c_double * MAX_POP), ("eta", C.c_double),
+                ("nu", C.c_double), ("hyper_mode", C.c_int), ("Te", C.c_double)]
+
+
+class Patch(C.Structure):
+    _fields_ = [("lower", C.c_int * 3), ("upper", C.c_int * 3), ("dx", C.c_double * 3),
+                ("level", C.c_int), ("periodic", C.c_int), ("n_nonlevel_ghost_boxes", C.c_int),
+                ("nonlevel_ghost_lower", (C.c_int * 3) * 27),
+                ("nonlevel_ghost_upper", (C.c_int * 3) * 27)]
+
+
+_lib = None
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"HIP library missing: {LIB_PATH} (run `python -m phare_amd.build`)")
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        L.phx_create.argtypes = [C.POINTER(Config), C.POINTER(Patch), C.c_size_t, C.POINTER(vp)]
+        L.phx_destroy.argtypes = [vp]
+        L.phx_destroy.restype = None
+        L.phx_last_error.argtypes = [vp]
+        L.phx_last_error.restype = C.c_char_p
+        L.phx_synchronize.argtypes = [vp]
+        L.phx_stream.argtypes = [vp]
+        L.phx_stream.restype = vp
+        L.phx_field_shape.argtypes = [vp, C.c_int, _ip]
+        L.phx_set_field.argtypes = [vp, C.c_int, _dp]
+        L.phx_get_field.argtypes = [vp, C.c_int, _dp]
+        L.phx_set_particles.argtypes = [vp, C.c_int, C.c_int, C.c_size_t, _ip, _dp, _dp, _dp, _dp]
+        L.phx_particle_count.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_size_t)]
+        L.phx_get_particles.argtypes = [vp, C.c_int, C.c_int, _ip, _dp, _dp, _dp, _dp]
+        L.phx_set_particles_aos.argtypes = [vp, C.c_int, C.c_int, C.c_size_t, vp]
+        L.phx_get_particles_aos.argtypes = [vp, C.c_int, C.c_int, vp]
+        L.phx_load_maxwellian.argtypes = [vp, C.c_int] + [_dp] * 7 + [C.c_double, C.c_uint32,
+                                                                      C.c_uint64, C.c_double]
+        L.phx_update_populations.argtypes = [vp, C.c_int, C.c_double, C.c_int]
+        for n in ("phx_update_ions", "phx_fill_moment_borders", "phx_fill_ion_borders",
+                  "phx_fill_ion_ghost_particles", "phx_electrons_update", "phx_average",
+                  "phx_initialize"):
+            getattr(L, n).argtypes = [vp]
+        L.phx_fill_ghosts.argtypes = [vp, C.c_int]
+        L.phx_faraday.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_double]
+        L.phx_ampere.argtypes = [vp, C.c_int]
+        L.phx_ohm.argtypes = [vp, C.c_int, C.c_int]
+        L.phx_move_ions.argtypes = [vp, C.c_double, C.c_int]
+        L.phx_advance.argtypes = [vp, C.c_double, C.c_int]
+        L.phx_diagnostics.argtypes = [vp, _dp]
+        L.phx_profile_push.argtypes = [vp, C.c_double, C.c_int, C.c_int, C.POINTER(C.c_double)]
+        _lib = L
+    return _lib
+
+
+EXPORTED = ["phx_create", "phx_destroy", "phx_last_error", "phx_synchronize", "phx_field_shape",
+            "phx_set_field", "phx_get_field", "phx_set_particles", "phx_particle_count",
+            "phx_get_particles", "phx_set_particles_aos", "phx_get_particles_aos",
+            "phx_load_maxwellian", "phx_update_populations", "phx_update_ions",
+            "phx_fill_moment_borders", "phx_fill_ion_borders", "phx_fill_ion_ghost_particles",
+            "phx_fill_ghosts", "phx_faraday", "phx_ampere", "phx_electrons_update", "phx_ohm",
+            "phx_average", "phx_move_ions", "phx_initialize", "phx_advance", "phx_diagnostics",
+            "phx_stream", "phx_profile_push"]
+
+
+class Level:
+    """One patch level resident on the GPU (a periodic level: one patch covering the domain)."""
+
+    def __init__(self, dim, order, ncells, dx, lower=None, masses=(1.0,), Te=0.1, eta=0.0, nu=1e-4,
+                 hyper_mode=0, level=0, periodic=True, capacity=0, precision=PRECISION_FP64):
+        L = lib()
+        self.dim, self.order = dim, order
+        lower = list(lower or [0] * dim)
+        cfg = Config()
+        cfg.dim, cfg.interp_order, cfg.precision, cfg.n_pop = dim, order, precision, len(masses)
+        for i, m in enumerate(masses):
+            cfg.mass[i] = m
+        cfg.eta, cfg.nu, cfg.hyper_mode, cfg.Te = eta, nu, hyper_mode, Te
+        p = Patch()
+        for d in range(dim):
+            p.lower[d] = lower[d]
+            p.upper[d] = lower[d] + ncells[d] - 1
+            p.dx[d] = dx[d]
+        p.level, p.periodic, p.n_nonlevel_ghost_boxes = level, int(periodic), 0
+        h = C.c_void_p()
+        rc = L.phx_create(C.byref(cfg), C.byref(p), int(capacity), C.byref(h))
+        if rc != PHX_OK:
+            raise PhxError(rc, "phx_create failed")
+        self.h = h
+        self.npop = len(masses)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().phx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _rc(self, rc):
+        if rc != PHX_OK:
+            raise PhxError(rc, lib().phx_last_error(self.h).decode())
+
+    def stream(self):
+        return lib().phx_stream(self.h)
+
+    def sync(self):
+        self._rc(lib().phx_synchronize(self.h))
+
+    def shape(self, fid):
+        s = np.zeros(3, dtype=np.int32)
+        self._rc(lib().phx_field_shape(self.h, fid, s))
+        return tuple(int(x) for x in s[: self.dim])
+
+    def get(self, fid):
+        a = np.zeros(self.shape(fid))
+        self._rc(lib().phx_get_field(self.h, fid, a.reshape(-1)))
+        return a
+
+    def set(self, fid, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        assert a.shape == self.shape(fid), (a.shape, self.shape(fid))
+        self._rc(lib().phx_set_field(self.h, fid, a.reshape(-1)))
+
+    def set_particles(self, pop, kind, p):
+        n = len(p["weight"])
+        self._rc(lib().phx_set_particles(
+            self.h, pop, kind, n, np.ascontiguousarray(p["icell"], dtype=np.int32).reshape(-1),
+            np.ascontiguousarray(p["delta"], dtype=np.float64).reshape(-1),
+            np.ascontiguousarray(p["v"], dtype=np.float64).reshape(-1),
+            np.ascontiguousarray(p["weight"], dtype=np.float64),
+            np.ascontiguousarray(p["charge"], dtype=np.float64)))
+
+    def count(self, pop=0, kind=DOMAIN):
+        n = C.c_size_t(0)
+        self._rc(lib().phx_particle_count(self.h, pop, kind, C.byref(n)))
+        return int(n.value)
+
+    def get_particles(self, pop=0, kind=DOMAIN):
+        n = self.count(pop, kind)
+        p = dict(icell=np.zeros((n, self.dim), dtype=np.int32), delta=np.zeros((n, self.dim)),
+                 v=np.zeros((n, 3)), weight=np.zeros(n), charge=np.zeros(n))
+        if n:
+            self._rc(lib().phx_get_particles(self.h, pop, kind, p["icell"].reshape(-1),
+                                             p["delta"].reshape(-1), p["v"].reshape(-1), p["weight"],
+                                             p["charge"]))
+        return p
+
+    def set_particles_aos(self, pop, kind, records):
+        buf = np.ascontiguousarray(records)
+        self._rc(lib().phx_set_particles_aos(self.h, pop, kind, len(buf), buf.ctypes.data))
+
+    def get_particles_aos(self, pop, kind, dtype):
+        n = self.count(pop, kind)
+        out = np.zeros(n, dtype=dtype)
+        if n:
+            self._rc(lib().phx_get_particles_aos(self.h, pop, kind, out.ctypes.data))
+        return out
+
+    def load_maxwellian(self, pop, n, V, Vth, charge, ppc, seed, cutoff=1e-16):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64).reshape(-1) for a in [n, *V, *Vth]]
+        self._rc(lib().phx_load_maxwellian(self.h, pop, *arrs, charge, ppc, seed, cutoff))
+
+    def initialize(self):
+        self._rc(lib().phx_initialize(self.h))
+
+    def update_populations(self, dt, mode, emset=EM_AVG):
+        self._rc(lib().phx_update_populations(self.h, emset, dt, mode))
+
+    def update_ions(self):
+        self._rc(lib().phx_update_ions(self.h))
+
+    def fill_moment_borders(self):
+        self._rc(lib().phx_fill_moment_borders(self.h))
+
+    def fill_ion_borders(self):
+        self._rc(lib().phx_fill_ion_borders(self.h))
+
+    def fill_ion_ghost_particles(self):
+        self._rc(lib().phx_fill_ion_ghost_particles(self.h))
+
+    def fill_ghosts(self, fid):
+        self._rc(lib().phx_fill_ghosts(self.h, fid))
+
+    def faraday(self, bset, eset, bnewset, dt):
+        self._rc(lib().phx_faraday(self.h, bset, eset, bnewset, dt))
+
+    def ampere(self, bset):
+        self._rc(lib().phx_ampere(self.h, bset))
+
+    def electrons(self):
+        self._rc(lib().phx_electrons_update(self.h))
+
+    def ohm(self, bset, eset):
+        self._rc(lib().phx_ohm(self.h, bset, eset))
+
+    def average(self):
+        self._rc(lib().phx_average(self.h))
+
+    def move_ions(self, dt, mode):
+        self._rc(lib().phx_move_ions(self.h, dt, mode))
+
+    def advance(self, dt, nsteps=1):
+        self._rc(lib().phx_advance(self.h, dt, nsteps))
+
+    def profile_push(self, dt, mode, reps):
+        ms = C.c_double(0.0)
+        self._rc(lib().phx_profile_push(self.h, dt, mode, reps, C.byref(ms)))
+        return ms.value
+
+    def diagnostics(self):
+        out = np.zeros(4)
+        self._rc(lib().phx_diagnostics(self.h, out))
+        return dict(kinetic=out[0], magnetic=out[1], charge=out[2], count=out[3])
+
+
+def particle_dtype(dim):
+    """numpy dtype of the reference's Particle<dim> record (particle.hpp:37-61)"""
+    fields = [("weight", "<f8"), ("charge", "<f8"), ("iCell", "<i4", (dim,))]
+    if dim in (1, 3):
+        fields.append(("_pad", "<i4"))
+    fields += [("delta", "<f8", (dim,)), ("v", "<f8", (3,))]
+    return np.dtype(fields)

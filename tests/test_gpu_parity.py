@@ -1,0 +1,300 @@
+"""Parity of the HIP path (libphx.so through the C ABI) with the CPU oracle on identical seeded
+inputs. The plain FP64 path must give identical cell indices, and deltas, velocities, moments,
+E and B within 1e-12 relative (max-norm over each array); a single push is bit-exact.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+import parity_util as U
+from phare_amd import phx
+from phare_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+CASES = [(1, 1), (1, 2), (1, 3), (2, 1), (2, 2), (2, 3), (3, 1), (3, 2), (3, 3)]
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _pair(dim, order, ncells, dx, masses=(1.0,), **kw):
+    g = phx.Level(dim, order, ncells, dx, masses=masses, **kw)
+    o = O.Sim(dim, order, ncells, dx, masses=masses, **kw)
+    return g, o
+
+
+def _ncells(dim):
+    return [[23], [14, 11], [9, 10, 11]][dim - 1]
+
+
+def _random_em(rng, g, o, eset=2, scale=0.3):
+    base = {0: 0, 1: 6, 2: 12}[eset]
+    for k in range(6):
+        a = scale * rng.standard_normal(g.shape(base + k))
+        if k >= 3:
+            a += 1.0 if k == 3 else 0.4
+        g.set(base + k, a)
+        o.set(base + k, a)
+
+
+@pytest.mark.parametrize("dim,order", CASES)
+@pytest.mark.parametrize("mode", [1, 2])
+def test_update_populations_matches_oracle(dim, order, mode):
+    _say(f"[gpu] update_populations dim={dim} order={order} mode={mode}")
+    rng = np.random.default_rng(1000 + 10 * dim + order + 100 * mode)
+    nc = _ncells(dim)
+    dx = [0.3, 0.25, 0.2][:dim]
+    g, o = _pair(dim, order, nc, dx)
+    _random_em(rng, g, o)
+    pg = [1, 2, 2][order - 1]
+    p = U.random_particles(rng, dim, 3000, nc, [0] * dim, 0, vscale=1.5)
+    g.set_particles(0, phx.DOMAIN, p)
+    o.set_particles(0, 0, p)
+    dt = 0.05
+    g.update_populations(dt, mode)
+    o.update_populations(dt, mode)
+    for k in range(5):
+        U.assert_close(g.get(phx.f_pop(0, k)), o.get(O.f_pop(0, k)), what=f"moment {k}")
+    if mode == 2:
+        # a single push is bit-exact; the arrays hold the same particle sets
+        U.assert_same_particles(g.get_particles(0, phx.DOMAIN), o.get_particles(0, 0), exact=True,
+                                what="domain")
+        U.assert_same_particles(g.get_particles(0, phx.PATCH_GHOST), o.get_particles(0, 1),
+                                exact=True, what="patchGhost")
+    else:
+        # domain_only leaves the particles untouched (ion_updater.hpp:181)
+        U.assert_same_particles(g.get_particles(0, phx.DOMAIN), p, exact=True, what="untouched")
+    g.close()
+
+
+@pytest.mark.parametrize("dim,order", [(1, 1), (2, 2), (3, 1)])
+def test_level_ghost_particles_match_oracle(dim, order):
+    """level-ghost selection (inGhostBox first, inDomainBox / inGhostLayer second) and their
+    export into the domain (ion_updater.hpp:195-217, 275-288)"""
+    _say(f"[gpu] level ghosts dim={dim} order={order}")
+    rng = np.random.default_rng(2000 + 10 * dim + order)
+    nc = _ncells(dim)
+    dx = [0.3, 0.25, 0.2][:dim]
+    for mode in (1, 2):
+        g, o = _pair(dim, order, nc, dx)
+        _random_em(rng, g, o)
+        pg = [1, 2, 2][order - 1]
+        dom = U.random_particles(rng, dim, 800, nc, [0] * dim, 0, vscale=2.0)
+        # level ghosts in the ghost layer (outside the domain, inside the ghost box)
+        lg = U.random_particles(rng, dim, 2000, nc, [0] * dim, pg, vscale=2.0)
+        inside = np.all((lg["icell"] >= 0) & (lg["icell"] < np.array(nc)), axis=1)
+        lg = {k: v[~inside] for k, v in lg.items()}
+        for sim, kinds in ((g, (phx.DOMAIN, phx.LEVEL_GHOST)), (o, (0, 2))):
+            sim.set_particles(0, kinds[0], dom)
+            sim.set_particles(0, kinds[1], lg)
+        g.update_populations(0.08, mode)
+        o.update_populations(0.08, mode)
+        for k in range(5):
+            U.assert_close(g.get(phx.f_pop(0, k)), o.get(O.f_pop(0, k)), what=f"moment {k}")
+        if mode == 2:
+            U.assert_same_particles(g.get_particles(0, phx.DOMAIN), o.get_particles(0, 0), exact=True,
+                                    what="domain")
+            U.assert_same_particles(g.get_particles(0, phx.LEVEL_GHOST), o.get_particles(0, 2),
+                                    exact=True, what="levelGhost")
+        g.close()
+
+
+@pytest.mark.parametrize("dim,order", CASES)
+def test_field_operators_match_oracle(dim, order):
+    _say(f"[gpu] field operators dim={dim} order={order}")
+    rng = np.random.default_rng(3000 + 10 * dim + order)
+    nc = _ncells(dim)
+    dx = [0.3, 0.25, 0.2][:dim]
+    for hyper in (0, 1):
+        g, o = _pair(dim, order, nc, dx, eta=0.01, nu=0.02, hyper_mode=hyper, Te=0.2)
+        for eset in (0, 1, 2):
+            _random_em(rng, g, o, eset)
+        for fid in list(range(18, 24)) + [24, 26, 27, 28, 29]:
+            a = rng.standard_normal(g.shape(fid))
+            g.set(fid, a)
+            o.set(fid, a)
+        rho = 1.0 + 0.3 * rng.random(g.shape(phx.F_RHO))
+        g.set(phx.F_RHO, rho)
+        o.set(O.F_RHO, rho)
+        g.faraday(0, 2, 1, 0.07)
+        o.faraday(0, 2, 1, 0.07)
+        g.ampere(1)
+        o.ampere(1)
+        g.electrons()
+        o.electrons()
+        g.ohm(1, 1)
+        o.ohm(1, 1)
+        for fid in list(range(6, 12)) + list(range(18, 25)):
+            U.assert_close(g.get(fid), o.get(fid), rtol=0, what=f"field {fid} hyper {hyper}")
+        # in-place Faraday (corrector) and the averaged set with its ghost fill
+        g.faraday(0, 2, 0, 0.03)
+        o.faraday(0, 2, 0, 0.03)
+        g.average()
+        o.average()
+        for fid in range(0, 18):
+            U.assert_close(g.get(fid), o.get(fid), rtol=0, what=f"field {fid} after average")
+        g.close()
+
+
+@pytest.mark.parametrize("dim,order", CASES)
+def test_fills_match_oracle(dim, order):
+    _say(f"[gpu] periodic fills dim={dim} order={order}")
+    rng = np.random.default_rng(4000 + 10 * dim + order)
+    nc = _ncells(dim)
+    dx = [0.3, 0.25, 0.2][:dim]
+    g, o = _pair(dim, order, nc, dx)
+    for k in range(5):
+        a = rng.random(g.shape(phx.f_pop(0, k)))
+        g.set(phx.f_pop(0, k), a)
+        o.set(O.f_pop(0, k), a)
+    g.fill_moment_borders()
+    o.fill_moment_borders()
+    for k in range(5):
+        U.assert_close(g.get(phx.f_pop(0, k)), o.get(O.f_pop(0, k)), rtol=0, what=f"border sum {k}")
+    g.update_ions()
+    o.update_ions()
+    g.fill_ion_borders()
+    o.fill_ion_borders()
+    for fid in (25, 26, 27, 28, 29):
+        U.assert_close(g.get(fid), o.get(fid), rtol=0, what=f"ion total {fid}")
+    for base in (0, 3, 18):
+        for k in range(3):
+            a = rng.standard_normal(g.shape(base + k))
+            g.set(base + k, a)
+            o.set(base + k, a)
+        g.fill_ghosts(base)
+        for k in range(3):
+            o.fill_ghosts(base + k)
+            U.assert_close(g.get(base + k), o.get(base + k), rtol=0, what=f"ghost fill {base + k}")
+    g.close()
+
+
+def test_maxwellian_loader_bitwise():
+    """the seeded loader reproduces the oracle's particles bit for bit (same libstdc++ draws)"""
+    _say("[gpu] maxwellian loader")
+    wl = W.harris2d(ncells=(20, 16), ppc=7)
+    g = phx.Level(2, 2, wl.ncells, wl.dx)
+    o = O.Sim(2, 2, wl.ncells, wl.dx)
+    n, V, T = wl.profiles()
+    g.load_maxwellian(0, n, V, T, 1.0, wl.ppc, 12334)
+    o.load_maxwellian(0, n, V, T, 1.0, wl.ppc, 12334)
+    a, b = g.get_particles(), o.get_particles()
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
+    g.close()
+
+
+def test_aos_records_round_trip():
+    """reference Particle<dim> records in and out (particle.hpp:37-61; 56/64/80 B)"""
+    _say("[gpu] AoS round trip")
+    for dim, size in ((1, 56), (2, 64), (3, 80)):
+        dt = phx.particle_dtype(dim)
+        assert dt.itemsize == size
+        rng = np.random.default_rng(dim)
+        nc = _ncells(dim)
+        rec = np.zeros(100, dtype=dt)
+        rec["weight"] = rng.random(100)
+        rec["charge"] = 1.0
+        rec["iCell"] = rng.integers(0, 9, (100, dim))
+        rec["delta"] = rng.random((100, dim))
+        rec["v"] = rng.standard_normal((100, 3))
+        g = phx.Level(dim, 1, nc, [0.2] * dim)
+        g.set_particles_aos(0, phx.DOMAIN, rec)
+        back = g.get_particles_aos(0, phx.DOMAIN, dt)
+        for k in ("weight", "charge", "iCell", "delta", "v"):
+            assert np.array_equal(back[k], rec[k]), (dim, k)
+        g.close()
+
+
+def test_two_cell_move_is_an_error():
+    """boris.hpp:164-165 -> DictionaryException "Updater::updatePopulations"
+    (tests/simulator/test_exceptions.py: v=1000, dt=0.001, dl=0.3)"""
+    _say("[gpu] two-cell move error")
+    g = phx.Level(1, 1, [20], [0.3])
+    p = dict(icell=np.array([[5]], np.int32), delta=np.array([[0.5]]), v=np.array([[1000.0, 0, 0]]),
+             weight=np.ones(1), charge=np.ones(1))
+    g.set_particles(0, phx.DOMAIN, p)
+    with pytest.raises(phx.PhxError) as e:
+        g.update_populations(0.002, phx.MODE_ALL)
+    assert "moved 2 cells" in str(e.value)
+    g.close()
+
+
+def _run_pair(wl, steps):
+    g = phx.Level(wl.dim, wl.order, wl.ncells, wl.dx, masses=wl.masses, Te=wl.Te,
+                  capacity=wl.n_particles)
+    o = O.Sim(wl.dim, wl.order, wl.ncells, wl.dx, masses=wl.masses, Te=wl.Te)
+    wl.setup(g, phx.F_B)
+    wl.setup(o, O.F_B)
+    for fid in list(range(0, 6)) + [18, 19, 20, 25]:
+        U.assert_close(g.get(fid), o.get(fid), what=f"initial field {fid}")
+    g.advance(wl.dt, steps)
+    o.advance(wl.dt, steps)
+    g.sync()
+    return g, o
+
+
+def _check_state(g, o, wl, rtol=U.RTOL):
+    for fid in range(0, 6):
+        U.assert_close(g.get(fid), o.get(fid), rtol, what=f"field {fid}")
+    for k in range(5):
+        U.assert_close(g.get(phx.f_pop(0, k)), o.get(O.f_pop(0, k)), rtol, what=f"moment {k}")
+    U.assert_close(g.get(phx.F_RHO), o.get(O.F_RHO), rtol, what="rho")
+    U.assert_same_particles(g.get_particles(), o.get_particles(), rtol)
+
+
+def test_advance_uniform1d_full():
+    """the 1D CPU-runnable workload in full: 200 cells, 20k particles, 100 PPC steps"""
+    _say("[gpu] advance uniform1d 100 steps")
+    wl = W.uniform1d()
+    g, o = _run_pair(wl, wl.nsteps)
+    _check_state(g, o, wl)
+    g.close()
+
+
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_advance_harris2d_small(order):
+    _say(f"[gpu] advance harris2d small order={order}")
+    wl = W.harris2d(ncells=(32, 24), ppc=12, order=order)
+    g, o = _run_pair(wl, 20)
+    _check_state(g, o, wl)
+    g.close()
+
+
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_advance_uniform3d_small(order):
+    _say(f"[gpu] advance uniform3d small order={order}")
+    wl = W.uniform3d(ncells=(10, 9, 11), ppc=6, order=order)
+    g, o = _run_pair(wl, 8)
+    _check_state(g, o, wl)
+    g.close()
+
+
+def test_move_ions_general_path_matches_fast_path():
+    """updatePopulations + fills (the drop-in sequence of moveIons_) gives the same level state
+    as the fused periodic path that wraps leavers in place"""
+    _say("[gpu] general vs fused moveIons")
+    wl = W.harris2d(ncells=(24, 18), ppc=10)
+    a = phx.Level(2, 2, wl.ncells, wl.dx)
+    b = phx.Level(2, 2, wl.ncells, wl.dx)
+    wl.setup(a, phx.F_B)
+    wl.setup(b, phx.F_B)
+    for lv in (a, b):
+        lv.average()
+    a.move_ions(0.3, phx.MODE_ALL)  # fused
+    b.update_populations(0.3, phx.MODE_ALL)
+    b.fill_moment_borders()
+    b.fill_ion_ghost_particles()
+    b.update_ions()
+    b.fill_ion_borders()
+    assert b.count(0, phx.PATCH_GHOST) == 0
+    U.assert_same_particles(a.get_particles(), b.get_particles(), rtol=0)
+    for fid in (25, 26, 27, 28, 29):
+        U.assert_close(a.get(fid), b.get(fid), what=f"total {fid}")
+    a.close()
+    b.close()
