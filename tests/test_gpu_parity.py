@@ -92,8 +92,9 @@ def test_level_ghost_particles_match_oracle(dim, order):
         for sim, kinds in ((g, (phx.DOMAIN, phx.LEVEL_GHOST)), (o, (0, 2))):
             sim.set_particles(0, kinds[0], dom)
             sim.set_particles(0, kinds[1], lg)
-        g.update_populations(0.08, mode)
-        o.update_populations(0.08, mode)
+        # dt small enough that no particle moves two cells in a half step
+        g.update_populations(0.03, mode)
+        o.update_populations(0.03, mode)
         for k in range(5):
             U.assert_close(g.get(phx.f_pop(0, k)), o.get(O.f_pop(0, k)), what=f"moment {k}")
         if mode == 2:
@@ -293,7 +294,9 @@ def test_move_ions_general_path_matches_fast_path():
     b.update_ions()
     b.fill_ion_borders()
     assert b.count(0, phx.PATCH_GHOST) == 0
-    U.assert_same_particles(a.get_particles(), b.get_particles(), rtol=0)
+    # two levels initialised apart: their deposits sum in different orders (LDS atomics), so
+    # the states agree to round-off, not bit for bit
+    U.assert_same_particles(a.get_particles(), b.get_particles())
     for fid in (25, 26, 27, 28, 29):
         U.assert_close(a.get(fid), b.get(fid), what=f"total {fid}")
     a.close()
