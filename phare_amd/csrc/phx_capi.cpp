@@ -74,6 +74,12 @@ struct phx_ctx
     double* Vi[3] = {};
     Pop pops[PHX_MAX_POP];
     Store scratch;
+    // fused periodic path: two raw deposit sets (one filled while the other is cleared) and
+    // the totals before their border max
+    double* raw[2][PHX_MAX_POP][5] = {};
+    int rawCur                     = 0;
+    int graphRawCur                = -1;
+    double* tot5[5]                = {};
 
     DevError* err           = nullptr;
     unsigned int* dropped   = nullptr;
@@ -311,6 +317,12 @@ PushArgs pushArgs(phx_ctx* c, int pop, int kind, int mode, int emSet, double dt)
     A.err          = c->err;
     A.status       = nullptr;
     A.dropped      = c->dropped;
+    A.pushOn       = 1;
+    // tile shape from the density of particles per cell of this array
+    double ppc = static_cast<double>(A.n);
+    for (int d = 0; d < c->cfg.dim; ++d)
+        ppc /= c->G.n[d];
+    tileShape(c->cfg.dim, c->cfg.interp_order, ppc, A.tileLo, A.tileExt);
     return A;
 }
 
@@ -395,55 +407,93 @@ bool fastPath(phx_ctx* c)
     return true;
 }
 
-// enqueue the particle stage of SolverPPC::moveIons_ on the fast path (no host sync)
+// enqueue the particle stage of SolverPPC::moveIons_ on the fast path (no host sync): the push
+// deposits into the current raw set, then one kernel border-sums every population moment,
+// clears the other raw set and computes the totals, and one kernel applies their border max
 phx_status enqueueMoveIonsFast(phx_ctx* c, double dt, int mode)
 {
-    zeroMoments(c);
+    int const cur = c->rawCur;
+    MomSetArgs M{};
+    M.npop = c->cfg.n_pop;
     for (int p = 0; p < c->cfg.n_pop; ++p)
     {
-        PushArgs A     = pushArgs(c, p, KIND_DOMAIN, mode, PHX_EM_AVG, dt);
+        PushArgs A = pushArgs(c, p, KIND_DOMAIN, mode, PHX_EM_AVG, dt);
+        for (int m = 0; m < 5; ++m)
+            A.mom[m] = c->raw[cur][p][m];
         A.periodicWrap = 1;
         PView v        = c->pops[p].arr[PHX_DOMAIN].view();
         launchPush(A, v, v, c->stream);
+        M.mass[p] = c->pops[p].mass;
+        for (int m = 0; m < 5; ++m)
+        {
+            M.raw[p][m]  = c->raw[cur][p][m];
+            M.mom[p][m]  = c->pops[p].mom[m];
+            M.zero[p][m] = c->raw[1 - cur][p][m];
+        }
     }
     PHX_TRY(checkLaunch(c, "push"));
-    PHX_TRY(fillMomentBorders(c));
-    updateIons(c);
-    return fillIonBorders(c);
+    launchFoldTotals(c->G, M, c->tot5[0], c->tot5[1], vec(c->tot5 + 2), c->stream);
+    double const* src[5] = {c->tot5[0], c->tot5[1], c->tot5[2], c->tot5[3], c->tot5[4]};
+    double* dst[5]       = {c->rho, c->massD, c->Vi[0], c->Vi[1], c->Vi[2]};
+    launchMax5(c->G, src, dst, c->stream);
+    c->rawCur = 1 - cur;
+    return checkLaunch(c, "moment fills");
 }
 
-// SolverPPC predictor1_/predictor2_ (solver_ppc.hpp:346-428)
+// SolverPPC predictor1_/predictor2_ (solver_ppc.hpp:346-428): Faraday writes Bpred with its
+// ghost layers, Ampere writes J with its ghost layers
 void enqueuePredictor(phx_ctx* c, double dt, int eset)
 {
-    launchFaraday(c->G, cvec(c->em[PHX_EM] + 3), cvec(c->em[eset]), vec(c->em[PHX_EM_PRED] + 3), dt,
-                  c->stream);
-    fillGhosts(c, c->em[PHX_EM_PRED] + 3, QBX);
-    launchAmpere(c->G, cvec(c->em[PHX_EM_PRED] + 3), vec(c->J), c->stream);
-    fillGhosts(c, c->J, QJX);
+    if (!c->patch.periodic)
+    {
+        // a patch of a wider level: its ghost layers come from the caller's messenger
+        launchFaraday(c->G, cvec(c->em[PHX_EM] + 3), cvec(c->em[eset]), vec(c->em[PHX_EM_PRED] + 3), dt,
+                      c->stream);
+        launchAmpere(c->G, cvec(c->em[PHX_EM_PRED] + 3), vec(c->J), c->stream);
+        electrons(c);
+        ohm(c, PHX_EM_PRED, PHX_EM_PRED);
+        return;
+    }
+    launchFaradayFill(c->G, cvec(c->em[PHX_EM] + 3), cvec(c->em[eset]), vec(c->em[PHX_EM_PRED] + 3), dt,
+                      c->stream);
+    launchAmpereFill(c->G, cvec(c->em[PHX_EM_PRED] + 3), vec(c->J), c->stream);
     electrons(c);
     ohm(c, PHX_EM_PRED, PHX_EM_PRED);
 }
 
-// SolverPPC::average_ (solver_ppc.hpp:483-510)
+// SolverPPC::average_ (solver_ppc.hpp:483-510) with the Eavg ghost fill
 void enqueueAverage(phx_ctx* c)
 {
-    for (int k = 0; k < 6; ++k)
-        launchAverage(c->em[PHX_EM][k], c->em[PHX_EM_PRED][k], c->em[PHX_EM_AVG][k],
-                      count(c, k < 3 ? QEX + k : QBX + k - 3), c->stream);
-    fillGhosts(c, c->em[PHX_EM_AVG], QEX);
+    if (!c->patch.periodic)
+    {
+        for (int k = 0; k < 6; ++k)
+            launchAverage(c->em[PHX_EM][k], c->em[PHX_EM_PRED][k], c->em[PHX_EM_AVG][k],
+                          count(c, k < 3 ? QEX + k : QBX + k - 3), c->stream);
+        return;
+    }
+    launchAverageFill(c->G, cvec(c->em[PHX_EM]), cvec(c->em[PHX_EM_PRED]), cvec(c->em[PHX_EM] + 3),
+                      cvec(c->em[PHX_EM_PRED] + 3), vec(c->em[PHX_EM_AVG]), vec(c->em[PHX_EM_AVG] + 3),
+                      c->stream);
 }
 
-// SolverPPC::corrector_ (solver_ppc.hpp:433-479)
+// SolverPPC::corrector_ (solver_ppc.hpp:433-479): in-place Faraday, then the B ghost fill;
+// Ampere and Ohm write their ghost layers themselves
 void enqueueCorrector(phx_ctx* c, double dt)
 {
     launchFaraday(c->G, cvec(c->em[PHX_EM] + 3), cvec(c->em[PHX_EM_AVG]), vec(c->em[PHX_EM] + 3), dt,
                   c->stream);
-    fillGhosts(c, c->em[PHX_EM] + 3, QBX);
-    launchAmpere(c->G, cvec(c->em[PHX_EM] + 3), vec(c->J), c->stream);
-    fillGhosts(c, c->J, QJX);
+    if (!c->patch.periodic)
+    {
+        launchAmpere(c->G, cvec(c->em[PHX_EM] + 3), vec(c->J), c->stream);
+        electrons(c);
+        ohm(c, PHX_EM, PHX_EM);
+        return;
+    }
+    launchFill3(c->G, QBX, vec(c->em[PHX_EM] + 3), c->stream);
+    launchAmpereFill(c->G, cvec(c->em[PHX_EM] + 3), vec(c->J), c->stream);
     electrons(c);
-    ohm(c, PHX_EM, PHX_EM);
-    fillGhosts(c, c->em[PHX_EM], QEX);
+    launchOhm(c->G, c->cfg.eta, c->cfg.nu, c->cfg.hyper_mode, c->patch.level, c->rho, cvec(c->Ve),
+              c->Pe, cvec(c->em[PHX_EM] + 3), cvec(c->J), vec(c->em[PHX_EM]), c->stream, 1);
 }
 
 phx_status enqueueAdvanceFast(phx_ctx* c, double dt)
@@ -467,30 +517,74 @@ phx_status validPop(phx_ctx* c, int pop, int kind)
     return PHX_OK;
 }
 
+// storage order inside each chunk of the push kernel: slot k*BLOCK + t (t = wave*64 + lane)
+// holds the particle (r*PPT + k) of the caller's order with r = lane*NW + wave, so a thread
+// owns PPT consecutive particles and the lanes of a wavefront own runs spread across the chunk;
+// a last partial chunk keeps the caller's order
+std::vector<size_t> chunkInterleave(size_t n, int dim, int order)
+{
+    size_t const C   = static_cast<size_t>(chunkSize(dim, order));
+    size_t const BT  = static_cast<size_t>(blockThreads());
+    size_t const PPT = C / BT, NWv = BT / 64;
+    std::vector<size_t> ord(n);
+    for (size_t c0 = 0; c0 < n; c0 += C)
+    {
+        size_t const m = std::min(C, n - c0);
+        for (size_t s = 0; s < m; ++s)
+        {
+            if (m < C)
+            {
+                ord[c0 + s] = c0 + s;
+                continue;
+            }
+            size_t const k = s / BT, t = s % BT;
+            size_t const wave = t / 64, lane = t % 64;
+            size_t const r    = lane * NWv + wave;
+            ord[c0 + s]       = c0 + r * PPT + k;
+        }
+    }
+    return ord;
+}
+
 phx_status upload(phx_ctx* c, Store& s, size_t n, int32_t const* ic, double const* de, double const* v,
                   double const* w, double const* q, size_t offset)
 {
     int const D = c->cfg.dim;
     PHX_TRY(storeReserve(c, s, offset + n));
+    // appended particles start a fresh chunk alignment only when the array was empty
+    std::vector<size_t> const ord = offset == 0 ? chunkInterleave(n, D, c->cfg.interp_order)
+                                                : chunkInterleave(0, D, c->cfg.interp_order);
+    std::vector<size_t> identity;
+    if (offset != 0)
+    {
+        identity.resize(n);
+        for (size_t i = 0; i < n; ++i)
+            identity[i] = i;
+    }
+    std::vector<size_t> const& o = offset == 0 ? ord : identity;
     std::vector<int32_t> ib(n);
     std::vector<double> db(n);
     for (int d = 0; d < D; ++d)
     {
         for (size_t i = 0; i < n; ++i)
-            ib[i] = ic[i * D + d];
+            ib[i] = ic[o[i] * D + d];
         PHX_HIP(c, hipMemcpy(s.ic[d] + offset, ib.data(), n * 4, hipMemcpyHostToDevice));
         for (size_t i = 0; i < n; ++i)
-            db[i] = de[i * D + d];
+            db[i] = de[o[i] * D + d];
         PHX_HIP(c, hipMemcpy(s.de[d] + offset, db.data(), n * 8, hipMemcpyHostToDevice));
     }
     for (int d = 0; d < 3; ++d)
     {
         for (size_t i = 0; i < n; ++i)
-            db[i] = v[i * 3 + d];
+            db[i] = v[o[i] * 3 + d];
         PHX_HIP(c, hipMemcpy(s.v[d] + offset, db.data(), n * 8, hipMemcpyHostToDevice));
     }
-    PHX_HIP(c, hipMemcpy(s.w + offset, w, n * 8, hipMemcpyHostToDevice));
-    PHX_HIP(c, hipMemcpy(s.q + offset, q, n * 8, hipMemcpyHostToDevice));
+    for (size_t i = 0; i < n; ++i)
+        db[i] = w[o[i]];
+    PHX_HIP(c, hipMemcpy(s.w + offset, db.data(), n * 8, hipMemcpyHostToDevice));
+    for (size_t i = 0; i < n; ++i)
+        db[i] = q[o[i]];
+    PHX_HIP(c, hipMemcpy(s.q + offset, db.data(), n * 8, hipMemcpyHostToDevice));
     s.n = offset + n;
     return PHX_OK;
 }
@@ -647,6 +741,12 @@ phx_status phx_create(phx_config const* cfg, phx_patch const* patch, size_t part
         s = alloc(&c->massD, QMOM);
     if (s == PHX_OK)
         s = alloc(&c->tmp, QMOM);
+    for (int k = 0; k < 5 && s == PHX_OK; ++k)
+        s = alloc(&c->tot5[k], QMOM);
+    for (int p = 0; p < cfg->n_pop && s == PHX_OK; ++p)
+        for (int r = 0; r < 2 && s == PHX_OK; ++r)
+            for (int m = 0; m < 5 && s == PHX_OK; ++m)
+                s = alloc(&c->raw[r][p][m], QMOM);
     for (int p = 0; p < cfg->n_pop && s == PHX_OK; ++p)
     {
         c->pops[p].mass = cfg->mass[p];
@@ -678,6 +778,12 @@ phx_status phx_create(phx_config const* cfg, phx_patch const* patch, size_t part
     }
     for (double* p : {c->Pe, c->rho, c->massD, c->tmp})
         launchZero(p, G.size(QMOM), c->stream);
+    for (int k = 0; k < 5; ++k)
+        launchZero(c->tot5[k], G.size(QMOM), c->stream);
+    for (int p = 0; p < cfg->n_pop; ++p)
+        for (int r = 0; r < 2; ++r)
+            for (int m = 0; m < 5; ++m)
+                launchZero(c->raw[r][p][m], G.size(QMOM), c->stream);
     zeroMoments(c);
     DevError z{};
     (void)hipMemcpy(c->err, &z, sizeof(z), hipMemcpyHostToDevice);
@@ -1165,7 +1271,7 @@ phx_status phx_advance(phx_ctx* c, double dt, int nsteps)
         }
         return checkDeviceError(c);
     }
-    if (!c->graph || c->graphDt != dt)
+    if (!c->graph || c->graphDt != dt || c->graphRawCur != c->rawCur)
     {
         if (c->graph)
         {
@@ -1174,8 +1280,10 @@ phx_status phx_advance(phx_ctx* c, double dt, int nsteps)
         }
         PHX_HIP(c, hipStreamSynchronize(c->stream));
         hipGraph_t g = nullptr;
+        int const cur0 = c->rawCur;
         PHX_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         phx_status s = enqueueAdvanceFast(c, dt);
+        c->rawCur    = cur0; // a step uses both raw sets and ends on the one it started with
         hipError_t e = hipStreamEndCapture(c->stream, &g);
         if (s != PHX_OK)
         {
@@ -1192,7 +1300,8 @@ phx_status phx_advance(phx_ctx* c, double dt, int nsteps)
             c->graph = nullptr;
             return fail(c, PHX_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
         }
-        c->graphDt = dt;
+        c->graphDt     = dt;
+        c->graphRawCur = c->rawCur;
     }
     for (int s = 0; s < nsteps; ++s)
         PHX_HIP(c, hipGraphLaunch(c->graph, c->stream));
@@ -1256,25 +1365,35 @@ phx_status phx_profile_push(phx_ctx* c, double dt, int mode, int reps, double* a
         return PHX_ERR_ARG;
     if (!fastPath(c))
         return fail(c, PHX_ERR_STATE, "phx_profile_push needs a periodic level without ghost arrays");
+    // the push kernel alone, captured in a graph and replayed back to back between two events
+    // on the context stream (the mean over the replays is the kernel time; per-launch event
+    // pairs around direct launches read high on this runtime)
+    PushArgs A     = pushArgs(c, 0, KIND_DOMAIN, mode, PHX_EM_AVG, dt);
+    A.periodicWrap = 1;
+    PView v        = c->pops[0].arr[PHX_DOMAIN].view();
+    PHX_HIP(c, hipStreamSynchronize(c->stream));
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ge = nullptr;
+    PHX_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    launchPush(A, v, v, c->stream);
+    PHX_HIP(c, hipStreamEndCapture(c->stream, &g));
+    PHX_HIP(c, hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+    (void)hipGraphDestroy(g);
     hipEvent_t a = nullptr, b = nullptr;
     PHX_HIP(c, hipEventCreate(&a));
     PHX_HIP(c, hipEventCreate(&b));
-    double total = 0.;
+    PHX_HIP(c, hipGraphLaunch(ge, c->stream)); // warm-up
+    PHX_HIP(c, hipEventRecord(a, c->stream));
     for (int r = 0; r < reps; ++r)
-    {
-        PushArgs A     = pushArgs(c, 0, KIND_DOMAIN, mode, PHX_EM_AVG, dt);
-        A.periodicWrap = 1;
-        PView v        = c->pops[0].arr[PHX_DOMAIN].view();
-        PHX_HIP(c, hipEventRecord(a, c->stream));
-        launchPush(A, v, v, c->stream);
-        PHX_HIP(c, hipEventRecord(b, c->stream));
-        PHX_HIP(c, hipEventSynchronize(b));
-        float ms = 0.f;
-        PHX_HIP(c, hipEventElapsedTime(&ms, a, b));
-        total += ms;
-    }
+        PHX_HIP(c, hipGraphLaunch(ge, c->stream));
+    PHX_HIP(c, hipEventRecord(b, c->stream));
+    PHX_HIP(c, hipEventSynchronize(b));
+    float ms = 0.f;
+    PHX_HIP(c, hipEventElapsedTime(&ms, a, b));
+    double total = ms;
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
+    (void)hipGraphExecDestroy(ge);
     *avg_ms = total / reps;
     return checkDeviceError(c);
 }

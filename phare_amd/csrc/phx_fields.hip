@@ -254,20 +254,42 @@ __global__ __launch_bounds__(FB) void k_electrons(Geom G, double Te, CVecPtr J, 
 // ---- Ohm --------------------------------------------------------------------------------------
 __global__ __launch_bounds__(FB) void k_ohm(Geom G, double eta, double nu, int hyper, double lvlCoef,
                                             double const* nn, CVecPtr Ve, double const* Pe, CVecPtr B,
-                                            CVecPtr J, VecPtr E)
+                                            CVecPtr J, VecPtr E, int fill)
 {
     long const t = static_cast<long>(blockIdx.x) * FB + threadIdx.x;
-    int i, j, k;
-    if (!nodeOf(G, QMOM, t, i, j, k))
+    int i0, j0, k0;
+    if (!nodeOf(G, QMOM, t, i0, j0, k0))
         return;
     int const D = G.dim;
     for (int c = 0; c < 3; ++c)
     {
         int const q = QEX + c;
-        int lo[3], hi[3];
-        physBox(G, q, lo, hi);
-        if (!inRange(G, i, j, k, lo, hi))
-            continue;
+        int i = i0, j = j0, k = k0;
+        int const dstNode = G.idx(q, i0, j0, k0);
+        if (fill)
+        {
+            // ghost layers too: a ghost node gets Ohm evaluated at its periodic source
+            if (i0 >= G.shape[q][0] || j0 >= G.shape[q][1] || k0 >= G.shape[q][2])
+                continue;
+            int s[3] = {i0, j0, k0};
+            for (int d = 0; d < D; ++d)
+            {
+                if (s[d] < G.g)
+                    s[d] += G.n[d];
+                else if (s[d] > G.physEnd(q, d))
+                    s[d] -= G.n[d];
+            }
+            i = s[0];
+            j = s[1];
+            k = s[2];
+        }
+        else
+        {
+            int lo[3], hi[3];
+            physBox(G, q, lo, hi);
+            if (!inRange(G, i, j, k, lo, hi))
+                continue;
+        }
         // moments -> E_c: primal->dual along c
         int const mx = c == 0 ? P2D : PN, my = c == 1 ? P2D : PN, mz = c == 2 ? P2D : PN;
         double ideal, press = 0., bxE, byE, bzE;
@@ -333,7 +355,7 @@ __global__ __launch_bounds__(FB) void k_ohm(Geom G, double eta, double nu, int h
             double const b    = sqrt(bx * bx + by * by + bz * bz);
             hyp               = -nu * (b / (nOnE + 0.1) + 1) * lvlCoef * lapJ;
         }
-        E.c[c][n] = ideal + press + res + hyp;
+        E.c[c][dstNode] = ideal + press + res + hyp;
     }
 }
 
@@ -466,6 +488,304 @@ __global__ __launch_bounds__(FB) void k_border(Geom G, double const* orig, doubl
     out[n] = v;
 }
 
+// ---- fused kernels of the periodic fast path ------------------------------------------------
+// A pure ghost node of a periodic patch holds the value of its periodic source node; computing
+// the stencil directly at the source gives that value bit for bit, so the producer kernels
+// below write their ghost layers themselves instead of a separate fill.
+__device__ __forceinline__ bool wrapNode(Geom const& G, int q, int* c)
+{
+    bool ghost = false;
+    for (int d = 0; d < G.dim; ++d)
+    {
+        if (c[d] < G.g)
+        {
+            c[d] += G.n[d];
+            ghost = true;
+        }
+        else if (c[d] > G.physEnd(q, d))
+        {
+            c[d] -= G.n[d];
+            ghost = true;
+        }
+    }
+    return ghost;
+}
+
+__device__ __forceinline__ bool inShape(Geom const& G, int q, int i, int j, int k)
+{
+    return i < G.shape[q][0] && j < G.shape[q][1] && k < G.shape[q][2];
+}
+
+__device__ __forceinline__ double faradayAt(Geom const& G, CVecPtr B, CVecPtr E, double dt, int c,
+                                            int i, int j, int k)
+{
+    int const D = G.dim;
+    int const n = G.idx(QBX + c, i, j, k);
+    if (c == 0)
+    {
+        if (D == 1)
+            return B.c[0][n];
+        if (D == 2)
+            return B.c[0][n] - dt * deriv(G, E.c[2], QEZ, 1, i, j, k);
+        return B.c[0][n] - dt * deriv(G, E.c[2], QEZ, 1, i, j, k) + dt * deriv(G, E.c[1], QEY, 2, i, j, k);
+    }
+    if (c == 1)
+    {
+        if (D < 3)
+            return B.c[1][n] + dt * deriv(G, E.c[2], QEZ, 0, i, j, k);
+        return B.c[1][n] - dt * deriv(G, E.c[0], QEX, 2, i, j, k) + dt * deriv(G, E.c[2], QEZ, 0, i, j, k);
+    }
+    if (D == 1)
+        return B.c[2][n] - dt * deriv(G, E.c[1], QEY, 0, i, j, k);
+    return B.c[2][n] - dt * deriv(G, E.c[1], QEY, 0, i, j, k) + dt * deriv(G, E.c[0], QEX, 1, i, j, k);
+}
+
+// Faraday into a separate output, ghost layers included (predictor: B, E -> Bpred)
+__global__ __launch_bounds__(FB) void k_faraday_fill(Geom G, CVecPtr B, CVecPtr E, VecPtr Bn, double dt)
+{
+    long const t = static_cast<long>(blockIdx.x) * FB + threadIdx.x;
+    int i, j, k;
+    if (!nodeOf(G, QMOM, t, i, j, k))
+        return;
+    for (int c = 0; c < 3; ++c)
+    {
+        if (!inShape(G, QBX + c, i, j, k))
+            continue;
+        int s[3] = {i, j, k};
+        wrapNode(G, QBX + c, s);
+        Bn.c[c][G.idx(QBX + c, i, j, k)] = faradayAt(G, B, E, dt, c, s[0], s[1], s[2]);
+    }
+}
+
+__device__ __forceinline__ double ampereAt(Geom const& G, CVecPtr B, int c, int i, int j, int k)
+{
+    int const D = G.dim;
+    if (c == 0)
+    {
+        if (D == 1)
+            return 0.0;
+        if (D == 2)
+            return deriv(G, B.c[2], QBZ, 1, i, j, k);
+        return deriv(G, B.c[2], QBZ, 1, i, j, k) - deriv(G, B.c[1], QBY, 2, i, j, k);
+    }
+    if (c == 1)
+    {
+        if (D < 3)
+            return -deriv(G, B.c[2], QBZ, 0, i, j, k);
+        return deriv(G, B.c[0], QBX, 2, i, j, k) - deriv(G, B.c[2], QBZ, 0, i, j, k);
+    }
+    if (D == 1)
+        return deriv(G, B.c[1], QBY, 0, i, j, k);
+    return deriv(G, B.c[1], QBY, 0, i, j, k) - deriv(G, B.c[0], QBX, 1, i, j, k);
+}
+
+// Ampere + fillCurrentGhosts
+__global__ __launch_bounds__(FB) void k_ampere_fill(Geom G, CVecPtr B, VecPtr J)
+{
+    long const t = static_cast<long>(blockIdx.x) * FB + threadIdx.x;
+    int i, j, k;
+    if (!nodeOf(G, QMOM, t, i, j, k))
+        return;
+    for (int c = 0; c < 3; ++c)
+    {
+        if (!inShape(G, QJX + c, i, j, k))
+            continue;
+        int s[3] = {i, j, k};
+        wrapNode(G, QJX + c, s);
+        J.c[c][G.idx(QJX + c, i, j, k)] = ampereAt(G, B, c, s[0], s[1], s[2]);
+    }
+}
+
+// ghost fill of three components in one launch
+__global__ __launch_bounds__(FB) void k_fill3(Geom G, int q0, VecPtr F)
+{
+    long const t = static_cast<long>(blockIdx.x) * FB + threadIdx.x;
+    int i, j, k;
+    if (!nodeOf(G, QMOM, t, i, j, k))
+        return;
+    for (int c = 0; c < 3; ++c)
+    {
+        int const q = q0 + c;
+        if (!inShape(G, q, i, j, k))
+            continue;
+        int s[3] = {i, j, k};
+        if (wrapNode(G, q, s))
+            F.c[c][G.idx(q, i, j, k)] = F.c[c][G.idx(q, s[0], s[1], s[2])];
+    }
+}
+
+// average of E and B sets (algorithm.hpp:68-76) + fillElectricGhosts of Eavg
+__global__ __launch_bounds__(FB) void k_average_fill(Geom G, CVecPtr E, CVecPtr Ep, CVecPtr B,
+                                                     CVecPtr Bp, VecPtr Ea, VecPtr Ba)
+{
+    long const t = static_cast<long>(blockIdx.x) * FB + threadIdx.x;
+    int i, j, k;
+    if (!nodeOf(G, QMOM, t, i, j, k))
+        return;
+    for (int c = 0; c < 3; ++c)
+    {
+        if (inShape(G, QEX + c, i, j, k))
+        {
+            int s[3] = {i, j, k};
+            wrapNode(G, QEX + c, s);
+            int const src = G.idx(QEX + c, s[0], s[1], s[2]);
+            Ea.c[c][G.idx(QEX + c, i, j, k)] = (E.c[c][src] + Ep.c[c][src]) * .5;
+        }
+        if (inShape(G, QBX + c, i, j, k))
+        {
+            int const n = G.idx(QBX + c, i, j, k);
+            Ba.c[c][n]  = (B.c[c][n] + Bp.c[c][n]) * .5;
+        }
+    }
+}
+
+struct MomSet
+{
+    double const* raw[4][5];
+    double* mom[4][5];
+    double* zero[4][5]; // the other raw set, cleared for the next deposit
+    double mass[4];
+    int npop;
+};
+
+// fillFluxBorders + fillDensityBorders (border sum of every population moment) + updateIons
+// (totals into tmp) ; the max fill of the totals follows in k_max5
+__global__ __launch_bounds__(FB) void k_fold_totals(Geom G, MomSet M, double* trho, double* tmd,
+                                                    VecPtr tvi)
+{
+    long const t = static_cast<long>(blockIdx.x) * FB + threadIdx.x;
+    int i, j, k;
+    if (!nodeOf(G, QMOM, t, i, j, k))
+        return;
+    int const c[3] = {i, j, k};
+    int const n    = G.idx(QMOM, i, j, k);
+    int total      = 1;
+    for (int d = 0; d < G.dim; ++d)
+        total *= 3;
+    // sources of this node: itself, then the overlapping periodic images in lexicographic order
+    int src[27];
+    int ns = 0;
+    for (int im = 0; im < total; ++im)
+    {
+        int kk[3] = {0, 0, 0};
+        int r     = im;
+        bool zero = true;
+        for (int d = G.dim - 1; d >= 0; --d)
+        {
+            kk[d] = (r % 3) - 1;
+            r /= 3;
+            if (kk[d])
+                zero = false;
+        }
+        if (zero)
+            continue;
+        bool in  = true;
+        int s[3] = {i, j, k};
+        for (int d = 0; d < G.dim; ++d)
+        {
+            int const end = G.ghostEnd(QMOM, d);
+            int const sh  = kk[d] * G.n[d];
+            int const lo  = sh > 0 ? sh : 0;
+            int const hi  = end + sh < end ? end + sh : end;
+            if (c[d] < lo || c[d] > hi)
+            {
+                in = false;
+                break;
+            }
+            s[d] = c[d] - sh;
+        }
+        if (in)
+            src[ns++] = G.idx(QMOM, s[0], s[1], s[2]);
+    }
+    double r = 0., m = 0., vx = 0., vy = 0., vz = 0.;
+    for (int p = 0; p < M.npop; ++p)
+    {
+        double f[5];
+        for (int q = 0; q < 5; ++q)
+        {
+            double v = M.raw[p][q][n];
+            for (int x = 0; x < ns; ++x)
+                v += M.raw[p][q][src[x]];
+            f[q]          = v;
+            M.mom[p][q][n] = v;
+            M.zero[p][q][n] = 0.;
+        }
+        r  = r + f[1];
+        m  = m + f[0] * M.mass[p];
+        vx = vx + f[2] * M.mass[p];
+        vy = vy + f[3] * M.mass[p];
+        vz = vz + f[4] * M.mass[p];
+    }
+    trho[n]     = r;
+    tmd[n]      = m;
+    tvi.c[0][n] = vx / m;
+    tvi.c[1][n] = vy / m;
+    tvi.c[2][n] = vz / m;
+}
+
+struct Five
+{
+    double const* src[5];
+    double* dst[5];
+};
+
+// fillIonBorders: max over the periodic images (snapshot in src)
+__global__ __launch_bounds__(FB) void k_max5(Geom G, Five F)
+{
+    long const t = static_cast<long>(blockIdx.x) * FB + threadIdx.x;
+    int i, j, k;
+    if (!nodeOf(G, QMOM, t, i, j, k))
+        return;
+    int const c[3] = {i, j, k};
+    int const n    = G.idx(QMOM, i, j, k);
+    double v[5];
+    for (int q = 0; q < 5; ++q)
+        v[q] = F.src[q][n];
+    int total = 1;
+    for (int d = 0; d < G.dim; ++d)
+        total *= 3;
+    for (int im = 0; im < total; ++im)
+    {
+        int kk[3] = {0, 0, 0};
+        int r     = im;
+        bool zero = true;
+        for (int d = G.dim - 1; d >= 0; --d)
+        {
+            kk[d] = (r % 3) - 1;
+            r /= 3;
+            if (kk[d])
+                zero = false;
+        }
+        if (zero)
+            continue;
+        bool in  = true;
+        int s[3] = {i, j, k};
+        for (int d = 0; d < G.dim; ++d)
+        {
+            int const end = G.ghostEnd(QMOM, d);
+            int const sh  = kk[d] * G.n[d];
+            int const lo  = sh > 0 ? sh : 0;
+            int const hi  = end + sh < end ? end + sh : end;
+            if (c[d] < lo || c[d] > hi)
+            {
+                in = false;
+                break;
+            }
+            s[d] = c[d] - sh;
+        }
+        if (!in)
+            continue;
+        int const sn = G.idx(QMOM, s[0], s[1], s[2]);
+        for (int q = 0; q < 5; ++q)
+        {
+            double const o = F.src[q][sn];
+            v[q]           = v[q] < o ? o : v[q];
+        }
+    }
+    for (int q = 0; q < 5; ++q)
+        F.dst[q][n] = v[q];
+}
+
 unsigned int gridFor(long n) { return static_cast<unsigned int>((n + FB - 1) / FB); }
 
 } // namespace
@@ -484,13 +804,13 @@ void launchElectrons(Geom const& G, double Te, CVecPtr J, CVecPtr Vi, double con
     k_electrons<<<gridFor(G.size(QMOM)), FB, 0, s>>>(G, Te, J, Vi, Ne, Ve, Pe);
 }
 void launchOhm(Geom const& G, double eta, double nu, int hyper, int level, double const* n,
-               CVecPtr Ve, double const* Pe, CVecPtr B, CVecPtr J, VecPtr E, hipStream_t s)
+               CVecPtr Ve, double const* Pe, CVecPtr B, CVecPtr J, VecPtr E, hipStream_t s, int fill)
 {
     double lvlCoef = 1.;
     for (int l = 0; l < level; ++l)
         lvlCoef *= 4.;
     lvlCoef = 1. / lvlCoef;
-    k_ohm<<<gridFor(G.size(QMOM)), FB, 0, s>>>(G, eta, nu, hyper, lvlCoef, n, Ve, Pe, B, J, E);
+    k_ohm<<<gridFor(G.size(QMOM)), FB, 0, s>>>(G, eta, nu, hyper, lvlCoef, n, Ve, Pe, B, J, E, fill);
 }
 void launchAverage(double const* a, double const* b, double* out, long count, hipStream_t s)
 {
@@ -517,6 +837,51 @@ void launchIonTotals(PopMoments const& P, double* rho, double* massDensity, VecP
                      hipStream_t s)
 {
     k_ion_totals<<<gridFor(count), FB, 0, s>>>(P, rho, massDensity, Vi, count);
+}
+
+void launchFaradayFill(Geom const& G, CVecPtr B, CVecPtr E, VecPtr Bnew, double dt, hipStream_t s)
+{
+    k_faraday_fill<<<gridFor(G.size(QMOM)), FB, 0, s>>>(G, B, E, Bnew, dt);
+}
+void launchAmpereFill(Geom const& G, CVecPtr B, VecPtr J, hipStream_t s)
+{
+    k_ampere_fill<<<gridFor(G.size(QMOM)), FB, 0, s>>>(G, B, J);
+}
+void launchFill3(Geom const& G, int q0, VecPtr F, hipStream_t s)
+{
+    k_fill3<<<gridFor(G.size(QMOM)), FB, 0, s>>>(G, q0, F);
+}
+void launchAverageFill(Geom const& G, CVecPtr E, CVecPtr Ep, CVecPtr B, CVecPtr Bp, VecPtr Ea,
+                       VecPtr Ba, hipStream_t s)
+{
+    k_average_fill<<<gridFor(G.size(QMOM)), FB, 0, s>>>(G, E, Ep, B, Bp, Ea, Ba);
+}
+void launchFoldTotals(Geom const& G, MomSetArgs const& A, double* trho, double* tmd, VecPtr tvi,
+                      hipStream_t s)
+{
+    MomSet M{};
+    M.npop = A.npop;
+    for (int p = 0; p < A.npop; ++p)
+    {
+        M.mass[p] = A.mass[p];
+        for (int q = 0; q < 5; ++q)
+        {
+            M.raw[p][q]  = A.raw[p][q];
+            M.mom[p][q]  = A.mom[p][q];
+            M.zero[p][q] = A.zero[p][q];
+        }
+    }
+    k_fold_totals<<<gridFor(G.size(QMOM)), FB, 0, s>>>(G, M, trho, tmd, tvi);
+}
+void launchMax5(Geom const& G, double const* const* src, double* const* dst, hipStream_t s)
+{
+    Five F{};
+    for (int q = 0; q < 5; ++q)
+    {
+        F.src[q] = src[q];
+        F.dst[q] = dst[q];
+    }
+    k_max5<<<gridFor(G.size(QMOM)), FB, 0, s>>>(G, F);
 }
 
 } // namespace phx

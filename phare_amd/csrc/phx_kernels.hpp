@@ -61,6 +61,9 @@ struct PushArgs
     double coef;
     Boxes box;
     int kind, mode, periodicWrap, pop;
+    int pushOn;     // 0: deposit only (unpushed arrays, initial moments)
+    int tileLo[3];  // LDS tile origin relative to the chunk's first cell (local node index)
+    int tileExt[3]; // LDS tile extents (product <= the tile capacity)
     unsigned long long n;
     DevError* err;
     uint8_t* status;
@@ -75,6 +78,9 @@ struct Wrap
 
 void launchPush(PushArgs const& A, PView const& in, PView const& out, hipStream_t s);
 void launchDeposit(PushArgs const& A, PView const& in, hipStream_t s);
+void tileShape(int dim, int order, double ppc, int* lo, int* ext);
+int chunkSize(int dim, int order);
+int blockThreads();
 unsigned int compactBlocks(unsigned long long n);
 void launchCompact(PView const& src, PView const& dst, uint8_t const* status, unsigned long long n,
                    uint8_t want, unsigned int* blockCounts, unsigned long long* total,
@@ -95,7 +101,8 @@ void launchAmpere(Geom const& G, CVecPtr B, VecPtr J, hipStream_t s);
 void launchElectrons(Geom const& G, double Te, CVecPtr J, CVecPtr Vi, double const* Ne, VecPtr Ve,
                      double* Pe, hipStream_t s);
 void launchOhm(Geom const& G, double eta, double nu, int hyper, int level, double const* n,
-               CVecPtr Ve, double const* Pe, CVecPtr B, CVecPtr J, VecPtr E, hipStream_t s);
+               CVecPtr Ve, double const* Pe, CVecPtr B, CVecPtr J, VecPtr E, hipStream_t s,
+               int fill = 0);
 void launchAverage(double const* a, double const* b, double* out, long count, hipStream_t s);
 void launchFillGhosts(Geom const& G, int q, double* f, hipStream_t s);
 void launchBorderSum(Geom const& G, double const* orig, double* out, hipStream_t s);
@@ -112,5 +119,23 @@ struct PopMoments
 };
 void launchIonTotals(PopMoments const& P, double* rho, double* massDensity, VecPtr Vi, long count,
                      hipStream_t s);
+
+// fused kernels of the periodic fast path
+struct MomSetArgs
+{
+    double const* raw[4][5];
+    double* mom[4][5];
+    double* zero[4][5];
+    double mass[4];
+    int npop;
+};
+void launchFaradayFill(Geom const& G, CVecPtr B, CVecPtr E, VecPtr Bnew, double dt, hipStream_t s);
+void launchAmpereFill(Geom const& G, CVecPtr B, VecPtr J, hipStream_t s);
+void launchFill3(Geom const& G, int q0, VecPtr F, hipStream_t s);
+void launchAverageFill(Geom const& G, CVecPtr E, CVecPtr Ep, CVecPtr B, CVecPtr Bp, VecPtr Ea,
+                       VecPtr Ba, hipStream_t s);
+void launchFoldTotals(Geom const& G, MomSetArgs const& A, double* trho, double* tmd, VecPtr tvi,
+                      hipStream_t s);
+void launchMax5(Geom const& G, double const* const* src, double* const* dst, hipStream_t s);
 
 } // namespace phx

@@ -17,9 +17,30 @@ namespace phx
 namespace
 {
 constexpr int BLOCK = 256;
-constexpr int PPT   = 4; // particles per thread
-constexpr int CHUNK = BLOCK * PPT;
-constexpr int TILE_NODES = 1536; // LDS tile capacity in nodes (x5 moments x8 B = 60 KB)
+constexpr int NW    = BLOCK / 64; // wavefronts per workgroup
+
+// Per (dim, order): whether the deposit is accumulated in registers over a union window, the
+// particles per thread and the window. With accumulation, each thread owns PPT consecutive
+// particles of the caller's order (mostly one cell) and scatters to LDS only when its cell
+// changes. Only 1D keeps the window (5 moments x 4 nodes) in registers without losing
+// occupancy; 2D/3D scatter every particle to the LDS tile.
+template<int D, int O>
+struct Cfg
+{
+    static constexpr bool ACC  = D == 1;
+    static constexpr int PPT   = ACC ? 8 : 2;
+    static constexpr int CHUNK = BLOCK * PPT;
+    static constexpr int W     = O == 1 ? 2 : 4;  // window width per direction
+    static constexpr int WB    = O == 1 ? 0 : 1;  // window base = local cell - WB
+    static constexpr int WN    = D == 1 ? W : (D == 2 ? W * W : W * W * W);
+};
+
+// LDS moment tile: 5 moments x TILE nodes (20 KB in 1D/2D, 40 KB in 3D)
+template<int D>
+struct Tile
+{
+    static constexpr int NODES = D == 3 ? 1024 : 512;
+};
 
 __device__ __forceinline__ bool inBox(int const* c, int const* lo, int const* hi, int D)
 {
@@ -37,13 +58,11 @@ __device__ __forceinline__ bool inNLGB(int const* c, Boxes const& b, int D)
     return false;
 }
 
-// map of (thread, iteration) -> particle offset inside the chunk: lane l of wave w at step k
-// takes l*16 + w*4 + k, so one wavefront spans the whole chunk
-__device__ __forceinline__ int chunkOffset(int t, int k)
-{
-    int lane = t & 63, wave = t >> 6;
-    return lane * 16 + wave * 4 + k;
-}
+// (thread, step) -> slot in the chunk: consecutive threads read consecutive slots (coalesced).
+// Arrays are stored chunk-interleaved (chunkInterleave on the host): slot k*BLOCK + t holds the
+// particle (r*PPT + k) of the caller's order, r = lane*NW + wave. Each thread thus owns PPT
+// consecutive particles, and the lanes of a wavefront own runs spread across the chunk.
+__device__ __forceinline__ int chunkOffset(int t, int k) { return k * BLOCK + t; }
 
 template<int D, int O>
 struct DepositData
@@ -51,7 +70,6 @@ struct DepositData
     int start[D];
     double w[D][O + 1];
     double val[5];
-    bool on;
 };
 
 template<int D, int O>
@@ -66,7 +84,7 @@ __device__ __forceinline__ void primalWeights(Geom const& G, int const* cell, do
     }
 }
 
-// contribution of one particle to one node (a,b,c) of its stencil, in the reference's
+// contribution of one particle to node (a,b,c) of its stencil, in the reference's
 // multiplication order (interpolator.hpp:290-361)
 template<int D, int O>
 __device__ __forceinline__ double nodeWeight(DepositData<D, O> const& dd, double dep, int a, int b,
@@ -80,61 +98,198 @@ __device__ __forceinline__ double nodeWeight(DepositData<D, O> const& dd, double
         return dep * dd.w[0][a] * dd.w[1][b] * dd.w[2][c];
 }
 
+// scatter-add the 5 moments of one particle: into the LDS tile when its whole stencil lies in
+// the tile, else straight into the HBM moment arrays
 template<int D, int O>
-__device__ __forceinline__ void depositTo(DepositData<D, O> const& dd, double* const* dst,
-                                          int const* origin, int const* ext, bool lds)
+__device__ __forceinline__ void depositOne(DepositData<D, O> const& dd, Geom const& G, double* tile,
+                                           int const* torg, int const* text, double* const* mom)
 {
     constexpr int S  = O + 1;
     constexpr int SY = D > 1 ? S : 1;
     constexpr int SZ = D > 2 ? S : 1;
-    for (int a = 0; a < S; ++a)
-        for (int b = 0; b < SY; ++b)
-            for (int c = 0; c < SZ; ++c)
-            {
-                int i = dd.start[0] + a - origin[0];
-                int j = D > 1 ? dd.start[1] + b - origin[1] : 0;
-                int k = D > 2 ? dd.start[2] + c - origin[2] : 0;
-                int node = (i * ext[1] + j) * ext[2] + k;
+    constexpr int TN = Tile<D>::NODES;
+    bool inTile      = true;
 #pragma unroll
-                for (int m = 0; m < 5; ++m)
+    for (int d = 0; d < D; ++d)
+        inTile = inTile && dd.start[d] >= torg[d] && dd.start[d] + O < torg[d] + text[d];
+    if (inTile)
+    {
+#pragma unroll
+        for (int a = 0; a < S; ++a)
+#pragma unroll
+            for (int b = 0; b < SY; ++b)
+#pragma unroll
+                for (int c = 0; c < SZ; ++c)
                 {
-                    double contrib = nodeWeight<D, O>(dd, dd.val[m], a, b, c);
-                    if (lds)
-                        atomicAdd(dst[0] + m * TILE_NODES + node, contrib);
-                    else
-                        atomicAdd(dst[m] + node, contrib);
+                    int i    = dd.start[0] + a - torg[0];
+                    int j    = D > 1 ? dd.start[1] + b - torg[1] : 0;
+                    int k    = D > 2 ? dd.start[2] + c - torg[2] : 0;
+                    int node = (i * text[1] + j) * text[2] + k;
+#pragma unroll
+                    for (int m = 0; m < 5; ++m)
+                        atomicAdd(tile + m * TN + node, nodeWeight<D, O>(dd, dd.val[m], a, b, c));
                 }
+    }
+    else
+    {
+        for (int a = 0; a < S; ++a)
+            for (int b = 0; b < SY; ++b)
+                for (int c = 0; c < SZ; ++c)
+                {
+                    int node = G.idx(QMOM, dd.start[0] + a, D > 1 ? dd.start[1] + b : 0,
+                                     D > 2 ? dd.start[2] + c : 0);
+#pragma unroll
+                    for (int m = 0; m < 5; ++m)
+                        atomicAdd(mom[m] + node, nodeWeight<D, O>(dd, dd.val[m], a, b, c));
+                }
+    }
+}
+
+// register accumulator of one thread: the 5 moments over the union window of its current cell
+template<int D, int O>
+struct Acc
+{
+    static constexpr int W  = Cfg<D, O>::W;
+    static constexpr int WN = Cfg<D, O>::WN;
+    double a[5][WN];
+    int base[D]; // local node index of the window origin
+    bool live;
+};
+
+template<int D, int O>
+__device__ __forceinline__ void accZero(Acc<D, O>& A)
+{
+#pragma unroll
+    for (int m = 0; m < 5; ++m)
+#pragma unroll
+        for (int x = 0; x < Acc<D, O>::WN; ++x)
+            A.a[m][x] = 0.;
+    A.live = false;
+}
+
+// scatter the window: nodes inside the LDS tile by LDS atomics, others straight to HBM;
+// never-touched nodes hold exactly zero and are skipped
+template<int D, int O>
+__device__ __forceinline__ void accFlush(Acc<D, O> const& A, Geom const& G, double* tile, int const* torg,
+                                         int const* text, double* const* mom)
+{
+    constexpr int W  = Acc<D, O>::W;
+    constexpr int TN = Tile<D>::NODES;
+    if (!A.live)
+        return;
+#pragma unroll
+    for (int x = 0; x < Acc<D, O>::WN; ++x)
+    {
+        int const a = D == 1 ? x : (D == 2 ? x / W : x / (W * W));
+        int const b = D == 1 ? 0 : (D == 2 ? x % W : (x / W) % W);
+        int const c = D == 3 ? x % W : 0;
+        int gi = A.base[0] + a;
+        int gj = D > 1 ? A.base[1] + b : 0;
+        int gk = D > 2 ? A.base[2] + c : 0;
+        bool inTile = gi >= torg[0] && gi < torg[0] + text[0];
+        if (D > 1)
+            inTile = inTile && gj >= torg[1] && gj < torg[1] + text[1];
+        if (D > 2)
+            inTile = inTile && gk >= torg[2] && gk < torg[2] + text[2];
+        int const tnode = ((gi - torg[0]) * text[1] + (D > 1 ? gj - torg[1] : 0)) * text[2]
+                          + (D > 2 ? gk - torg[2] : 0);
+#pragma unroll
+        for (int m = 0; m < 5; ++m)
+        {
+            double const v = A.a[m][x];
+            if (v != 0.)
+            {
+                if (inTile)
+                    atomicAdd(tile + m * TN + tnode, v);
+                else
+                    atomicAdd(mom[m] + G.idx(QMOM, gi, gj, gk), v);
             }
+        }
+    }
+}
+
+// add one particle's deposit (already weighted per the reference's products) to the window
+template<int D, int O>
+__device__ __forceinline__ void accAdd(Acc<D, O>& A, DepositData<D, O> const& dd)
+{
+    constexpr int W = Acc<D, O>::W;
+    constexpr int S = O + 1;
+    // weights placed in the window: offset of the stencil start from the window base
+    double w[D][W];
+#pragma unroll
+    for (int d = 0; d < D; ++d)
+    {
+        int const off = dd.start[d] - A.base[d];
+#pragma unroll
+        for (int a = 0; a < W; ++a)
+        {
+            double x = 0.;
+#pragma unroll
+            for (int e = 0; e < S; ++e)
+                x = (a - off == e) ? dd.w[d][e] : x;
+            w[d][a] = x;
+        }
+    }
+#pragma unroll
+    for (int x = 0; x < Acc<D, O>::WN; ++x)
+    {
+        int const a = D == 1 ? x : (D == 2 ? x / W : x / (W * W));
+        int const b = D == 1 ? 0 : (D == 2 ? x % W : (x / W) % W);
+        int const c = D == 3 ? x % W : 0;
+#pragma unroll
+        for (int m = 0; m < 5; ++m)
+        {
+            double v;
+            if constexpr (D == 1)
+                v = dd.val[m] * w[0][a];
+            else if constexpr (D == 2)
+                v = dd.val[m] * w[0][a] * w[1][b];
+            else
+                v = dd.val[m] * w[0][a] * w[1][b] * w[2][c];
+            A.a[m][x] += v;
+        }
+    }
 }
 
 } // namespace
 
 // ---------------------------------------------------------------------------------------------
-// fused push + deposit
+// fused push + deposit (and deposit-only for unpushed arrays)
 // ---------------------------------------------------------------------------------------------
 template<int D, int O>
 __global__ __launch_bounds__(BLOCK) void k_push(PushArgs A, PView in, PView out)
 {
-    constexpr int S = O + 1;
-    __shared__ double tile[5 * TILE_NODES];
-    __shared__ int bb[6];
+    using C                = Cfg<D, O>;
+    constexpr int TN       = Tile<D>::NODES;
+    constexpr int PPT      = C::PPT;
+    constexpr int CHUNK    = C::CHUNK;
+    __shared__ double tile[5 * TN];
+    __shared__ int org[3];
 
     Geom const& G = A.G;
     int const t   = threadIdx.x;
-    if (t < 3)
+    unsigned long long const base = static_cast<unsigned long long>(blockIdx.x) * CHUNK;
+    Acc<D, O> acc;
+    if constexpr (C::ACC)
+        accZero(acc);
+
+    for (int x = t; x < 5 * TN; x += BLOCK)
+        tile[x] = 0.;
+    if (t == 0)
     {
-        bb[t]     = 0x7fffffff;
-        bb[3 + t] = -0x7fffffff;
+        // the tile is anchored at the chunk's first particle (particles are kept close to
+        // cell order), offset by the host-chosen margins
+        int c0 = in.ic[0][base];
+        for (int d = 0; d < 3; ++d)
+            org[d] = d < D ? (c0 == DEAD_CELL ? -0x20000000
+                                              : in.ic[d][base] - (G.lower[d] - G.g) - A.tileLo[d])
+                           : 0;
     }
     __syncthreads();
-    unsigned long long const base = static_cast<unsigned long long>(blockIdx.x) * CHUNK;
+    int torg[3] = {org[0], org[1], org[2]};
 
-    DepositData<D, O> dd[PPT];
-
-#pragma unroll
     for (int k = 0; k < PPT; ++k)
     {
-        dd[k].on                   = false;
         unsigned long long const i = base + chunkOffset(t, k);
         if (i >= A.n)
             continue;
@@ -153,89 +308,102 @@ __global__ __launch_bounds__(BLOCK) void k_push(PushArgs A, PView in, PView out)
         v[2] = in.v[2][i];
         double const w = in.w[i], q = in.q[i];
 
-        double bd, bv;
-        // prePushStep_ (boris.hpp:180-216)
-        if (!advancePosition<D>(cell, delta, v, A.hdtdx, bd, bv))
+        bool deposit = true, sel1 = true, inDom = true;
+        uint8_t st   = ST_KEEP;
+        if (A.pushOn)
         {
-            raiseError(A.err, 3u, 1u, i, bd, bv, A.kind, A.pop);
-            continue;
-        }
-        // first selector: noop (domain) or inGhostBox (level ghosts)
-        bool const sel1 = A.kind == KIND_DOMAIN ? true : inBox(cell, A.box.gh_lo, A.box.gh_hi, D);
-        if (sel1)
-        {
-            Stencil<D, O> s;
-            makeStencil<D, O>(G, cell, delta, s);
-            if (!stencilInside<D, O>(G, s, QEX) || !stencilInside<D, O>(G, s, QBX)
-                || !stencilInside<D, O>(G, s, QEY) || !stencilInside<D, O>(G, s, QBY)
-                || !stencilInside<D, O>(G, s, QEZ) || !stencilInside<D, O>(G, s, QBZ))
-            {
-                raiseError(A.err, 4u, 2u, i, delta[0], v[0], A.kind, A.pop);
-                continue;
-            }
-            double E[3], B[3];
-            E[0] = interp<D, O>(G, A.em.E[0], QEX, s);
-            E[1] = interp<D, O>(G, A.em.E[1], QEY, s);
-            E[2] = interp<D, O>(G, A.em.E[2], QEZ, s);
-            B[0] = interp<D, O>(G, A.em.B[0], QBX, s);
-            B[1] = interp<D, O>(G, A.em.B[1], QBY, s);
-            B[2] = interp<D, O>(G, A.em.B[2], QBZ, s);
-            borisAccelerate(v, E, B, q, A.dto2m);
-            // postPushStep_ (boris.hpp:218-234)
+            double bd, bv;
+            // prePushStep_ (boris.hpp:180-216)
             if (!advancePosition<D>(cell, delta, v, A.hdtdx, bd, bv))
             {
-                raiseError(A.err, 3u, 2u, i, bd, bv, A.kind, A.pop);
+                raiseError(A.err, 3u, 1u, i, bd, bv, A.kind, A.pop);
                 continue;
             }
-        }
-
-        // selection of what is deposited and what the array keeps
-        bool const inDom = inBox(cell, A.box.dom_lo, A.box.dom_hi, D);
-        bool deposit;
-        uint8_t st = ST_KEEP;
-        if (A.kind == KIND_DOMAIN)
-        {
-            bool const nl = inNLGB(cell, A.box, D);
-            deposit       = nl;
-            st            = inDom ? ST_KEEP : (nl ? ST_LEAVE : ST_DROP);
-        }
-        else
-        {
-            deposit = sel1 && inDom;
-            bool const layer
-                = sel1 && !inDom && inBox(cell, A.box.gh_lo, A.box.gh_hi, D);
-            st = !sel1 ? ST_DROP : (inDom ? ST_EXPORT : (layer ? ST_KEEP : ST_DROP));
+            // first selector: noop (domain) or inGhostBox (level ghosts)
+            sel1 = A.kind == KIND_DOMAIN ? true : inBox(cell, A.box.gh_lo, A.box.gh_hi, D);
+            if (sel1)
+            {
+                Stencil<D, O> s;
+                makeStencil<D, O>(G, cell, delta, s);
+                if (!stencilInside<D, O>(G, s, QEX) || !stencilInside<D, O>(G, s, QBX)
+                    || !stencilInside<D, O>(G, s, QEY) || !stencilInside<D, O>(G, s, QBY)
+                    || !stencilInside<D, O>(G, s, QEZ) || !stencilInside<D, O>(G, s, QBZ))
+                {
+                    raiseError(A.err, 4u, 2u, i, delta[0], v[0], A.kind, A.pop);
+                    continue;
+                }
+                double E[3], B[3];
+                E[0] = interp<D, O>(G, A.em.E[0], QEX, s);
+                E[1] = interp<D, O>(G, A.em.E[1], QEY, s);
+                E[2] = interp<D, O>(G, A.em.E[2], QEZ, s);
+                B[0] = interp<D, O>(G, A.em.B[0], QBX, s);
+                B[1] = interp<D, O>(G, A.em.B[1], QBY, s);
+                B[2] = interp<D, O>(G, A.em.B[2], QBZ, s);
+                borisAccelerate(v, E, B, q, A.dto2m);
+                // postPushStep_ (boris.hpp:218-234)
+                if (!advancePosition<D>(cell, delta, v, A.hdtdx, bd, bv))
+                {
+                    raiseError(A.err, 3u, 2u, i, bd, bv, A.kind, A.pop);
+                    continue;
+                }
+            }
+            // what is deposited and what the array keeps (ion_updater.hpp:169-295)
+            inDom = inBox(cell, A.box.dom_lo, A.box.dom_hi, D);
+            if (A.kind == KIND_DOMAIN)
+            {
+                bool const nl = inNLGB(cell, A.box, D);
+                deposit       = nl;
+                st            = inDom ? ST_KEEP : (nl ? ST_LEAVE : ST_DROP);
+            }
+            else
+            {
+                deposit = sel1 && inDom;
+                bool const layer = sel1 && !inDom && inBox(cell, A.box.gh_lo, A.box.gh_hi, D);
+                st = !sel1 ? ST_DROP : (inDom ? ST_EXPORT : (layer ? ST_KEEP : ST_DROP));
+            }
         }
 
         if (deposit)
         {
-            primalWeights<D, O>(G, cell, delta, dd[k]);
+            DepositData<D, O> dd;
+            primalWeights<D, O>(G, cell, delta, dd);
             bool ok = true;
 #pragma unroll
             for (int d = 0; d < D; ++d)
-                ok = ok && dd[k].start[d] >= 0 && dd[k].start[d] + O < G.shape[QMOM][d];
+                ok = ok && dd.start[d] >= 0 && dd.start[d] + O < G.shape[QMOM][d];
             if (!ok)
-            {
                 raiseError(A.err, 4u, 3u, i, delta[0], v[0], A.kind, A.pop);
-            }
             else
             {
-                dd[k].val[0]    = 1. * w * A.coef;
-                dd[k].val[1]    = q * w * A.coef;
-                dd[k].val[2]    = v[0] * w * A.coef;
-                dd[k].val[3]    = v[1] * w * A.coef;
-                dd[k].val[4]    = v[2] * w * A.coef;
-                dd[k].on        = true;
-#pragma unroll
-                for (int d = 0; d < D; ++d)
+                dd.val[0] = 1. * w * A.coef;
+                dd.val[1] = q * w * A.coef;
+                dd.val[2] = v[0] * w * A.coef;
+                dd.val[3] = v[1] * w * A.coef;
+                dd.val[4] = v[2] * w * A.coef;
+                if constexpr (C::ACC)
                 {
-                    atomicMin(&bb[d], dd[k].start[d]);
-                    atomicMax(&bb[3 + d], dd[k].start[d] + O);
+                    // same cell as the window -> accumulate; else flush and restart there
+                    bool same = acc.live;
+#pragma unroll
+                    for (int d = 0; d < D; ++d)
+                        same = same && acc.base[d] == cell[d] - (G.lower[d] - G.g) - C::WB;
+                    if (!same)
+                    {
+                        accFlush<D, O>(acc, G, tile, torg, A.tileExt, A.mom);
+                        accZero(acc);
+#pragma unroll
+                        for (int d = 0; d < D; ++d)
+                            acc.base[d] = cell[d] - (G.lower[d] - G.g) - C::WB;
+                        acc.live = true;
+                    }
+                    accAdd<D, O>(acc, dd);
                 }
+                else
+                    depositOne<D, O>(dd, G, tile, torg, A.tileExt, A.mom);
             }
         }
 
-        if (A.mode == MODE_ALL)
+        if (A.pushOn && A.mode == MODE_ALL)
         {
             if (A.periodicWrap && A.kind == KIND_DOMAIN)
             {
@@ -276,99 +444,32 @@ __global__ __launch_bounds__(BLOCK) void k_push(PushArgs A, PView in, PView out)
                 A.status[i] = st;
         }
     }
+    if constexpr (C::ACC)
+        accFlush<D, O>(acc, G, tile, torg, A.tileExt, A.mom);
     __syncthreads();
 
-    // deposit: LDS tile over the chunk's footprint when it fits, else straight to HBM
-    int origin[3] = {0, 0, 0}, ext[3] = {1, 1, 1};
-    long vol = 1;
-    bool any = bb[0] <= bb[3];
-#pragma unroll
-    for (int d = 0; d < D; ++d)
+    // flush the tile: one HBM atomic per touched node and moment
+    int const vol = A.tileExt[0] * A.tileExt[1] * A.tileExt[2];
+    for (int x = t; x < vol; x += BLOCK)
     {
-        origin[d] = bb[d];
-        ext[d]    = bb[3 + d] - bb[d] + 1;
-        vol *= ext[d];
-    }
-    if (!any)
-        return;
-    bool const useLds = vol <= TILE_NODES;
-    if (useLds)
-    {
-        for (int x = t; x < 5 * TILE_NODES; x += BLOCK)
-            tile[x] = 0.;
-        __syncthreads();
-        double* tp[1] = {tile};
+        int const e12 = A.tileExt[1] * A.tileExt[2];
+        int const ti  = x / e12;
+        int const r   = x - ti * e12;
+        int const tj  = r / A.tileExt[2];
+        int const tk  = r - tj * A.tileExt[2];
+        int node      = -1;
 #pragma unroll
-        for (int k = 0; k < PPT; ++k)
-            if (dd[k].on)
-                depositTo<D, O>(dd[k], tp, origin, ext, true);
-        __syncthreads();
-        // flush: one HBM atomic per touched node and moment
-        for (int x = t; x < vol; x += BLOCK)
+        for (int m = 0; m < 5; ++m)
         {
-            int i = x / (ext[1] * ext[2]);
-            int r = x - i * ext[1] * ext[2];
-            int j = r / ext[2];
-            int k = r - j * ext[2];
-            int gi = (i + origin[0]);
-            int gj = D > 1 ? (j + origin[1]) : 0;
-            int gk = D > 2 ? (k + origin[2]) : 0;
-            int node = G.idx(QMOM, gi, gj, gk);
-#pragma unroll
-            for (int m = 0; m < 5; ++m)
+            double const val = tile[m * TN + x];
+            if (val != 0.)
             {
-                double val = tile[m * TILE_NODES + x];
-                if (val != 0.)
-                    atomicAdd(A.mom[m] + node, val);
+                if (node < 0)
+                    node = G.idx(QMOM, torg[0] + ti, D > 1 ? torg[1] + tj : 0, D > 2 ? torg[2] + tk : 0);
+                atomicAdd(A.mom[m] + node, val);
             }
         }
     }
-    else
-    {
-        int zero[3] = {0, 0, 0};
-        int ext3[3] = {G.shape[QMOM][0], G.shape[QMOM][1], G.shape[QMOM][2]};
-#pragma unroll
-        for (int k = 0; k < PPT; ++k)
-            if (dd[k].on)
-                depositTo<D, O>(dd[k], A.mom, zero, ext3, false);
-    }
-}
-
-// deposit without push (patch-ghost particles in mode all, ion_updater.hpp:292-293)
-template<int D, int O>
-__global__ __launch_bounds__(BLOCK) void k_deposit(PushArgs A, PView in)
-{
-    unsigned long long const i = static_cast<unsigned long long>(blockIdx.x) * BLOCK + threadIdx.x;
-    if (i >= A.n)
-        return;
-    int cell[D];
-    double delta[D];
-#pragma unroll
-    for (int d = 0; d < D; ++d)
-    {
-        cell[d]  = in.ic[d][i];
-        delta[d] = in.de[d][i];
-    }
-    if (cell[0] == DEAD_CELL)
-        return;
-    DepositData<D, O> dd;
-    primalWeights<D, O>(A.G, cell, delta, dd);
-#pragma unroll
-    for (int d = 0; d < D; ++d)
-        if (dd.start[d] < 0 || dd.start[d] + O >= A.G.shape[QMOM][d])
-        {
-            raiseError(A.err, 4u, 3u, i, delta[0], 0., A.kind, A.pop);
-            return;
-        }
-    double const w = in.w[i], q = in.q[i];
-    dd.val[0]      = 1. * w * A.coef;
-    dd.val[1]      = q * w * A.coef;
-    dd.val[2]      = in.v[0][i] * w * A.coef;
-    dd.val[3]      = in.v[1][i] * w * A.coef;
-    dd.val[4]      = in.v[2][i] * w * A.coef;
-    int zero[3]    = {0, 0, 0};
-    int ext3[3]    = {A.G.shape[QMOM][0], A.G.shape[QMOM][1], A.G.shape[QMOM][2]};
-    depositTo<D, O>(dd, A.mom, zero, ext3, false);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -484,8 +585,58 @@ static void launchPushT(PushArgs const& A, PView const& in, PView const& out, hi
 {
     if (A.n == 0)
         return;
-    unsigned int const grid = static_cast<unsigned int>((A.n + CHUNK - 1) / CHUNK);
-    k_push<D, O><<<grid, BLOCK, 0, s>>>(A, in, out);
+    PushArgs B = A;
+    B.pushOn   = 1;
+    constexpr int CH = Cfg<D, O>::CHUNK;
+    unsigned int const grid = static_cast<unsigned int>((A.n + CH - 1) / CH);
+    k_push<D, O><<<grid, BLOCK, 0, s>>>(B, in, out);
+}
+
+// LDS tile extents for a level: the chunk of CHUNK consecutive particles spans about
+// CHUNK/ppc cells along the fastest dimension; margins cover the stencil and one cell of motion
+template<int D, int O>
+static int chunkOf()
+{
+    return Cfg<D, O>::CHUNK;
+}
+
+int chunkSize(int dim, int order)
+{
+    switch (dim * 10 + order)
+    {
+        case 11: return chunkOf<1, 1>();
+        case 12: return chunkOf<1, 2>();
+        case 13: return chunkOf<1, 3>();
+        case 21: return chunkOf<2, 1>();
+        case 22: return chunkOf<2, 2>();
+        case 23: return chunkOf<2, 3>();
+        case 31: return chunkOf<3, 1>();
+        case 32: return chunkOf<3, 2>();
+        case 33: return chunkOf<3, 3>();
+    }
+    return BLOCK;
+}
+
+int blockThreads() { return BLOCK; }
+
+void tileShape(int dim, int order, double ppc, int* lo, int* ext)
+{
+    int const cap   = dim == 3 ? Tile<3>::NODES : Tile<2>::NODES;
+    int const CHUNK = chunkSize(dim, order);
+    int const mlo = 2, mhi = order + 2;
+    for (int d = 0; d < 3; ++d)
+    {
+        lo[d]  = d < dim ? mlo : 0;
+        ext[d] = d < dim ? 1 + mlo + mhi : 1;
+    }
+    int fast     = dim - 1;
+    double cells = ppc > 0 ? CHUNK / ppc : CHUNK;
+    int want     = static_cast<int>(cells) + 2 + mlo + mhi;
+    int others   = 1;
+    for (int d = 0; d < dim - 1; ++d)
+        others *= ext[d];
+    int maxFast = cap / others;
+    ext[fast]   = want < maxFast ? want : maxFast;
 }
 
 template<int D, int O>
@@ -493,8 +644,11 @@ static void launchDepositT(PushArgs const& A, PView const& in, hipStream_t s)
 {
     if (A.n == 0)
         return;
-    unsigned int const grid = static_cast<unsigned int>((A.n + BLOCK - 1) / BLOCK);
-    k_deposit<D, O><<<grid, BLOCK, 0, s>>>(A, in);
+    PushArgs B = A;
+    B.pushOn   = 0;
+    constexpr int CH = Cfg<D, O>::CHUNK;
+    unsigned int const grid = static_cast<unsigned int>((A.n + CH - 1) / CH);
+    k_push<D, O><<<grid, BLOCK, 0, s>>>(B, in, in);
 }
 
 #define PHX_DISPATCH_DO(FN, ...)                                                                   \
