@@ -29,6 +29,11 @@ from phare_amd import workloads as W  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 
+# HBM bytes per k_push dispatch (mode all) from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of
+# this script, FETCH_SIZE scaled x2 and WRITE_SIZE x1 by a 1 GiB device copy's readings; keyed by
+# workload (kernel variant, precision), measured for the default Harris workload only
+MEASURED_TRAFFIC = {("harris2d", "fp64"): 395.0e6}
+
 
 def _hip():
     for name in ("libamdhip64.so", "/opt/rocm/lib/libamdhip64.so"):
@@ -43,6 +48,22 @@ def device_sync():
     rc = _hip().hipDeviceSynchronize()
     if rc != 0:
         raise RuntimeError(f"hipDeviceSynchronize failed: {rc}")
+
+
+def calibrate_copy(nbytes):
+    """one device-to-device copy of nbytes (HIP's blit kernel), so that the FETCH_SIZE /
+    WRITE_SIZE counters of a kernel with known traffic can scale the push kernel's readings"""
+    hip = _hip()
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(src), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(dst), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(src, 1, ctypes.c_size_t(nbytes)) == 0
+    for _ in range(3):
+        assert hip.hipMemcpy(dst, src, ctypes.c_size_t(nbytes), 3) == 0  # device to device
+    device_sync()
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    print(json.dumps({"calibration": "device-to-device copy", "bytes": nbytes, "copies": 3}))
 
 
 def workload(name):
@@ -106,7 +127,12 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=20)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
+                    help="only run a device-to-device copy of BYTES (counter calibration)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
     if args.gpus != 1:
         print("bench.py runs on one GPU: this build has no multi-GPU path (patch sharding over "
               "xGMI is a design, not built); use --gpus 1", file=sys.stderr)
@@ -131,7 +157,8 @@ def main():
     ms_step = 1e3 * elapsed / args.steps
     value = wl.n_particles * args.steps / elapsed
 
-    # dominant kernel: the fused push + gather + deposit, timed with HIP events on its stream
+    # dominant kernel: the fused push + gather + deposit, captured alone in a graph and replayed
+    # back to back between two HIP events on the level's stream
     ms_all = lvl.profile_push(wl.dt, phx.MODE_ALL, args.kernel_reps)
     ms_dom = lvl.profile_push(wl.dt, phx.MODE_DOMAIN_ONLY, args.kernel_reps)
     bytes_all = push_bytes(wl, phx.MODE_ALL)
@@ -159,7 +186,8 @@ def main():
                    "particles": wl.n_particles, "cells": wl.n_cells, "interp_order": wl.order,
                    "precision": "fp64", "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": MEASURED_TRAFFIC.get((args.workload, "fp64")),
                      "kernel": "k_push (fused prePush+gather+Boris+postPush+deposit, mode all)",
                      "kernel_ms": ms_all, "kernel_ms_domain_only": ms_dom,
                      "algorithmic_bytes_per_launch": bytes_all},

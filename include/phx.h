@@ -175,9 +175,10 @@ phx_status phx_diagnostics(phx_ctx* ctx, double out[4]);
 void* phx_stream(phx_ctx* ctx);
 
 /* ---- measurement (no counterpart in the reference: a benchmark hook) ------------------------ */
-/* launches only the fused push+gather+deposit kernel of a periodic level (population 0, E/B of
-   the averaged set) `reps` times, each between two HIP events on the context stream, and returns
-   the mean kernel time in milliseconds. Mode PHX_MODE_ALL advances the particles in place. */
+/* captures only the fused push+gather+deposit kernel of a periodic level (population 0, E/B of
+   the averaged set) in a hipGraph, replays it `reps` times back to back between two HIP events
+   on the context stream, and returns the mean kernel time in milliseconds. Mode PHX_MODE_ALL
+   advances the particles in place. */
 phx_status phx_profile_push(phx_ctx* ctx, double dt, int mode, int reps, double* avg_ms);
 
 #ifdef __cplusplus
