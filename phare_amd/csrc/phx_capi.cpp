@@ -20,6 +20,22 @@
 #include <string>
 #include <vector>
 
+// LDS tile policy of the push kernel: moment copies per dimension (lanes of a wavefront add into
+// different copies)
+#ifndef PHX_TILE_COPIES_1D
+#define PHX_TILE_COPIES_1D 1
+#endif
+#ifndef PHX_TILE_COPIES_2D
+#define PHX_TILE_COPIES_2D 8
+#endif
+#ifndef PHX_TILE_COPIES_3D
+#define PHX_TILE_COPIES_3D 2
+#endif
+// moment copies of the segmented push (its lanes own distinct cells)
+#ifndef PHX_SEG_COPIES
+#define PHX_SEG_COPIES 1
+#endif
+
 using namespace phx;
 
 namespace
@@ -28,6 +44,7 @@ namespace
 struct Store
 {
     size_t n = 0, cap = 0;
+    bool seg = false; // cell-segmented by the sort (the population's segment tables describe it)
     int32_t* ic[3] = {nullptr, nullptr, nullptr};
     double* de[3]  = {nullptr, nullptr, nullptr};
     double* v[3]   = {nullptr, nullptr, nullptr};
@@ -54,6 +71,8 @@ struct Pop
     double mass = 1.;
     Store arr[3]; // domain, patch ghost, level ghost
     double* mom[5] = {};
+    SegTables seg{}; // tables of the segmented domain array (items sized for itemsAlloc items)
+    unsigned int itemsAlloc = 0;
 };
 
 } // namespace
@@ -78,7 +97,6 @@ struct phx_ctx
     // the totals before their border max
     double* raw[2][PHX_MAX_POP][5] = {};
     int rawCur                     = 0;
-    int graphRawCur                = -1;
     double* tot5[5]                = {};
 
     DevError* err           = nullptr;
@@ -89,8 +107,24 @@ struct phx_ctx
     size_t blockCap         = 0;
     unsigned long long* tot = nullptr;
 
-    hipGraphExec_t graph = nullptr;
-    double graphDt       = std::numeric_limits<double>::quiet_NaN();
+    // cell sort of the domain arrays: a second store per population (the sort writes there and
+    // the two swap), the key counts, and the interval in PPC steps of phx_advance
+    Store sortBuf[PHX_MAX_POP];
+    SegGeom sg{};
+    unsigned int* segWork = nullptr; // per-cell counts and per-segment work items of a sort
+    int sortInterval      = 25;
+    int stepsSinceSort    = 0;
+
+    // captured PPC steps, keyed on dt, the raw deposit set and the particle storage they address
+    struct GraphEntry
+    {
+        hipGraphExec_t exec = nullptr;
+        double dt           = 0.;
+        int rawCur          = -1;
+        std::vector<unsigned long long> key;
+    };
+    GraphEntry graphs[2];
+    int graphNext = 0;
     std::string lastError;
 };
 
@@ -293,7 +327,9 @@ phx_status checkDeviceError(phx_ctx* c)
     return fail(c, static_cast<phx_status>(e.code), os.str());
 }
 
-PushArgs pushArgs(phx_ctx* c, int pop, int kind, int mode, int emSet, double dt)
+// arguments of the push kernel; `segOk` lets a cell-segmented domain array use the segmented
+// kernel (callers that need a fate for every slot, the compacting path, pass false)
+PushArgs pushArgs(phx_ctx* c, int pop, int kind, int mode, int emSet, double dt, bool segOk)
 {
     PushArgs A{};
     A.G = c->G;
@@ -322,7 +358,19 @@ PushArgs pushArgs(phx_ctx* c, int pop, int kind, int mode, int emSet, double dt)
     double ppc = static_cast<double>(A.n);
     for (int d = 0; d < c->cfg.dim; ++d)
         ppc /= c->G.n[d];
-    tileShape(c->cfg.dim, c->cfg.interp_order, ppc, A.tileLo, A.tileExt);
+    int const copies = c->cfg.dim == 1 ? PHX_TILE_COPIES_1D
+                                       : (c->cfg.dim == 2 ? PHX_TILE_COPIES_2D : PHX_TILE_COPIES_3D);
+    A.tile = tileShape(c->cfg.dim, c->cfg.interp_order, ppc, copies);
+    Pop const& P   = c->pops[pop];
+    Store const& s = P.arr[kind];
+    if (segOk && kind == KIND_DOMAIN && s.seg && segItemsCap(c->sg, s.cap) <= P.itemsAlloc)
+    {
+        A.segOn       = 1;
+        A.sg          = c->sg;
+        A.st          = P.seg;
+        A.st.itemsCap = segItemsCap(c->sg, s.cap);
+        A.tile        = segTileShape(c->cfg.dim, c->cfg.interp_order, c->sg, PHX_SEG_COPIES);
+    }
     return A;
 }
 
@@ -417,7 +465,7 @@ phx_status enqueueMoveIonsFast(phx_ctx* c, double dt, int mode)
     M.npop = c->cfg.n_pop;
     for (int p = 0; p < c->cfg.n_pop; ++p)
     {
-        PushArgs A = pushArgs(c, p, KIND_DOMAIN, mode, PHX_EM_AVG, dt);
+        PushArgs A = pushArgs(c, p, KIND_DOMAIN, mode, PHX_EM_AVG, dt, true);
         for (int m = 0; m < 5; ++m)
             A.mom[m] = c->raw[cur][p][m];
         A.periodicWrap = 1;
@@ -551,6 +599,7 @@ phx_status upload(phx_ctx* c, Store& s, size_t n, int32_t const* ic, double cons
 {
     int const D = c->cfg.dim;
     PHX_TRY(storeReserve(c, s, offset + n));
+    s.seg = false;
     // appended particles start a fresh chunk alignment only when the array was empty
     std::vector<size_t> const ord = offset == 0 ? chunkInterleave(n, D, c->cfg.interp_order)
                                                 : chunkInterleave(0, D, c->cfg.interp_order);
@@ -646,6 +695,114 @@ size_t recordSize(int dim) { return dim == 1 ? 56 : (dim == 2 ? 64 : 80); }
 // =============================================================================================
 // C ABI
 // =============================================================================================
+namespace
+{
+// counting sort of every population's domain array into cell-segmented storage in its sort
+// buffer; the two swap. Lost particles are dropped.
+phx_status sortDomain(phx_ctx* c)
+{
+    for (int p = 0; p < c->cfg.n_pop; ++p)
+    {
+        Pop& P     = c->pops[p];
+        Store& dom = P.arr[PHX_DOMAIN];
+        Store& buf = c->sortBuf[p];
+        launchSegCount(c->sg, dom.view(), dom.n, c->segWork, P.seg, c->stream);
+        PHX_TRY(checkLaunch(c, "particle sort (count)"));
+        unsigned long long slots = 0;
+        PHX_HIP(c, hipStreamSynchronize(c->stream));
+        PHX_HIP(c, hipMemcpy(&slots, P.seg.off + c->sg.nsegs + 1, 8, hipMemcpyDeviceToHost));
+        if (buf.cap < slots || buf.cap < dom.cap)
+        {
+            storeFree(c, buf);
+            PHX_TRY(storeAlloc(c, buf, std::max<size_t>(std::max<size_t>(slots, dom.cap), 1024)));
+        }
+        // work items for the larger of the two storages the sort alternates between
+        unsigned int const need = segItemsCap(c->sg, std::max(buf.cap, dom.cap));
+        if (need > P.itemsAlloc)
+        {
+            devFree(c, P.seg.items);
+            P.seg.items = nullptr;
+            PHX_TRY(devAlloc(c, (void**)&P.seg.items, static_cast<size_t>(need) * 8));
+            P.itemsAlloc = need;
+        }
+        launchSegScatter(c->sg, dom.view(), buf.view(), dom.n, c->segWork, P.seg, c->cfg.dim,
+                         c->stream);
+        PHX_TRY(checkLaunch(c, "particle sort (scatter)"));
+        buf.n   = slots;
+        buf.seg = true;
+        std::swap(dom, buf);
+        buf.seg = false;
+    }
+    c->stepsSinceSort = 0;
+    return PHX_OK;
+}
+
+// the particle storage a captured step addresses: pointers and counts of every array
+std::vector<unsigned long long> storageKey(phx_ctx const* c)
+{
+    std::vector<unsigned long long> k;
+    for (int p = 0; p < c->cfg.n_pop; ++p)
+        for (Store const& s : c->pops[p].arr)
+        {
+            // segmented storage: the kernels read their extent from the segment tables, so a
+            // sort that changes the slot count keeps the captured step
+            k.push_back(reinterpret_cast<unsigned long long>(s.ic[0]));
+            k.push_back(reinterpret_cast<unsigned long long>(s.w));
+            k.push_back(s.cap);
+            k.push_back(s.seg ? ~0ull : s.n);
+        }
+    return k;
+}
+
+// the captured PPC step for dt on the current storage (captured on first use; two are kept, so
+// that the two storages a periodic sort alternates between each keep theirs)
+phx_status stepGraph(phx_ctx* c, double dt, phx_ctx::GraphEntry** out)
+{
+    std::vector<unsigned long long> const key = storageKey(c);
+    for (auto& g : c->graphs)
+        if (g.exec && g.dt == dt && g.rawCur == c->rawCur && g.key == key)
+        {
+            *out = &g;
+            return PHX_OK;
+        }
+    phx_ctx::GraphEntry& g = c->graphs[c->graphNext];
+    c->graphNext           = (c->graphNext + 1) % 2;
+    if (g.exec)
+    {
+        PHX_HIP(c, hipStreamSynchronize(c->stream));
+        (void)hipGraphExecDestroy(g.exec);
+        g.exec = nullptr;
+    }
+    PHX_HIP(c, hipStreamSynchronize(c->stream));
+    hipGraph_t graph = nullptr;
+    int const cur0   = c->rawCur;
+    PHX_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    phx_status s = enqueueAdvanceFast(c, dt);
+    c->rawCur    = cur0; // a step uses both raw sets and ends on the one it started with
+    hipError_t e = hipStreamEndCapture(c->stream, &graph);
+    if (s != PHX_OK)
+    {
+        if (graph)
+            (void)hipGraphDestroy(graph);
+        return s;
+    }
+    if (e != hipSuccess)
+        return fail(c, PHX_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess)
+    {
+        g.exec = nullptr;
+        return fail(c, PHX_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    }
+    g.dt     = dt;
+    g.rawCur = c->rawCur;
+    g.key    = key;
+    *out     = &g;
+    return PHX_OK;
+}
+} // namespace
+
 extern "C" {
 
 phx_status phx_create(phx_config const* cfg, phx_patch const* patch, size_t particle_capacity,
@@ -756,6 +913,22 @@ phx_status phx_create(phx_config const* cfg, phx_patch const* patch, size_t part
             s = storeAlloc(c, c->pops[p].arr[k], k == 0 ? std::max<size_t>(particle_capacity, 1024) : 1024);
     }
     if (s == PHX_OK)
+    {
+        // segmented storage over the domain box
+        c->sg = segGeom(cfg->dim, c->boxes.dom_lo, c->boxes.dom_hi);
+        size_t const nsegs = static_cast<size_t>(c->sg.nsegs);
+        s = devAlloc(c, (void**)&c->segWork, (static_cast<size_t>(c->sg.ncells) + 1 + nsegs + 1) * 4);
+        for (int p = 0; p < cfg->n_pop && s == PHX_OK; ++p)
+        {
+            SegTables& T = c->pops[p].seg;
+            s = devAlloc(c, (void**)&T.rows, (nsegs + 1) * 4);
+            if (s == PHX_OK)
+                s = devAlloc(c, (void**)&T.off, (nsegs + 2) * 8);
+            if (s == PHX_OK)
+                s = devAlloc(c, (void**)&T.nItems, 64);
+        }
+    }
+    if (s == PHX_OK)
         s = devAlloc(c, (void**)&c->err, sizeof(DevError));
     if (s == PHX_OK)
         s = devAlloc(c, (void**)&c->dropped, 64);
@@ -804,8 +977,9 @@ void phx_destroy(phx_ctx* c)
         return;
     if (c->stream)
         (void)hipStreamSynchronize(c->stream);
-    if (c->graph)
-        (void)hipGraphExecDestroy(c->graph);
+    for (auto& g : c->graphs)
+        if (g.exec)
+            (void)hipGraphExecDestroy(g.exec);
     for (void* p : c->allocs)
         (void)hipFree(p);
     if (c->stream)
@@ -877,10 +1051,11 @@ phx_status phx_particle_count(phx_ctx* c, int pop, int kind, size_t* n)
     Store const& s = c->pops[pop].arr[kind];
     if (kind == PHX_DOMAIN)
     {
+        // lost particles and the empty slots of segmented storage hold dead entries
         unsigned int dropped = 0;
         PHX_HIP(c, hipStreamSynchronize(c->stream));
         PHX_HIP(c, hipMemcpy(&dropped, c->dropped, 4, hipMemcpyDeviceToHost));
-        if (dropped)
+        if (dropped || s.seg)
         {
             std::vector<int32_t> ic;
             std::vector<double> de, v, w, q;
@@ -1028,11 +1203,11 @@ phx_status phx_update_populations(phx_ctx* c, int em_set, double dt, int mode)
         if (mode == PHX_MODE_DOMAIN_ONLY)
         {
             // pushed copies: nothing written back
-            PushArgs A = pushArgs(c, p, KIND_DOMAIN, mode, em_set, dt);
+            PushArgs A = pushArgs(c, p, KIND_DOMAIN, mode, em_set, dt, true);
             launchPush(A, dom.view(), dom.view(), c->stream);
             if (lg.n)
             {
-                PushArgs L = pushArgs(c, p, KIND_LEVEL_GHOST, mode, em_set, dt);
+                PushArgs L = pushArgs(c, p, KIND_LEVEL_GHOST, mode, em_set, dt, false);
                 launchPush(L, lg.view(), lg.view(), c->stream);
             }
             PHX_TRY(checkLaunch(c, "push"));
@@ -1041,7 +1216,7 @@ phx_status phx_update_populations(phx_ctx* c, int em_set, double dt, int mode)
         // mode all: existing patch ghosts are deposited unpushed
         if (pg.n)
         {
-            PushArgs G = pushArgs(c, p, KIND_PATCH_GHOST, mode, em_set, dt);
+            PushArgs G = pushArgs(c, p, KIND_PATCH_GHOST, mode, em_set, dt, false);
             launchDeposit(G, pg.view(), c->stream);
         }
         size_t const nd = dom.n, nl = lg.n;
@@ -1049,7 +1224,7 @@ phx_status phx_update_populations(phx_ctx* c, int em_set, double dt, int mode)
         // worst-case room: every domain particle may leave, every level ghost may enter
         PHX_TRY(storeReserve(c, pg, pg.n + nd));
         // domain: push in place with a fate per particle, then split keep / leave
-        PushArgs A = pushArgs(c, p, KIND_DOMAIN, mode, em_set, dt);
+        PushArgs A = pushArgs(c, p, KIND_DOMAIN, mode, em_set, dt, false);
         A.status   = c->status;
         launchPush(A, dom.view(), dom.view(), c->stream);
         PHX_TRY(checkLaunch(c, "push"));
@@ -1067,15 +1242,17 @@ phx_status phx_update_populations(phx_ctx* c, int em_set, double dt, int mode)
         PHX_HIP(c, hipStreamSynchronize(c->stream));
         PHX_HIP(c, hipMemcpy(&got, c->tot, 8, hipMemcpyDeviceToHost));
         std::swap(dom, c->scratch);
-        dom.n        = got;
-        c->scratch.n = 0;
+        dom.n          = got;
+        dom.seg        = false;
+        c->scratch.n   = 0;
+        c->scratch.seg = false;
         if (nl)
         {
             // level ghosts (:275-288): pushed with inGhostBox / inGhostLayer selection; those that
             // entered the domain were deposited by the push and join the domain array
             PHX_TRY(storeReserve(c, dom, dom.n + nl));
             PHX_TRY(scratchReserve(c, nl));
-            PushArgs L = pushArgs(c, p, KIND_LEVEL_GHOST, mode, em_set, dt);
+            PushArgs L = pushArgs(c, p, KIND_LEVEL_GHOST, mode, em_set, dt, false);
             L.status   = c->status;
             launchPush(L, lg.view(), lg.view(), c->stream);
             PHX_TRY(checkDeviceError(c));
@@ -1143,6 +1320,7 @@ phx_status phx_fill_ion_ghost_particles(phx_ctx* c)
             launchCompact(pg.view(), dom.view(), c->status, pg.n, ST_KEEP, c->blockCnt, c->tot, dom.n,
                           D, wrap, c->stream);
             dom.n += pg.n;
+            dom.seg = false;
         }
         pg.n = 0;
     }
@@ -1228,15 +1406,32 @@ phx_status phx_move_ions(phx_ctx* c, double dt, int mode)
     return checkDeviceError(c);
 }
 
+
+phx_status phx_sort_particles(phx_ctx* c)
+{
+    if (!c)
+        return PHX_ERR_ARG;
+    return sortDomain(c);
+}
+
+phx_status phx_set_sort_interval(phx_ctx* c, int steps)
+{
+    if (!c || steps < 0)
+        return PHX_ERR_ARG;
+    c->sortInterval = steps;
+    return PHX_OK;
+}
+
 phx_status phx_initialize(phx_ctx* c)
 {
     if (!c)
         return PHX_ERR_ARG;
     fillGhosts(c, c->em[PHX_EM] + 3, QBX);
+    PHX_TRY(sortDomain(c));
     zeroMoments(c);
     for (int p = 0; p < c->cfg.n_pop; ++p)
     {
-        PushArgs A = pushArgs(c, p, KIND_DOMAIN, PHX_MODE_ALL, PHX_EM, 0.);
+        PushArgs A = pushArgs(c, p, KIND_DOMAIN, PHX_MODE_ALL, PHX_EM, 0., true);
         launchDeposit(A, c->pops[p].arr[PHX_DOMAIN].view(), c->stream);
     }
     PHX_TRY(checkLaunch(c, "initial deposit"));
@@ -1268,43 +1463,19 @@ phx_status phx_advance(phx_ctx* c, double dt, int nsteps)
             enqueueAverage(c);
             PHX_TRY(phx_move_ions(c, dt, PHX_MODE_ALL));
             enqueueCorrector(c, dt);
+            if (c->sortInterval > 0 && ++c->stepsSinceSort >= c->sortInterval)
+                PHX_TRY(sortDomain(c));
         }
         return checkDeviceError(c);
     }
-    if (!c->graph || c->graphDt != dt || c->graphRawCur != c->rawCur)
-    {
-        if (c->graph)
-        {
-            (void)hipGraphExecDestroy(c->graph);
-            c->graph = nullptr;
-        }
-        PHX_HIP(c, hipStreamSynchronize(c->stream));
-        hipGraph_t g = nullptr;
-        int const cur0 = c->rawCur;
-        PHX_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        phx_status s = enqueueAdvanceFast(c, dt);
-        c->rawCur    = cur0; // a step uses both raw sets and ends on the one it started with
-        hipError_t e = hipStreamEndCapture(c->stream, &g);
-        if (s != PHX_OK)
-        {
-            if (g)
-                (void)hipGraphDestroy(g);
-            return s;
-        }
-        if (e != hipSuccess)
-            return fail(c, PHX_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
-        e = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        if (e != hipSuccess)
-        {
-            c->graph = nullptr;
-            return fail(c, PHX_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
-        }
-        c->graphDt     = dt;
-        c->graphRawCur = c->rawCur;
-    }
     for (int s = 0; s < nsteps; ++s)
-        PHX_HIP(c, hipGraphLaunch(c->graph, c->stream));
+    {
+        phx_ctx::GraphEntry* g = nullptr;
+        PHX_TRY(stepGraph(c, dt, &g));
+        PHX_HIP(c, hipGraphLaunch(g->exec, c->stream));
+        if (c->sortInterval > 0 && ++c->stepsSinceSort >= c->sortInterval)
+            PHX_TRY(sortDomain(c));
+    }
     return PHX_OK;
 }
 
@@ -1368,7 +1539,7 @@ phx_status phx_profile_push(phx_ctx* c, double dt, int mode, int reps, double* a
     // the push kernel alone, captured in a graph and replayed back to back between two events
     // on the context stream (the mean over the replays is the kernel time; per-launch event
     // pairs around direct launches read high on this runtime)
-    PushArgs A     = pushArgs(c, 0, KIND_DOMAIN, mode, PHX_EM_AVG, dt);
+    PushArgs A     = pushArgs(c, 0, KIND_DOMAIN, mode, PHX_EM_AVG, dt, true);
     A.periodicWrap = 1;
     PView v        = c->pops[0].arr[PHX_DOMAIN].view();
     PHX_HIP(c, hipStreamSynchronize(c->stream));

@@ -173,13 +173,10 @@ __device__ __forceinline__ bool stencilInside(Geom const& G, Stencil<D, O> const
     return true;
 }
 
+// start index and weights of quantity q's stencil
 template<int D, int O>
-__device__ __forceinline__ double interp(Geom const& G, double const* __restrict__ f, int q,
-                                         Stencil<D, O> const& s)
+__device__ __forceinline__ void stencilOf(int q, Stencil<D, O> const& s, int* st, double const** w)
 {
-    constexpr int S = O + 1;
-    int st[D];
-    double const* w[D];
 #pragma unroll
     for (int d = 0; d < D; ++d)
     {
@@ -187,6 +184,15 @@ __device__ __forceinline__ double interp(Geom const& G, double const* __restrict
         st[d]  = p ? s.sp[d] : s.sd[d];
         w[d]   = p ? s.wp[d] : s.wd[d];
     }
+}
+
+// interpolation of a C-order array with row strides s1 (dimension 1) and s2 (dimension 2) from
+// the stencil starting at st, in the reference's order of sums (MeshToParticle)
+template<int D, int O>
+__device__ __forceinline__ double interpStrided(double const* __restrict__ f, int const* st,
+                                                double const* const* w, int s1, int s2)
+{
+    constexpr int S = O + 1;
     if constexpr (D == 1)
     {
         double r = 0.;
@@ -197,8 +203,7 @@ __device__ __forceinline__ double interp(Geom const& G, double const* __restrict
     }
     else if constexpr (D == 2)
     {
-        int const s1 = G.shape[q][1];
-        double r     = 0.;
+        double r = 0.;
 #pragma unroll
         for (int i = 0; i < S; ++i)
         {
@@ -213,8 +218,7 @@ __device__ __forceinline__ double interp(Geom const& G, double const* __restrict
     }
     else
     {
-        int const s1 = G.shape[q][1], s2 = G.shape[q][2];
-        double r     = 0.;
+        double r = 0.;
 #pragma unroll
         for (int i = 0; i < S; ++i)
         {
@@ -233,6 +237,16 @@ __device__ __forceinline__ double interp(Geom const& G, double const* __restrict
         }
         return r;
     }
+}
+
+template<int D, int O>
+__device__ __forceinline__ double interp(Geom const& G, double const* __restrict__ f, int q,
+                                         Stencil<D, O> const& s)
+{
+    int st[D];
+    double const* w[D];
+    stencilOf<D, O>(q, s, st, w);
+    return interpStrided<D, O>(f, st, w, G.shape[q][1], G.shape[q][2]);
 }
 
 struct EMPtrs

@@ -51,6 +51,44 @@ struct Boxes
     int nlgb_lo[MAX_NLGB][3], nlgb_hi[MAX_NLGB][3];
 };
 
+// LDS tile of one workgroup: a box of nodes placed over the chunk's cells, holding `copies`
+// copies of the 5 moment accumulators. Every slab (one moment of one copy) is `stride` doubles
+// long; stride = nodes rounded up to 16, plus 1, so that the copies of a node sit in different
+// LDS banks.
+struct TileSpec
+{
+    int lo[3];  // margin below the lowest cell of the chunk (nodes)
+    int ext[3]; // extents
+    int nodes;  // ext[0]*ext[1]*ext[2]
+    int copies; // moment copies (power of two): lane l accumulates into copy l % copies
+    int stride;
+    unsigned int bytes; // dynamic LDS of the launch
+};
+
+// Cell-segmented storage of a domain array: the domain box is cut into segments, boxes of at
+// most 64 cells (shape `seg`); a segment holds rows of 64 slots, slot l of row r being the r-th
+// particle of the segment's l-th cell (row-major inside the segment), empty slots holding dead
+// particles. One more segment at the end holds, unordered, the particles outside the domain box.
+struct SegGeom
+{
+    int dim;
+    int seg[3];  // segment shape in cells
+    int nseg[3]; // segments per dimension
+    int nsegs;   // segments over the domain box (the overflow segment is number nsegs)
+    int lo[3];   // lowest AMR cell of the domain box
+    int n[3];    // domain cells per dimension
+    int ncells;  // n[0]*n[1]*n[2]
+};
+// tables of the current segmented storage, rebuilt by each sort
+struct SegTables
+{
+    unsigned int* rows;       // [nsegs + 1] rows per segment
+    unsigned long long* off;  // [nsegs + 2] first slot per segment; off[nsegs + 1] = all slots
+    unsigned int* items;      // [2 * itemsCap] (segment, first row) of each work item
+    unsigned int* nItems;     // number of work items
+    unsigned int itemsCap;    // work items the storage's capacity allows (grid of the push)
+};
+
 struct PushArgs
 {
     Geom G;
@@ -61,13 +99,15 @@ struct PushArgs
     double coef;
     Boxes box;
     int kind, mode, periodicWrap, pop;
-    int pushOn;     // 0: deposit only (unpushed arrays, initial moments)
-    int tileLo[3];  // LDS tile origin relative to the chunk's first cell (local node index)
-    int tileExt[3]; // LDS tile extents (product <= the tile capacity)
+    int pushOn; // 0: deposit only (unpushed arrays, initial moments)
+    TileSpec tile;
     unsigned long long n;
     DevError* err;
     uint8_t* status;
     unsigned int* dropped;
+    int segOn; // the array is cell-segmented (sg, st valid)
+    SegGeom sg;
+    SegTables st;
 };
 
 struct Wrap
@@ -78,10 +118,25 @@ struct Wrap
 
 void launchPush(PushArgs const& A, PView const& in, PView const& out, hipStream_t s);
 void launchDeposit(PushArgs const& A, PView const& in, hipStream_t s);
-void tileShape(int dim, int order, double ppc, int* lo, int* ext);
+// tile of a level whose arrays hold `ppc` particles per cell on average
+TileSpec tileShape(int dim, int order, double ppc, int copies);
 int chunkSize(int dim, int order);
 int blockThreads();
 unsigned int compactBlocks(unsigned long long n);
+
+// segmented storage: geometry of a domain box, LDS tile of the segmented push, work rows per item
+SegGeom segGeom(int dim, int const* lo, int const* hi);
+TileSpec segTileShape(int dim, int order, SegGeom const& sg, int copies);
+int segRowsPerItem();
+unsigned int segItemsCap(SegGeom const& sg, unsigned long long capacity);
+// first half of a sort of src (n slots) into cell-segmented storage: counts per cell and the
+// segment tables up to the total slot count (tables.off[nsegs + 1]); `work` holds
+// ncells + 1 + nsegs + 1 words
+void launchSegCount(SegGeom const& sg, PView const& src, unsigned long long n, unsigned int* work,
+                    SegTables const& tables, hipStream_t s);
+// second half, once dst can hold the slot count: work items, scatter and empty-slot marking
+void launchSegScatter(SegGeom const& sg, PView const& src, PView const& dst, unsigned long long n,
+                      unsigned int* work, SegTables const& tables, int dim, hipStream_t s);
 void launchCompact(PView const& src, PView const& dst, uint8_t const* status, unsigned long long n,
                    uint8_t want, unsigned int* blockCounts, unsigned long long* total,
                    unsigned long long dstOffset, int dim, Wrap const& wrap, hipStream_t s);

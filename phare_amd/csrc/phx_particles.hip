@@ -1,45 +1,60 @@
 // phx_particles.hip -- particle kernels for gfx950: fused prePush + gather + Boris + postPush +
 // moment deposit (IonUpdater::updatePopulations, ion_updater.hpp:88-295, with the Boris pusher
-// boris.hpp:93-138 and the Interpolator deposit interpolator.hpp:468-510), stable compaction of
-// particle arrays, and the periodic re-entry of leaving particles.
+// boris.hpp:93-138 and the Interpolator gather/deposit interpolator.hpp:420-510), the counting
+// sort of domain arrays into cell-segmented storage, stable compaction of particle arrays, and
+// the periodic re-entry of leaving particles.
 //
 // Particles are structure-of-arrays in HBM (int32 iCell[D], f64 delta[D], f64 v[3], f64 weight,
-// f64 charge), each component a contiguous stream. A workgroup owns a chunk of 1024 consecutive
-// particles (kept close to cell order by the loader), accumulates their moments in an LDS tile
-// spanning the chunk's deposit footprint, then flushes the tile to the moment arrays. Lanes take
-// particles with a stride across the chunk so that lanes of one wavefront fall into different
-// cells and the LDS accumulations of a wavefront rarely hit the same node.
+// f64 charge), each component a contiguous stream, in one of two storage orders:
+//  - linear: a workgroup owns a chunk of consecutive particles, lanes taking them with a stride
+//    across the chunk (any order is correct; cell order is faster);
+//  - cell-segmented (built by the sort): the 64 lanes of a wavefront read one row of a segment,
+//    so that each lane owns one cell and consecutive lanes own neighbouring cells. Reads stay
+//    coalesced and the lanes of one LDS deposit instruction add into distinct words.
+// Either way a workgroup accumulates its moments in an LDS tile over its cells and flushes the
+// tile with one HBM atomic per touched node and moment; the gather reads E/B through the L1/L2
+// caches, leaving the LDS to the deposit. Particles whose stencil leaves the tile deposit
+// straight into HBM.
 #include "phx_kernels.hpp"
+
+#include <algorithm>
+
+#ifndef PHX_PPT_ND
+#define PHX_PPT_ND 2 // particles per thread of the linear 2D/3D push
+#endif
+#ifndef PHX_SEG_ROWS
+#define PHX_SEG_ROWS 32 // segment rows per workgroup of the segmented push
+#endif
 
 namespace phx
 {
 
 namespace
 {
-constexpr int BLOCK = 256;
-constexpr int NW    = BLOCK / 64; // wavefronts per workgroup
+constexpr int BLOCK    = 256;
+constexpr int NW       = BLOCK / 64; // wavefronts per workgroup
+constexpr int LANES    = 64;
+constexpr int SEG_ROWS = PHX_SEG_ROWS;
+constexpr unsigned int TILE_BYTES_MAX = 65536;
+#ifndef PHX_TILE_SPREAD
+#define PHX_TILE_SPREAD 1
+#endif
+constexpr int TILE_SPREAD = PHX_TILE_SPREAD; // cells of drift since the sort in a segment tile
 
 // Per (dim, order): whether the deposit is accumulated in registers over a union window, the
-// particles per thread and the window. With accumulation, each thread owns PPT consecutive
-// particles of the caller's order (mostly one cell) and scatters to LDS only when its cell
-// changes. Only 1D keeps the window (5 moments x 4 nodes) in registers without losing
+// particles per thread of the linear push and the window. With accumulation, a thread's
+// successive particles (mostly one cell) add into the window and scatter to LDS only when the
+// cell changes. Only 1D keeps the window (5 moments x 4 nodes) in registers without losing
 // occupancy; 2D/3D scatter every particle to the LDS tile.
 template<int D, int O>
 struct Cfg
 {
     static constexpr bool ACC  = D == 1;
-    static constexpr int PPT   = ACC ? 8 : 2;
+    static constexpr int PPT   = ACC ? 8 : PHX_PPT_ND;
     static constexpr int CHUNK = BLOCK * PPT;
     static constexpr int W     = O == 1 ? 2 : 4;  // window width per direction
     static constexpr int WB    = O == 1 ? 0 : 1;  // window base = local cell - WB
     static constexpr int WN    = D == 1 ? W : (D == 2 ? W * W : W * W * W);
-};
-
-// LDS moment tile: 5 moments x TILE nodes (20 KB in 1D/2D, 40 KB in 3D)
-template<int D>
-struct Tile
-{
-    static constexpr int NODES = D == 3 ? 1024 : 512;
 };
 
 __device__ __forceinline__ bool inBox(int const* c, int const* lo, int const* hi, int D)
@@ -58,11 +73,28 @@ __device__ __forceinline__ bool inNLGB(int const* c, Boxes const& b, int D)
     return false;
 }
 
-// (thread, step) -> slot in the chunk: consecutive threads read consecutive slots (coalesced).
-// Arrays are stored chunk-interleaved (chunkInterleave on the host): slot k*BLOCK + t holds the
-// particle (r*PPT + k) of the caller's order, r = lane*NW + wave. Each thread thus owns PPT
-// consecutive particles, and the lanes of a wavefront own runs spread across the chunk.
+// linear storage: (thread, step) -> slot in the chunk; consecutive threads read consecutive slots
+// (coalesced). The loader's arrays are stored chunk-interleaved (chunkInterleave on the host):
+// slot k*BLOCK + t holds the particle (r*PPT + k) of the caller's order, r = lane*NW + wave, so
+// the lanes of a wavefront own runs spread across the chunk.
 __device__ __forceinline__ int chunkOffset(int t, int k) { return k * BLOCK + t; }
+
+// LDS view of the tile: moment slab (m, copy)
+struct TileView
+{
+    double* lds;
+    int org[3];
+    int ext[3];
+    int copies, stride;
+    __device__ double* mom(int m, int copy) const { return lds + (m * copies + copy) * stride; }
+    __device__ int node(int i, int j, int k) const { return (i * ext[1] + j) * ext[2] + k; }
+    // the O+1 nodes from start lie in the tile along d
+    template<int O>
+    __device__ bool holds(int d, int start) const
+    {
+        return start >= org[d] && start + O < org[d] + ext[d];
+    }
+};
 
 template<int D, int O>
 struct DepositData
@@ -98,22 +130,38 @@ __device__ __forceinline__ double nodeWeight(DepositData<D, O> const& dd, double
         return dep * dd.w[0][a] * dd.w[1][b] * dd.w[2][c];
 }
 
-// scatter-add the 5 moments of one particle: into the LDS tile when its whole stencil lies in
-// the tile, else straight into the HBM moment arrays
+// E and B at the particle (Interpolator::operator() for the fields, interpolator.hpp:420-456)
 template<int D, int O>
-__device__ __forceinline__ void depositOne(DepositData<D, O> const& dd, Geom const& G, double* tile,
-                                           int const* torg, int const* text, double* const* mom)
+__device__ __forceinline__ void gatherEB(Geom const& G, EMPtrs const& em, Stencil<D, O> const& s,
+                                         double* E, double* B)
+{
+    E[0] = interp<D, O>(G, em.E[0], QEX, s);
+    E[1] = interp<D, O>(G, em.E[1], QEY, s);
+    E[2] = interp<D, O>(G, em.E[2], QEZ, s);
+    B[0] = interp<D, O>(G, em.B[0], QBX, s);
+    B[1] = interp<D, O>(G, em.B[1], QBY, s);
+    B[2] = interp<D, O>(G, em.B[2], QBZ, s);
+}
+
+// scatter-add the 5 moments of one particle: into the lane's copy of the LDS tile when its
+// whole stencil lies in the tile, else straight into the HBM moment arrays
+template<int D, int O>
+__device__ __forceinline__ void depositOne(DepositData<D, O> const& dd, Geom const& G,
+                                           TileView const& T, int copy, double* const* mom)
 {
     constexpr int S  = O + 1;
     constexpr int SY = D > 1 ? S : 1;
     constexpr int SZ = D > 2 ? S : 1;
-    constexpr int TN = Tile<D>::NODES;
     bool inTile      = true;
 #pragma unroll
     for (int d = 0; d < D; ++d)
-        inTile = inTile && dd.start[d] >= torg[d] && dd.start[d] + O < torg[d] + text[d];
+        inTile = inTile && T.template holds<O>(d, dd.start[d]);
     if (inTile)
     {
+        double* slab[5];
+#pragma unroll
+        for (int m = 0; m < 5; ++m)
+            slab[m] = T.mom(m, copy);
 #pragma unroll
         for (int a = 0; a < S; ++a)
 #pragma unroll
@@ -121,19 +169,21 @@ __device__ __forceinline__ void depositOne(DepositData<D, O> const& dd, Geom con
 #pragma unroll
                 for (int c = 0; c < SZ; ++c)
                 {
-                    int i    = dd.start[0] + a - torg[0];
-                    int j    = D > 1 ? dd.start[1] + b - torg[1] : 0;
-                    int k    = D > 2 ? dd.start[2] + c - torg[2] : 0;
-                    int node = (i * text[1] + j) * text[2] + k;
+                    int const node = T.node(dd.start[0] + a - T.org[0],
+                                            D > 1 ? dd.start[1] + b - T.org[1] : 0,
+                                            D > 2 ? dd.start[2] + c - T.org[2] : 0);
 #pragma unroll
                     for (int m = 0; m < 5; ++m)
-                        atomicAdd(tile + m * TN + node, nodeWeight<D, O>(dd, dd.val[m], a, b, c));
+                        atomicAdd(slab[m] + node, nodeWeight<D, O>(dd, dd.val[m], a, b, c));
                 }
     }
     else
     {
+#pragma unroll
         for (int a = 0; a < S; ++a)
+#pragma unroll
             for (int b = 0; b < SY; ++b)
+#pragma unroll
                 for (int c = 0; c < SZ; ++c)
                 {
                     int node = G.idx(QMOM, dd.start[0] + a, D > 1 ? dd.start[1] + b : 0,
@@ -167,14 +217,13 @@ __device__ __forceinline__ void accZero(Acc<D, O>& A)
     A.live = false;
 }
 
-// scatter the window: nodes inside the LDS tile by LDS atomics, others straight to HBM;
-// never-touched nodes hold exactly zero and are skipped
+// scatter the window: nodes inside the LDS tile by LDS atomics into the lane's copy, others
+// straight to HBM; never-touched nodes hold exactly zero and are skipped
 template<int D, int O>
-__device__ __forceinline__ void accFlush(Acc<D, O> const& A, Geom const& G, double* tile, int const* torg,
-                                         int const* text, double* const* mom)
+__device__ __forceinline__ void accFlush(Acc<D, O> const& A, Geom const& G, TileView const& T,
+                                         int copy, double* const* mom)
 {
-    constexpr int W  = Acc<D, O>::W;
-    constexpr int TN = Tile<D>::NODES;
+    constexpr int W = Acc<D, O>::W;
     if (!A.live)
         return;
 #pragma unroll
@@ -183,16 +232,13 @@ __device__ __forceinline__ void accFlush(Acc<D, O> const& A, Geom const& G, doub
         int const a = D == 1 ? x : (D == 2 ? x / W : x / (W * W));
         int const b = D == 1 ? 0 : (D == 2 ? x % W : (x / W) % W);
         int const c = D == 3 ? x % W : 0;
-        int gi = A.base[0] + a;
-        int gj = D > 1 ? A.base[1] + b : 0;
-        int gk = D > 2 ? A.base[2] + c : 0;
-        bool inTile = gi >= torg[0] && gi < torg[0] + text[0];
-        if (D > 1)
-            inTile = inTile && gj >= torg[1] && gj < torg[1] + text[1];
-        if (D > 2)
-            inTile = inTile && gk >= torg[2] && gk < torg[2] + text[2];
-        int const tnode = ((gi - torg[0]) * text[1] + (D > 1 ? gj - torg[1] : 0)) * text[2]
-                          + (D > 2 ? gk - torg[2] : 0);
+        int const g[3] = {A.base[0] + a, D > 1 ? A.base[1] + b : 0, D > 2 ? A.base[2] + c : 0};
+        bool inTile    = true;
+#pragma unroll
+        for (int d = 0; d < D; ++d)
+            inTile = inTile && g[d] >= T.org[d] && g[d] < T.org[d] + T.ext[d];
+        int const tnode = T.node(g[0] - T.org[0], D > 1 ? g[1] - T.org[1] : 0,
+                                 D > 2 ? g[2] - T.org[2] : 0);
 #pragma unroll
         for (int m = 0; m < 5; ++m)
         {
@@ -200,9 +246,9 @@ __device__ __forceinline__ void accFlush(Acc<D, O> const& A, Geom const& G, doub
             if (v != 0.)
             {
                 if (inTile)
-                    atomicAdd(tile + m * TN + tnode, v);
+                    atomicAdd(T.mom(m, copy) + tnode, v);
                 else
-                    atomicAdd(mom[m] + G.idx(QMOM, gi, gj, gk), v);
+                    atomicAdd(mom[m] + G.idx(QMOM, g[0], g[1], g[2]), v);
             }
         }
     }
@@ -251,225 +297,329 @@ __device__ __forceinline__ void accAdd(Acc<D, O>& A, DepositData<D, O> const& dd
     }
 }
 
+// One particle of updatePopulations: prePush, gather, Boris, postPush (boris.hpp:93-138), the
+// selection of ion_updater.hpp:169-295, the deposit and, in mode all, the write-back. Dead slots
+// (lost particles, empty slots of segmented storage) are skipped, and dropped by a compaction.
+template<int D, int O>
+__device__ __forceinline__ void pushParticle(PushArgs const& A, PView const& in, PView const& out,
+                                             unsigned long long i, TileView const& T, int copy,
+                                             Acc<D, O>& acc)
+{
+    using C       = Cfg<D, O>;
+    Geom const& G = A.G;
+    int cell[D];
+    double delta[D], v[3];
+    cell[0] = in.ic[0][i];
+    if (cell[0] == DEAD_CELL)
+    {
+        if (A.pushOn && A.mode == MODE_ALL && A.status)
+            A.status[i] = ST_DROP;
+        return;
+    }
+#pragma unroll
+    for (int d = 1; d < D; ++d)
+        cell[d] = in.ic[d][i];
+#pragma unroll
+    for (int d = 0; d < D; ++d)
+        delta[d] = in.de[d][i];
+    v[0] = in.v[0][i];
+    v[1] = in.v[1][i];
+    v[2] = in.v[2][i];
+    double const w = in.w[i], q = in.q[i];
+
+    bool deposit = true, sel1 = true, inDom = true;
+    uint8_t st   = ST_KEEP;
+    if (A.pushOn)
+    {
+        double bd, bv;
+        // prePushStep_ (boris.hpp:180-216)
+        if (!advancePosition<D>(cell, delta, v, A.hdtdx, bd, bv))
+        {
+            raiseError(A.err, 3u, 1u, i, bd, bv, A.kind, A.pop);
+            return;
+        }
+        // first selector: noop (domain) or inGhostBox (level ghosts)
+        sel1 = A.kind == KIND_DOMAIN ? true : inBox(cell, A.box.gh_lo, A.box.gh_hi, D);
+        if (sel1)
+        {
+            Stencil<D, O> s;
+            makeStencil<D, O>(G, cell, delta, s);
+            if (!stencilInside<D, O>(G, s, QEX) || !stencilInside<D, O>(G, s, QBX)
+                || !stencilInside<D, O>(G, s, QEY) || !stencilInside<D, O>(G, s, QBY)
+                || !stencilInside<D, O>(G, s, QEZ) || !stencilInside<D, O>(G, s, QBZ))
+            {
+                raiseError(A.err, 4u, 2u, i, delta[0], v[0], A.kind, A.pop);
+                return;
+            }
+            double E[3], B[3];
+            gatherEB<D, O>(G, A.em, s, E, B);
+            borisAccelerate(v, E, B, q, A.dto2m);
+            // postPushStep_ (boris.hpp:218-234)
+            if (!advancePosition<D>(cell, delta, v, A.hdtdx, bd, bv))
+            {
+                raiseError(A.err, 3u, 2u, i, bd, bv, A.kind, A.pop);
+                return;
+            }
+        }
+        // what is deposited and what the array keeps (ion_updater.hpp:169-295)
+        inDom = inBox(cell, A.box.dom_lo, A.box.dom_hi, D);
+        if (A.kind == KIND_DOMAIN)
+        {
+            bool const nl = inNLGB(cell, A.box, D);
+            deposit       = nl;
+            st            = inDom ? ST_KEEP : (nl ? ST_LEAVE : ST_DROP);
+        }
+        else
+        {
+            deposit = sel1 && inDom;
+            bool const layer = sel1 && !inDom && inBox(cell, A.box.gh_lo, A.box.gh_hi, D);
+            st = !sel1 ? ST_DROP : (inDom ? ST_EXPORT : (layer ? ST_KEEP : ST_DROP));
+        }
+    }
+
+    if (deposit)
+    {
+        DepositData<D, O> dd;
+        primalWeights<D, O>(G, cell, delta, dd);
+        bool ok = true;
+#pragma unroll
+        for (int d = 0; d < D; ++d)
+            ok = ok && dd.start[d] >= 0 && dd.start[d] + O < G.shape[QMOM][d];
+        if (!ok)
+            raiseError(A.err, 4u, 3u, i, delta[0], v[0], A.kind, A.pop);
+        else
+        {
+            dd.val[0] = 1. * w * A.coef;
+            dd.val[1] = q * w * A.coef;
+            dd.val[2] = v[0] * w * A.coef;
+            dd.val[3] = v[1] * w * A.coef;
+            dd.val[4] = v[2] * w * A.coef;
+            if constexpr (C::ACC)
+            {
+                // same cell as the window -> accumulate; else flush and restart there
+                bool same = acc.live;
+#pragma unroll
+                for (int d = 0; d < D; ++d)
+                    same = same && acc.base[d] == cell[d] - (G.lower[d] - G.g) - C::WB;
+                if (!same)
+                {
+                    accFlush<D, O>(acc, G, T, copy, A.mom);
+                    accZero(acc);
+#pragma unroll
+                    for (int d = 0; d < D; ++d)
+                        acc.base[d] = cell[d] - (G.lower[d] - G.g) - C::WB;
+                    acc.live = true;
+                }
+                accAdd<D, O>(acc, dd);
+            }
+            else
+                depositOne<D, O>(dd, G, T, copy, A.mom);
+        }
+    }
+
+    if (A.pushOn && A.mode == MODE_ALL)
+    {
+        if (A.periodicWrap && A.kind == KIND_DOMAIN)
+        {
+            if (st == ST_LEAVE)
+            {
+                // fillIonGhostParticles: the leaver re-enters through the periodic image
+#pragma unroll
+                for (int d = 0; d < D; ++d)
+                {
+                    if (cell[d] < A.box.dom_lo[d])
+                        cell[d] += G.n[d];
+                    else if (cell[d] > A.box.dom_hi[d])
+                        cell[d] -= G.n[d];
+                }
+                st = ST_KEEP;
+            }
+            else if (st == ST_DROP)
+            {
+                cell[0] = DEAD_CELL;
+                atomicAdd(A.dropped, 1u);
+            }
+        }
+#pragma unroll
+        for (int d = 0; d < D; ++d)
+        {
+            out.ic[d][i] = cell[d];
+            out.de[d][i] = delta[d];
+        }
+        out.v[0][i] = v[0];
+        out.v[1][i] = v[1];
+        out.v[2][i] = v[2];
+        if (out.w != in.w)
+        {
+            out.w[i] = w;
+            out.q[i] = q;
+        }
+        if (A.status)
+            A.status[i] = st;
+    }
+}
+
+__device__ __forceinline__ void zeroTile(double* lds, TileSpec const& S)
+{
+    for (int x = threadIdx.x; x < 5 * S.copies * S.stride; x += BLOCK)
+        lds[x] = 0.;
+}
+
+// flush the tile: the copies of each node summed, one HBM atomic per touched node and moment
+template<int D>
+__device__ __forceinline__ void flushTile(Geom const& G, TileView const& T, int nodes,
+                                          double* const* mom)
+{
+    int const e12 = T.ext[1] * T.ext[2];
+    for (int x = threadIdx.x; x < nodes; x += BLOCK)
+    {
+        int node = -1;
+#pragma unroll
+        for (int m = 0; m < 5; ++m)
+        {
+            double val = 0.;
+            for (int c = 0; c < T.copies; ++c)
+                val += T.mom(m, c)[x];
+            if (val != 0.)
+            {
+                if (node < 0)
+                {
+                    int const ti = x / e12;
+                    int const r  = x - ti * e12;
+                    int const tj = r / T.ext[2];
+                    int const tk = r - tj * T.ext[2];
+                    node = G.idx(QMOM, T.org[0] + ti, D > 1 ? T.org[1] + tj : 0,
+                                 D > 2 ? T.org[2] + tk : 0);
+                }
+                atomicAdd(mom[m] + node, val);
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ void tileView(TileView& T, double* lds, TileSpec const& S)
+{
+    T.lds    = lds;
+    T.copies = S.copies;
+    T.stride = S.stride;
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+    {
+        T.ext[d] = S.ext[d];
+        T.org[d] = 0;
+    }
+}
+
 } // namespace
 
 // ---------------------------------------------------------------------------------------------
-// fused push + deposit (and deposit-only for unpushed arrays)
+// fused push + deposit over linear storage (and deposit-only for unpushed arrays)
 // ---------------------------------------------------------------------------------------------
 template<int D, int O>
 __global__ __launch_bounds__(BLOCK) void k_push(PushArgs A, PView in, PView out)
 {
-    using C                = Cfg<D, O>;
-    constexpr int TN       = Tile<D>::NODES;
-    constexpr int PPT      = C::PPT;
-    constexpr int CHUNK    = C::CHUNK;
-    __shared__ double tile[5 * TN];
-    __shared__ int org[3];
+    using C             = Cfg<D, O>;
+    constexpr int PPT   = C::PPT;
+    constexpr int CHUNK = C::CHUNK;
+    extern __shared__ double lds[];
 
-    Geom const& G = A.G;
-    int const t   = threadIdx.x;
+    Geom const& G  = A.G;
+    int const t    = threadIdx.x;
+    int const copy = (t & 63) & (A.tile.copies - 1);
     unsigned long long const base = static_cast<unsigned long long>(blockIdx.x) * CHUNK;
+
+    zeroTile(lds, A.tile);
+    // the tile is placed around the cell of the chunk's first particle (chunks of a cell-ordered
+    // array hold neighbouring cells), with host-chosen margins for the chunk's extent, one cell
+    // of motion and the stencil; a dead first particle leaves the tile out of reach
+    TileView T;
+    tileView(T, lds, A.tile);
+    int const c0   = in.ic[0][base];
+    bool const far = c0 == DEAD_CELL;
+#pragma unroll
+    for (int d = 0; d < D; ++d)
+        T.org[d] = far ? -0x20000000 : in.ic[d][base] - (G.lower[d] - G.g) - A.tile.lo[d];
+    __syncthreads();
+
     Acc<D, O> acc;
     if constexpr (C::ACC)
         accZero(acc);
-
-    for (int x = t; x < 5 * TN; x += BLOCK)
-        tile[x] = 0.;
-    if (t == 0)
-    {
-        // the tile is anchored at the chunk's first particle (particles are kept close to
-        // cell order), offset by the host-chosen margins
-        int c0 = in.ic[0][base];
-        for (int d = 0; d < 3; ++d)
-            org[d] = d < D ? (c0 == DEAD_CELL ? -0x20000000
-                                              : in.ic[d][base] - (G.lower[d] - G.g) - A.tileLo[d])
-                           : 0;
-    }
-    __syncthreads();
-    int torg[3] = {org[0], org[1], org[2]};
-
     for (int k = 0; k < PPT; ++k)
     {
         unsigned long long const i = base + chunkOffset(t, k);
-        if (i >= A.n)
-            continue;
-        int cell[D];
-        double delta[D], v[3];
-#pragma unroll
-        for (int d = 0; d < D; ++d)
-        {
-            cell[d]  = in.ic[d][i];
-            delta[d] = in.de[d][i];
-        }
-        if (cell[0] == DEAD_CELL)
-            continue;
-        v[0] = in.v[0][i];
-        v[1] = in.v[1][i];
-        v[2] = in.v[2][i];
-        double const w = in.w[i], q = in.q[i];
-
-        bool deposit = true, sel1 = true, inDom = true;
-        uint8_t st   = ST_KEEP;
-        if (A.pushOn)
-        {
-            double bd, bv;
-            // prePushStep_ (boris.hpp:180-216)
-            if (!advancePosition<D>(cell, delta, v, A.hdtdx, bd, bv))
-            {
-                raiseError(A.err, 3u, 1u, i, bd, bv, A.kind, A.pop);
-                continue;
-            }
-            // first selector: noop (domain) or inGhostBox (level ghosts)
-            sel1 = A.kind == KIND_DOMAIN ? true : inBox(cell, A.box.gh_lo, A.box.gh_hi, D);
-            if (sel1)
-            {
-                Stencil<D, O> s;
-                makeStencil<D, O>(G, cell, delta, s);
-                if (!stencilInside<D, O>(G, s, QEX) || !stencilInside<D, O>(G, s, QBX)
-                    || !stencilInside<D, O>(G, s, QEY) || !stencilInside<D, O>(G, s, QBY)
-                    || !stencilInside<D, O>(G, s, QEZ) || !stencilInside<D, O>(G, s, QBZ))
-                {
-                    raiseError(A.err, 4u, 2u, i, delta[0], v[0], A.kind, A.pop);
-                    continue;
-                }
-                double E[3], B[3];
-                E[0] = interp<D, O>(G, A.em.E[0], QEX, s);
-                E[1] = interp<D, O>(G, A.em.E[1], QEY, s);
-                E[2] = interp<D, O>(G, A.em.E[2], QEZ, s);
-                B[0] = interp<D, O>(G, A.em.B[0], QBX, s);
-                B[1] = interp<D, O>(G, A.em.B[1], QBY, s);
-                B[2] = interp<D, O>(G, A.em.B[2], QBZ, s);
-                borisAccelerate(v, E, B, q, A.dto2m);
-                // postPushStep_ (boris.hpp:218-234)
-                if (!advancePosition<D>(cell, delta, v, A.hdtdx, bd, bv))
-                {
-                    raiseError(A.err, 3u, 2u, i, bd, bv, A.kind, A.pop);
-                    continue;
-                }
-            }
-            // what is deposited and what the array keeps (ion_updater.hpp:169-295)
-            inDom = inBox(cell, A.box.dom_lo, A.box.dom_hi, D);
-            if (A.kind == KIND_DOMAIN)
-            {
-                bool const nl = inNLGB(cell, A.box, D);
-                deposit       = nl;
-                st            = inDom ? ST_KEEP : (nl ? ST_LEAVE : ST_DROP);
-            }
-            else
-            {
-                deposit = sel1 && inDom;
-                bool const layer = sel1 && !inDom && inBox(cell, A.box.gh_lo, A.box.gh_hi, D);
-                st = !sel1 ? ST_DROP : (inDom ? ST_EXPORT : (layer ? ST_KEEP : ST_DROP));
-            }
-        }
-
-        if (deposit)
-        {
-            DepositData<D, O> dd;
-            primalWeights<D, O>(G, cell, delta, dd);
-            bool ok = true;
-#pragma unroll
-            for (int d = 0; d < D; ++d)
-                ok = ok && dd.start[d] >= 0 && dd.start[d] + O < G.shape[QMOM][d];
-            if (!ok)
-                raiseError(A.err, 4u, 3u, i, delta[0], v[0], A.kind, A.pop);
-            else
-            {
-                dd.val[0] = 1. * w * A.coef;
-                dd.val[1] = q * w * A.coef;
-                dd.val[2] = v[0] * w * A.coef;
-                dd.val[3] = v[1] * w * A.coef;
-                dd.val[4] = v[2] * w * A.coef;
-                if constexpr (C::ACC)
-                {
-                    // same cell as the window -> accumulate; else flush and restart there
-                    bool same = acc.live;
-#pragma unroll
-                    for (int d = 0; d < D; ++d)
-                        same = same && acc.base[d] == cell[d] - (G.lower[d] - G.g) - C::WB;
-                    if (!same)
-                    {
-                        accFlush<D, O>(acc, G, tile, torg, A.tileExt, A.mom);
-                        accZero(acc);
-#pragma unroll
-                        for (int d = 0; d < D; ++d)
-                            acc.base[d] = cell[d] - (G.lower[d] - G.g) - C::WB;
-                        acc.live = true;
-                    }
-                    accAdd<D, O>(acc, dd);
-                }
-                else
-                    depositOne<D, O>(dd, G, tile, torg, A.tileExt, A.mom);
-            }
-        }
-
-        if (A.pushOn && A.mode == MODE_ALL)
-        {
-            if (A.periodicWrap && A.kind == KIND_DOMAIN)
-            {
-                if (st == ST_LEAVE)
-                {
-                    // fillIonGhostParticles: the leaver re-enters through the periodic image
-#pragma unroll
-                    for (int d = 0; d < D; ++d)
-                    {
-                        if (cell[d] < A.box.dom_lo[d])
-                            cell[d] += G.n[d];
-                        else if (cell[d] > A.box.dom_hi[d])
-                            cell[d] -= G.n[d];
-                    }
-                    st = ST_KEEP;
-                }
-                else if (st == ST_DROP)
-                {
-                    cell[0] = DEAD_CELL;
-                    atomicAdd(A.dropped, 1u);
-                }
-            }
-#pragma unroll
-            for (int d = 0; d < D; ++d)
-            {
-                out.ic[d][i] = cell[d];
-                out.de[d][i] = delta[d];
-            }
-            out.v[0][i] = v[0];
-            out.v[1][i] = v[1];
-            out.v[2][i] = v[2];
-            if (out.w != in.w)
-            {
-                out.w[i] = w;
-                out.q[i] = q;
-            }
-            if (A.status)
-                A.status[i] = st;
-        }
+        if (i < A.n)
+            pushParticle<D, O>(A, in, out, i, T, copy, acc);
     }
     if constexpr (C::ACC)
-        accFlush<D, O>(acc, G, tile, torg, A.tileExt, A.mom);
+        accFlush<D, O>(acc, G, T, copy, A.mom);
+    __syncthreads();
+    flushTile<D>(G, T, A.tile.nodes, A.mom);
+}
+
+// ---------------------------------------------------------------------------------------------
+// fused push + deposit over cell-segmented storage: one workgroup per work item (a segment and
+// SEG_ROWS of its rows); wavefront w takes rows first + w, first + w + NW, ...
+// ---------------------------------------------------------------------------------------------
+#ifndef PHX_SEG_WAVES
+#define PHX_SEG_WAVES 4 // waves per SIMD the register allocation of the segmented push targets
+#endif
+template<int D, int O>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(PHX_SEG_WAVES)))
+void k_push_seg(PushArgs A, PView in, PView out)
+{
+    using C = Cfg<D, O>;
+    extern __shared__ double lds[];
+    unsigned int const item = blockIdx.x;
+    if (item >= *A.st.nItems)
+        return;
+    Geom const& G          = A.G;
+    SegGeom const& S       = A.sg;
+    unsigned int const seg = A.st.items[2 * item];
+    unsigned int const r0  = A.st.items[2 * item + 1];
+    unsigned int const R   = A.st.rows[seg];
+    unsigned long long const off = A.st.off[seg];
+    int const t    = threadIdx.x;
+    int const lane = t & 63;
+    int const copy = lane & (A.tile.copies - 1);
+
+    zeroTile(lds, A.tile);
+    // the tile covers the segment's cells with host-chosen margins (stencil, one cell of
+    // motion, drift since the sort); the overflow segment has no cell range
+    TileView T;
+    tileView(T, lds, A.tile);
+    if (seg < static_cast<unsigned int>(S.nsegs))
+    {
+        int rem = static_cast<int>(seg);
+        int sidx[3];
+#pragma unroll
+        for (int d = D - 1; d >= 0; --d)
+        {
+            sidx[d] = rem % S.nseg[d];
+            rem /= S.nseg[d];
+        }
+#pragma unroll
+        for (int d = 0; d < D; ++d)
+            T.org[d] = S.lo[d] + sidx[d] * S.seg[d] - (G.lower[d] - G.g) - A.tile.lo[d];
+    }
+    else
+    {
+#pragma unroll
+        for (int d = 0; d < D; ++d)
+            T.org[d] = -0x20000000;
+    }
     __syncthreads();
 
-    // flush the tile: one HBM atomic per touched node and moment
-    int const vol = A.tileExt[0] * A.tileExt[1] * A.tileExt[2];
-    for (int x = t; x < vol; x += BLOCK)
-    {
-        int const e12 = A.tileExt[1] * A.tileExt[2];
-        int const ti  = x / e12;
-        int const r   = x - ti * e12;
-        int const tj  = r / A.tileExt[2];
-        int const tk  = r - tj * A.tileExt[2];
-        int node      = -1;
-#pragma unroll
-        for (int m = 0; m < 5; ++m)
-        {
-            double const val = tile[m * TN + x];
-            if (val != 0.)
-            {
-                if (node < 0)
-                    node = G.idx(QMOM, torg[0] + ti, D > 1 ? torg[1] + tj : 0, D > 2 ? torg[2] + tk : 0);
-                atomicAdd(A.mom[m] + node, val);
-            }
-        }
-    }
+    Acc<D, O> acc;
+    if constexpr (C::ACC)
+        accZero(acc);
+    unsigned int const rEnd = min(r0 + static_cast<unsigned int>(SEG_ROWS), R);
+    for (unsigned int r = r0 + (t >> 6); r < rEnd; r += NW)
+        pushParticle<D, O>(A, in, out, off + static_cast<unsigned long long>(r) * LANES + lane, T,
+                           copy, acc);
+    if constexpr (C::ACC)
+        accFlush<D, O>(acc, G, T, copy, A.mom);
+    __syncthreads();
+    flushTile<D>(G, T, A.tile.nodes, A.mom);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -576,6 +726,338 @@ __global__ __launch_bounds__(BLOCK) void k_scatter_status(PView src, PView dst,
     dst.w[j]    = src.w[i];
     dst.q[j]    = src.q[i];
 }
+// ---------------------------------------------------------------------------------------------
+// counting sort of a domain array into cell-segmented storage (dead particles are dropped)
+// ---------------------------------------------------------------------------------------------
+namespace
+{
+constexpr unsigned int DEAD_KEY = 0xffffffffu;
+
+// key of a particle: its row-major cell index in the domain box, ncells outside the box, DEAD_KEY
+// for a lost particle; c receives the cell relative to the box
+__device__ __forceinline__ unsigned int cellKey(SegGeom const& S, PView const& v,
+                                                unsigned long long i, int* c)
+{
+    if (v.ic[0][i] == DEAD_CELL)
+        return DEAD_KEY;
+    unsigned int key = 0;
+    for (int d = 0; d < S.dim; ++d)
+    {
+        c[d] = v.ic[d][i] - S.lo[d];
+        if (c[d] < 0 || c[d] >= S.n[d])
+            return static_cast<unsigned int>(S.ncells);
+        key = key * static_cast<unsigned int>(S.n[d]) + static_cast<unsigned int>(c[d]);
+    }
+    return key;
+}
+
+// segment and lane of a cell of the box
+__device__ __forceinline__ void segLane(SegGeom const& S, int const* c, unsigned int& seg, int& lane)
+{
+    unsigned int s = 0;
+    int l          = 0;
+    for (int d = 0; d < S.dim; ++d)
+    {
+        int const q = c[d] / S.seg[d];
+        s           = s * static_cast<unsigned int>(S.nseg[d]) + static_cast<unsigned int>(q);
+        l           = l * S.seg[d] + (c[d] - q * S.seg[d]);
+    }
+    seg  = s;
+    lane = l;
+}
+
+// key of lane l's cell in segment s, or DEAD_KEY where the segment overhangs the box
+__device__ __forceinline__ unsigned int laneKey(SegGeom const& S, unsigned int s, int l)
+{
+    int sidx[3], sub[3];
+    unsigned int rem = s;
+    int lr           = l;
+    for (int d = S.dim - 1; d >= 0; --d)
+    {
+        sidx[d] = static_cast<int>(rem % static_cast<unsigned int>(S.nseg[d]));
+        rem /= static_cast<unsigned int>(S.nseg[d]);
+        sub[d] = lr % S.seg[d];
+        lr /= S.seg[d];
+    }
+    if (lr != 0)
+        return DEAD_KEY; // lane beyond the segment's cells
+    unsigned int key = 0;
+    for (int d = 0; d < S.dim; ++d)
+    {
+        int const c = sidx[d] * S.seg[d] + sub[d];
+        if (c >= S.n[d])
+            return DEAD_KEY;
+        key = key * static_cast<unsigned int>(S.n[d]) + static_cast<unsigned int>(c);
+    }
+    return key;
+}
+} // namespace
+
+__global__ __launch_bounds__(BLOCK) void k_seg_zero(unsigned int* __restrict__ p, unsigned int m)
+{
+    unsigned int const i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i < m)
+        p[i] = 0u;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_seg_count(SegGeom S, PView v, unsigned long long n,
+                                                     unsigned int* __restrict__ counts)
+{
+    unsigned long long const i = static_cast<unsigned long long>(blockIdx.x) * BLOCK + threadIdx.x;
+    if (i >= n)
+        return;
+    int c[3];
+    unsigned int const key = cellKey(S, v, i, c);
+    if (key != DEAD_KEY)
+        atomicAdd(counts + key, 1u);
+}
+
+// rows of each segment (the most particles one of its cells holds; the overflow segment packs
+// its particles 64 to a row) and its work items
+__global__ __launch_bounds__(BLOCK) void k_seg_rows(SegGeom S, unsigned int const* __restrict__ counts,
+                                                    unsigned int* __restrict__ rows,
+                                                    unsigned int* __restrict__ itemCount)
+{
+    unsigned int const s = blockIdx.x * BLOCK + threadIdx.x;
+    if (s > static_cast<unsigned int>(S.nsegs))
+        return;
+    unsigned int R = 0;
+    if (s < static_cast<unsigned int>(S.nsegs))
+    {
+        for (int l = 0; l < LANES; ++l)
+        {
+            unsigned int const key = laneKey(S, s, l);
+            if (key != DEAD_KEY)
+                R = max(R, counts[key]);
+        }
+    }
+    else
+        R = (counts[S.ncells] + LANES - 1) / LANES;
+    rows[s]      = R;
+    itemCount[s] = (R + SEG_ROWS - 1) / SEG_ROWS;
+}
+
+// exclusive scans over the m = nsegs + 1 segments in one workgroup: first slots (64 per row;
+// off[m] = all slots) and first work items (in place; *nItems = all items)
+__global__ __launch_bounds__(1024) void k_seg_scan(unsigned int const* __restrict__ rows,
+                                                   unsigned long long* __restrict__ off,
+                                                   unsigned int* __restrict__ itemCount,
+                                                   unsigned int* __restrict__ nItems, unsigned int m)
+{
+    __shared__ unsigned long long ps[1024];
+    __shared__ unsigned int pi[1024];
+    unsigned int const per = (m + 1023) / 1024;
+    unsigned int const b0  = threadIdx.x * per;
+    unsigned int const b1  = min(b0 + per, m);
+    unsigned long long s   = 0;
+    unsigned int it        = 0;
+    for (unsigned int b = b0; b < b1; ++b)
+    {
+        s += static_cast<unsigned long long>(rows[b]) * LANES;
+        it += itemCount[b];
+    }
+    ps[threadIdx.x] = s;
+    pi[threadIdx.x] = it;
+    __syncthreads();
+    for (unsigned int o = 1; o < 1024; o <<= 1)
+    {
+        unsigned long long const as = threadIdx.x >= o ? ps[threadIdx.x - o] : 0ull;
+        unsigned int const ai       = threadIdx.x >= o ? pi[threadIdx.x - o] : 0u;
+        __syncthreads();
+        ps[threadIdx.x] += as;
+        pi[threadIdx.x] += ai;
+        __syncthreads();
+    }
+    unsigned long long rs = threadIdx.x ? ps[threadIdx.x - 1] : 0ull;
+    unsigned int ri       = threadIdx.x ? pi[threadIdx.x - 1] : 0u;
+    for (unsigned int b = b0; b < b1; ++b)
+    {
+        unsigned long long const a = static_cast<unsigned long long>(rows[b]) * LANES;
+        unsigned int const c       = itemCount[b];
+        off[b]                     = rs;
+        itemCount[b]               = ri;
+        rs += a;
+        ri += c;
+    }
+    if (threadIdx.x == 1023)
+    {
+        off[m]  = ps[1023];
+        *nItems = pi[1023];
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_seg_items(unsigned int const* __restrict__ rows,
+                                                     unsigned int const* __restrict__ itemOff,
+                                                     unsigned int* __restrict__ items, unsigned int m)
+{
+    unsigned int const s = blockIdx.x * BLOCK + threadIdx.x;
+    if (s >= m)
+        return;
+    unsigned int const k = (rows[s] + SEG_ROWS - 1) / SEG_ROWS;
+    for (unsigned int j = 0; j < k; ++j)
+    {
+        items[2 * (itemOff[s] + j)]     = s;
+        items[2 * (itemOff[s] + j) + 1] = j * SEG_ROWS;
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_seg_scatter(SegGeom S, PView src, PView dst,
+                                                       unsigned long long n,
+                                                       unsigned int* __restrict__ cursor,
+                                                       unsigned long long const* __restrict__ off)
+{
+    unsigned long long const i = static_cast<unsigned long long>(blockIdx.x) * BLOCK + threadIdx.x;
+    if (i >= n)
+        return;
+    int c[3];
+    unsigned int const key = cellKey(S, src, i, c);
+    if (key == DEAD_KEY)
+        return;
+    unsigned long long j;
+    if (key < static_cast<unsigned int>(S.ncells))
+    {
+        unsigned int seg;
+        int lane;
+        segLane(S, c, seg, lane);
+        unsigned int const r = atomicAdd(cursor + key, 1u);
+        j = off[seg] + static_cast<unsigned long long>(r) * LANES + lane;
+    }
+    else
+        j = off[S.nsegs] + atomicAdd(cursor + key, 1u);
+    for (int d = 0; d < S.dim; ++d)
+    {
+        dst.ic[d][j] = src.ic[d][i];
+        dst.de[d][j] = src.de[d][i];
+    }
+    dst.v[0][j] = src.v[0][i];
+    dst.v[1][j] = src.v[1][i];
+    dst.v[2][j] = src.v[2][i];
+    dst.w[j]    = src.w[i];
+    dst.q[j]    = src.q[i];
+}
+
+// marks the empty slots of every segment dead (one workgroup per segment)
+__global__ __launch_bounds__(BLOCK) void k_seg_pad(SegGeom S, PView dst,
+                                                   unsigned int const* __restrict__ rows,
+                                                   unsigned long long const* __restrict__ off,
+                                                   unsigned int const* __restrict__ counts)
+{
+    unsigned int const s = blockIdx.x;
+    unsigned int const slots = rows[s] * LANES;
+    for (unsigned int x = threadIdx.x; x < slots; x += BLOCK)
+    {
+        bool real;
+        if (s < static_cast<unsigned int>(S.nsegs))
+        {
+            unsigned int const key = laneKey(S, s, static_cast<int>(x % LANES));
+            real = key != DEAD_KEY && x / LANES < counts[key];
+        }
+        else
+            real = x < counts[S.ncells];
+        if (!real)
+            dst.ic[0][off[s] + x] = DEAD_CELL;
+    }
+}
+
+void launchSegCount(SegGeom const& sg, PView const& src, unsigned long long n, unsigned int* work,
+                    SegTables const& tables, hipStream_t s)
+{
+    unsigned int const nc = static_cast<unsigned int>(sg.ncells) + 1;
+    unsigned int const m  = static_cast<unsigned int>(sg.nsegs) + 1;
+    unsigned int* counts    = work;
+    unsigned int* itemCount = work + nc;
+    k_seg_zero<<<(nc + BLOCK - 1) / BLOCK, BLOCK, 0, s>>>(counts, nc);
+    if (n)
+        k_seg_count<<<static_cast<unsigned int>((n + BLOCK - 1) / BLOCK), BLOCK, 0, s>>>(sg, src, n,
+                                                                                       counts);
+    k_seg_rows<<<(m + BLOCK - 1) / BLOCK, BLOCK, 0, s>>>(sg, counts, tables.rows, itemCount);
+    k_seg_scan<<<1, 1024, 0, s>>>(tables.rows, tables.off, itemCount, tables.nItems, m);
+}
+
+void launchSegScatter(SegGeom const& sg, PView const& src, PView const& dst, unsigned long long n,
+                      unsigned int* work, SegTables const& tables, int, hipStream_t s)
+{
+    unsigned int const nc = static_cast<unsigned int>(sg.ncells) + 1;
+    unsigned int const m  = static_cast<unsigned int>(sg.nsegs) + 1;
+    unsigned int* counts    = work;
+    unsigned int* itemCount = work + nc;
+    k_seg_items<<<(m + BLOCK - 1) / BLOCK, BLOCK, 0, s>>>(tables.rows, itemCount, tables.items, m);
+    k_seg_zero<<<(nc + BLOCK - 1) / BLOCK, BLOCK, 0, s>>>(counts, nc);
+    if (n)
+        k_seg_scatter<<<static_cast<unsigned int>((n + BLOCK - 1) / BLOCK), BLOCK, 0, s>>>(
+            sg, src, dst, n, counts, tables.off);
+    k_seg_pad<<<m, BLOCK, 0, s>>>(sg, dst, tables.rows, tables.off, counts);
+}
+
+SegGeom segGeom(int dim, int const* lo, int const* hi)
+{
+    // preferred segment shapes: a run of 64 cells along the fastest dimension in 1D/2D; a
+    // 2x4x8 block in 3D, whose LDS tile with its margins stays small. Clipped to the box, the
+    // cells left over go to the slower dimensions.
+    static int const pref[3][3] = {{64, 1, 1}, {1, 64, 1}, {2, 4, 8}};
+    SegGeom S{};
+    S.dim    = dim;
+    S.nsegs  = 1;
+    S.ncells = 1;
+    int room = LANES;
+    for (int d = 2; d >= 0; --d)
+    {
+        S.lo[d] = d < dim ? lo[d] : 0;
+        S.n[d]  = d < dim ? hi[d] - lo[d] + 1 : 1;
+    }
+    for (int d = dim - 1; d >= 0; --d)
+    {
+        int const want = d == 0 ? room : std::min(pref[dim - 1][d], room);
+        S.seg[d]       = std::max(1, std::min(want, S.n[d]));
+        room /= S.seg[d];
+    }
+    for (int d = dim; d < 3; ++d)
+        S.seg[d] = 1;
+    for (int d = 0; d < 3; ++d)
+    {
+        S.nseg[d] = (S.n[d] + S.seg[d] - 1) / S.seg[d];
+        S.nsegs *= S.nseg[d];
+        S.ncells *= S.n[d];
+    }
+    return S;
+}
+
+TileSpec segTileShape(int dim, int order, SegGeom const& sg, int copies)
+{
+    TileSpec T{};
+    T.copies = copies;
+    // node range of a particle's deposit around its cell: [-1, 2] (orders 2, 3), [0, 1]
+    // (order 1); one cell of motion and TILE_SPREAD cells of drift since the sort on each side
+    int const mlo = (order >= 2 ? 1 : 0) + 1 + TILE_SPREAD;
+    int const mhi = (order == 1 ? 1 : 2) + 1 + TILE_SPREAD;
+    for (int d = 0; d < 3; ++d)
+    {
+        T.lo[d]  = d < dim ? mlo : 0;
+        T.ext[d] = d < dim ? sg.seg[d] + mlo + mhi : 1;
+    }
+    T.nodes  = T.ext[0] * T.ext[1] * T.ext[2];
+    T.stride = (T.nodes + 15) / 16 * 16; // copies 16 words apart: neighbouring lanes' banks differ
+    T.bytes  = static_cast<unsigned int>(8 * 5 * copies * T.stride);
+    if (T.bytes > TILE_BYTES_MAX)
+    {
+        // no room: every deposit goes straight to HBM
+        for (int d = 0; d < 3; ++d)
+            T.ext[d] = 1;
+        T.lo[0]  = -0x10000000;
+        T.nodes  = 1;
+        T.stride = 16;
+        T.bytes  = static_cast<unsigned int>(8 * 5 * copies * T.stride);
+    }
+    return T;
+}
+
+int segRowsPerItem() { return SEG_ROWS; }
+
+unsigned int segItemsCap(SegGeom const& sg, unsigned long long capacity)
+{
+    return static_cast<unsigned int>(capacity / (LANES * SEG_ROWS)) + static_cast<unsigned int>(sg.nsegs)
+           + 2u;
+}
 
 // ---------------------------------------------------------------------------------------------
 // launchers
@@ -583,17 +1065,19 @@ __global__ __launch_bounds__(BLOCK) void k_scatter_status(PView src, PView dst,
 template<int D, int O>
 static void launchPushT(PushArgs const& A, PView const& in, PView const& out, hipStream_t s)
 {
+    if (A.segOn)
+    {
+        k_push_seg<D, O><<<A.st.itemsCap, BLOCK, A.tile.bytes, s>>>(A, in, out);
+        return;
+    }
     if (A.n == 0)
         return;
-    PushArgs B = A;
-    B.pushOn   = 1;
     constexpr int CH = Cfg<D, O>::CHUNK;
     unsigned int const grid = static_cast<unsigned int>((A.n + CH - 1) / CH);
-    k_push<D, O><<<grid, BLOCK, 0, s>>>(B, in, out);
+    k_push<D, O><<<grid, BLOCK, A.tile.bytes, s>>>(A, in, out);
 }
 
-// LDS tile extents for a level: the chunk of CHUNK consecutive particles spans about
-// CHUNK/ppc cells along the fastest dimension; margins cover the stencil and one cell of motion
+// particles per workgroup of the linear push kernel
 template<int D, int O>
 static int chunkOf()
 {
@@ -619,36 +1103,54 @@ int chunkSize(int dim, int order)
 
 int blockThreads() { return BLOCK; }
 
-void tileShape(int dim, int order, double ppc, int* lo, int* ext)
+TileSpec tileShape(int dim, int order, double ppc, int copies)
 {
-    int const cap   = dim == 3 ? Tile<3>::NODES : Tile<2>::NODES;
+    TileSpec T{};
+    T.copies        = copies;
+    int const slabs = 5 * copies;
+    // nodes that fit the LDS budget with the bank padding of every slab
+    int const cap   = static_cast<int>(TILE_BYTES_MAX / (8u * static_cast<unsigned>(slabs))) - 17;
     int const CHUNK = chunkSize(dim, order);
+    // margins around the first particle's cell: the stencil (from cell - 1 for order >= 2) and
+    // one cell of motion
     int const mlo = 2, mhi = order + 2;
+    // a chunk of a cell-ordered array spans one cell across the slower dimensions and about
+    // CHUNK/ppc cells along the fastest
     for (int d = 0; d < 3; ++d)
     {
-        lo[d]  = d < dim ? mlo : 0;
-        ext[d] = d < dim ? 1 + mlo + mhi : 1;
+        T.lo[d]  = d < dim ? mlo : 0;
+        T.ext[d] = d < dim ? 1 + mlo + mhi : 1;
     }
-    int fast     = dim - 1;
-    double cells = ppc > 0 ? CHUNK / ppc : CHUNK;
-    int want     = static_cast<int>(cells) + 2 + mlo + mhi;
-    int others   = 1;
+    int const fast = dim - 1;
+    double cells   = ppc > 0 ? CHUNK / ppc : CHUNK;
+    int want       = static_cast<int>(cells) + 1 + mlo + mhi;
+    int others     = 1;
     for (int d = 0; d < dim - 1; ++d)
-        others *= ext[d];
-    int maxFast = cap / others;
-    ext[fast]   = want < maxFast ? want : maxFast;
+        others *= T.ext[d];
+    int const maxFast = cap / others;
+    T.ext[fast]       = want < maxFast ? want : maxFast;
+    T.nodes           = T.ext[0] * T.ext[1] * T.ext[2];
+    // copies one word past a multiple of 16: lanes of one cell adding into different copies of
+    // a node hit different banks
+    T.stride          = (T.nodes + 15) / 16 * 16 + 1;
+    T.bytes           = static_cast<unsigned int>(8 * slabs * T.stride);
+    return T;
 }
 
 template<int D, int O>
 static void launchDepositT(PushArgs const& A, PView const& in, hipStream_t s)
 {
-    if (A.n == 0)
-        return;
     PushArgs B = A;
     B.pushOn   = 0;
-    constexpr int CH = Cfg<D, O>::CHUNK;
-    unsigned int const grid = static_cast<unsigned int>((A.n + CH - 1) / CH);
-    k_push<D, O><<<grid, BLOCK, 0, s>>>(B, in, in);
+    launchPushT<D, O>(B, in, in, s);
+}
+
+template<int D, int O>
+static void launchPushOnT(PushArgs const& A, PView const& in, PView const& out, hipStream_t s)
+{
+    PushArgs B = A;
+    B.pushOn   = 1;
+    launchPushT<D, O>(B, in, out, s);
 }
 
 #define PHX_DISPATCH_DO(FN, ...)                                                                   \
@@ -667,7 +1169,7 @@ static void launchDepositT(PushArgs const& A, PView const& in, hipStream_t s)
 
 void launchPush(PushArgs const& A, PView const& in, PView const& out, hipStream_t s)
 {
-    PHX_DISPATCH_DO(launchPushT, A, in, out, s)
+    PHX_DISPATCH_DO(launchPushOnT, A, in, out, s)
 }
 
 void launchDeposit(PushArgs const& A, PView const& in, hipStream_t s)

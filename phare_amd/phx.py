@@ -76,8 +76,9 @@ def lib():
         L.phx_update_populations.argtypes = [vp, C.c_int, C.c_double, C.c_int]
         for n in ("phx_update_ions", "phx_fill_moment_borders", "phx_fill_ion_borders",
                   "phx_fill_ion_ghost_particles", "phx_electrons_update", "phx_average",
-                  "phx_initialize"):
+                  "phx_initialize", "phx_sort_particles"):
             getattr(L, n).argtypes = [vp]
+        L.phx_set_sort_interval.argtypes = [vp, C.c_int]
         L.phx_fill_ghosts.argtypes = [vp, C.c_int]
         L.phx_faraday.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_double]
         L.phx_ampere.argtypes = [vp, C.c_int]
@@ -97,7 +98,7 @@ EXPORTED = ["phx_create", "phx_destroy", "phx_last_error", "phx_synchronize", "p
             "phx_fill_moment_borders", "phx_fill_ion_borders", "phx_fill_ion_ghost_particles",
             "phx_fill_ghosts", "phx_faraday", "phx_ampere", "phx_electrons_update", "phx_ohm",
             "phx_average", "phx_move_ions", "phx_initialize", "phx_advance", "phx_diagnostics",
-            "phx_stream", "phx_profile_push"]
+            "phx_stream", "phx_profile_push", "phx_sort_particles", "phx_set_sort_interval"]
 
 
 class Level:
@@ -239,6 +240,12 @@ class Level:
 
     def advance(self, dt, nsteps=1):
         self._rc(lib().phx_advance(self.h, dt, nsteps))
+
+    def sort_particles(self):
+        self._rc(lib().phx_sort_particles(self.h))
+
+    def set_sort_interval(self, steps):
+        self._rc(lib().phx_set_sort_interval(self.h, steps))
 
     def profile_push(self, dt, mode, reps):
         ms = C.c_double(0.0)

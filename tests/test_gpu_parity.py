@@ -302,3 +302,58 @@ def test_move_ions_general_path_matches_fast_path():
         U.assert_close(a.get(fid), b.get(fid), what=f"total {fid}")
     a.close()
     b.close()
+
+
+def _segment_of(icell, ncells):
+    """segment index of each particle's cell: the domain box is cut into segments of at most 64
+    cells (a run of 64 cells along the fastest dimension in 1D/2D, a 2x4x8 block in 3D, clipped
+    to the box with the cells left over given to the slower dimensions), numbered row-major"""
+    dim = len(ncells)
+    pref = [[64], [1, 64], [2, 4, 8]][dim - 1]
+    seg, room = [1] * dim, 64
+    for d in range(dim - 1, -1, -1):
+        want = room if d == 0 else min(pref[d], room)
+        seg[d] = max(1, min(want, ncells[d]))
+        room //= seg[d]
+    nseg = [(ncells[d] + seg[d] - 1) // seg[d] for d in range(dim)]
+    s = np.zeros(len(icell), dtype=np.int64)
+    for d in range(dim):
+        s = s * nseg[d] + icell[:, d] // seg[d]
+    return s
+
+
+@pytest.mark.parametrize("dim", [1, 2, 3])
+def test_sort_groups_particles_by_segment(dim):
+    """phx_sort_particles keeps the particle set bit for bit and stores the particles segment
+    by segment, in segment order"""
+    _say(f"[gpu] sort dim={dim}")
+    rng = np.random.default_rng(11 + dim)
+    ncells = [[300], [40, 100], [12, 10, 9]][dim - 1]
+    g = phx.Level(dim, 2, ncells, [0.3] * dim)
+    p = U.random_particles(rng, dim, 7000, ncells, [0] * dim, 0)
+    g.set_particles(0, phx.DOMAIN, p)
+    g.sort_particles()
+    assert g.count() == 7000
+    q = g.get_particles()
+    U.assert_same_particles(q, p, exact=True)
+    assert np.all(np.diff(_segment_of(q["icell"], ncells)) >= 0)
+    g.close()
+
+
+@pytest.mark.parametrize("interval", [1, 3])
+def test_advance_with_frequent_sorts(interval):
+    """sorting every `interval` PPC steps (the captured step alternates between two storages)
+    leaves the run equal to the oracle's"""
+    _say(f"[gpu] advance harris2d sorted every {interval} steps")
+    wl = W.harris2d(ncells=(32, 24), ppc=12, order=2)
+    g = phx.Level(wl.dim, wl.order, wl.ncells, wl.dx, masses=wl.masses, Te=wl.Te,
+                  capacity=wl.n_particles)
+    o = O.Sim(wl.dim, wl.order, wl.ncells, wl.dx, masses=wl.masses, Te=wl.Te)
+    g.set_sort_interval(interval)
+    wl.setup(g, phx.F_B)
+    wl.setup(o, O.F_B)
+    g.advance(wl.dt, 12)
+    o.advance(wl.dt, 12)
+    g.sync()
+    _check_state(g, o, wl)
+    g.close()
