@@ -840,6 +840,21 @@ phx_status phx_create(phx_config const* cfg, phx_patch const* patch, size_t part
     for (int q = 0; q < NQ; ++q)
         for (int d = 0; d < 3; ++d)
             G.shape[q][d] = d < D ? G.n[d] + 1 + 2 * G.g - centering(q, d) : 1;
+    // local cells whose every E/B stencil fits its array: start = cell - shift >= 0 and
+    // start + order < shape for all shifts the centering allows
+    for (int d = 0; d < 3; ++d)
+    {
+        G.safeLo[d] = 0;
+        G.safeHi[d] = 0x3fffffff;
+        for (int q = QBX; q <= QEZ && d < D; ++q)
+        {
+            int const cq = centering(q, d);
+            int const mx = O == 1 ? maxShift<1>(cq) : (O == 2 ? maxShift<2>(cq) : maxShift<3>(cq));
+            int const mn = O == 1 ? minShift<1>(cq) : (O == 2 ? minShift<2>(cq) : minShift<3>(cq));
+            G.safeLo[d]  = std::max(G.safeLo[d], mx);
+            G.safeHi[d]  = std::min(G.safeHi[d], G.shape[q][d] - 1 - O + mn);
+        }
+    }
 
     Boxes& B = c->boxes;
     for (int d = 0; d < 3; ++d)

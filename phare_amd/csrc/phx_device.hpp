@@ -62,6 +62,8 @@ struct Geom
     double dx[3];
     double inv[3]; // 1./dx
     int shape[NQ][3];
+    // local cells from which every E/B gather stencil lies inside its array, whatever delta
+    int safeLo[3], safeHi[3];
 
     __host__ __device__ int size(int q) const { return shape[q][0] * shape[q][1] * shape[q][2]; }
     __host__ __device__ int idx(int q, int i, int j, int k) const
@@ -157,6 +159,18 @@ __device__ __forceinline__ void makeStencil(Geom const& G, int const* cell, doub
         s.sd[d]  = indexAndWeights<O>(CD, icl, delta[d], s.wd[d]);
         s.sp[d]  = indexAndWeights<O>(CP, icl, delta[d], s.wp[d]);
     }
+}
+
+// range of start shifts of a centering over delta in [0,1) (the shift table of startShift)
+template<int O>
+__host__ __device__ constexpr int minShift(int c)
+{
+    return O == 1 ? 0 : (O == 2 ? (c == CP ? 0 : 1) : 1);
+}
+template<int O>
+__host__ __device__ constexpr int maxShift(int c)
+{
+    return O == 1 ? (c == CP ? 0 : 1) : (O == 2 ? 1 : (c == CP ? 1 : 2));
 }
 
 // true when every node the stencil touches for quantity q is inside its array

@@ -13,9 +13,22 @@ namespace phx
 
 namespace
 {
-constexpr int FB = 256;
+constexpr int FB = 128;
 
-__device__ __forceinline__ bool nodeOf(Geom const& G, int q, long t, int& i, int& j, int& k)
+// the patch geometry with its dimension known at compile time: the helpers below take either
+// and fold every per-direction loop and projection when it is a GeomD
+template<int DIM>
+struct GeomD : Geom
+{
+    static constexpr int dim = DIM;
+    __host__ __device__ explicit GeomD(Geom const& g)
+        : Geom(g)
+    {
+    }
+};
+
+template<class GT>
+__device__ __forceinline__ bool nodeOf(GT const& G, int q, long t, int& i, int& j, int& k)
 {
     int const s1 = G.shape[q][1], s2 = G.shape[q][2];
     long const plane = static_cast<long>(s1) * s2;
@@ -29,7 +42,8 @@ __device__ __forceinline__ bool nodeOf(Geom const& G, int q, long t, int& i, int
 __device__ __forceinline__ int coord(int d, int i, int j, int k) { return d == 0 ? i : (d == 1 ? j : k); }
 
 // inside [lo, hi] of quantity q in the directions < dim
-__device__ __forceinline__ bool inRange(Geom const& G, int i, int j, int k, int const* lo, int const* hi)
+template<class GT>
+__device__ __forceinline__ bool inRange(GT const& G, int i, int j, int k, int const* lo, int const* hi)
 {
     for (int d = 0; d < G.dim; ++d)
     {
@@ -41,7 +55,8 @@ __device__ __forceinline__ bool inRange(Geom const& G, int i, int j, int k, int 
 }
 
 // GridLayout::deriv (gridlayout.hpp:571-632): operand centering decides next/prev
-__device__ __forceinline__ double deriv(Geom const& G, double const* f, int q, int dir, int i, int j,
+template<class GT>
+__device__ __forceinline__ double deriv(GT const& G, double const* f, int q, int dir, int i, int j,
                                         int k)
 {
     int const c = centering(q, dir);
@@ -67,7 +82,8 @@ __device__ __forceinline__ double deriv(Geom const& G, double const* f, int q, i
 }
 
 // GridLayout::laplacian (gridlayout.hpp:638-700)
-__device__ __forceinline__ double laplacian(Geom const& G, double const* f, int q, int i, int j, int k)
+template<class GT>
+__device__ __forceinline__ double laplacian(GT const& G, double const* f, int q, int i, int j, int k)
 {
     double res    = 0.;
     double const h = f[G.idx(q, i, j, k)];
@@ -90,7 +106,8 @@ enum : int
     P2D = 1,
     D2P = 2
 };
-__device__ __forceinline__ double project(Geom const& G, double const* f, int q, int i, int j, int k,
+template<class GT>
+__device__ __forceinline__ double project(GT const& G, double const* f, int q, int i, int j, int k,
                                           int mx, int my, int mz)
 {
     int m[3] = {mx, my, mz};
@@ -125,7 +142,8 @@ __device__ __forceinline__ double project(Geom const& G, double const* f, int q,
     return r;
 }
 
-__device__ __forceinline__ void physBox(Geom const& G, int q, int* lo, int* hi)
+template<class GT>
+__device__ __forceinline__ void physBox(GT const& G, int q, int* lo, int* hi)
 {
     for (int d = 0; d < 3; ++d)
     {
@@ -135,8 +153,10 @@ __device__ __forceinline__ void physBox(Geom const& G, int q, int* lo, int* hi)
 }
 
 // ---- Faraday ------------------------------------------------------------------------------
-__global__ __launch_bounds__(FB) void k_faraday(Geom G, CVecPtr B, CVecPtr E, VecPtr Bn, double dt)
+template<int DIM>
+__global__ __launch_bounds__(FB) void k_faraday(Geom G0, CVecPtr B, CVecPtr E, VecPtr Bn, double dt)
 {
+    GeomD<DIM> const G(G0);
     long const t = static_cast<long>(blockIdx.x) * FB + threadIdx.x;
     int i, j, k;
     // iterate over the primal-in-every-direction shape (the largest), each component on its box
@@ -182,8 +202,10 @@ __global__ __launch_bounds__(FB) void k_faraday(Geom G, CVecPtr B, CVecPtr E, Ve
 }
 
 // ---- Ampere (ghost box shrunk by one) -------------------------------------------------------
-__global__ __launch_bounds__(FB) void k_ampere(Geom G, CVecPtr B, VecPtr J)
+template<int DIM>
+__global__ __launch_bounds__(FB) void k_ampere(Geom G0, CVecPtr B, VecPtr J)
 {
+    GeomD<DIM> const G(G0);
     long const t = static_cast<long>(blockIdx.x) * FB + threadIdx.x;
     int i, j, k;
     if (!nodeOf(G, QMOM, t, i, j, k))
@@ -229,9 +251,11 @@ __global__ __launch_bounds__(FB) void k_ampere(Geom G, CVecPtr B, VecPtr J)
 }
 
 // ---- electrons: Ve on the physical primal box, Pe everywhere ---------------------------------
-__global__ __launch_bounds__(FB) void k_electrons(Geom G, double Te, CVecPtr J, CVecPtr Vi,
+template<int DIM>
+__global__ __launch_bounds__(FB) void k_electrons(Geom G0, double Te, CVecPtr J, CVecPtr Vi,
                                                    double const* Ne, VecPtr Ve, double* Pe)
 {
+    GeomD<DIM> const G(G0);
     long const t = static_cast<long>(blockIdx.x) * FB + threadIdx.x;
     int i, j, k;
     if (!nodeOf(G, QMOM, t, i, j, k))
@@ -252,16 +276,18 @@ __global__ __launch_bounds__(FB) void k_electrons(Geom G, double Te, CVecPtr J, 
 }
 
 // ---- Ohm --------------------------------------------------------------------------------------
-__global__ __launch_bounds__(FB) void k_ohm(Geom G, double eta, double nu, int hyper, double lvlCoef,
+template<int DIM>
+__global__ __launch_bounds__(FB) void k_ohm(Geom G0, double eta, double nu, int hyper, double lvlCoef,
                                             double const* nn, CVecPtr Ve, double const* Pe, CVecPtr B,
                                             CVecPtr J, VecPtr E, int fill)
 {
+    GeomD<DIM> const G(G0);
     long const t = static_cast<long>(blockIdx.x) * FB + threadIdx.x;
     int i0, j0, k0;
     if (!nodeOf(G, QMOM, t, i0, j0, k0))
         return;
     int const D = G.dim;
-    for (int c = 0; c < 3; ++c)
+    int const c = static_cast<int>(blockIdx.y); // one component per grid row
     {
         int const q = QEX + c;
         int i = i0, j = j0, k = k0;
@@ -270,7 +296,7 @@ __global__ __launch_bounds__(FB) void k_ohm(Geom G, double eta, double nu, int h
         {
             // ghost layers too: a ghost node gets Ohm evaluated at its periodic source
             if (i0 >= G.shape[q][0] || j0 >= G.shape[q][1] || k0 >= G.shape[q][2])
-                continue;
+                return;
             int s[3] = {i0, j0, k0};
             for (int d = 0; d < D; ++d)
             {
@@ -288,7 +314,7 @@ __global__ __launch_bounds__(FB) void k_ohm(Geom G, double eta, double nu, int h
             int lo[3], hi[3];
             physBox(G, q, lo, hi);
             if (!inRange(G, i, j, k, lo, hi))
-                continue;
+                return;
         }
         // moments -> E_c: primal->dual along c
         int const mx = c == 0 ? P2D : PN, my = c == 1 ? P2D : PN, mz = c == 2 ? P2D : PN;
@@ -407,8 +433,10 @@ __global__ __launch_bounds__(FB) void k_ion_totals(PopMoments P, double* rho, do
 
 // ---- periodic same-level fills ----------------------------------------------------------------
 // pure ghost nodes take the value of the periodic interior node (interior not overwritten)
-__global__ __launch_bounds__(FB) void k_fill_ghosts(Geom G, int q, double* f)
+template<int DIM>
+__global__ __launch_bounds__(FB) void k_fill_ghosts(Geom G0, int q, double* f)
 {
+    GeomD<DIM> const G(G0);
     long const t = static_cast<long>(blockIdx.x) * FB + threadIdx.x;
     int i, j, k;
     if (!nodeOf(G, q, t, i, j, k))
@@ -435,9 +463,10 @@ __global__ __launch_bounds__(FB) void k_fill_ghosts(Geom G, int q, double* f)
 
 // image k of {-1,0,1}^dim (lexicographic, zero excluded) overlaps the all-primal ghost box on
 // [max(0, k*N), min(end, end + k*N)]; node n receives orig[n - k*N]
-template<bool MAX>
-__global__ __launch_bounds__(FB) void k_border(Geom G, double const* orig, double* out)
+template<int DIM, bool MAX>
+__global__ __launch_bounds__(FB) void k_border(Geom G0, double const* orig, double* out)
 {
+    GeomD<DIM> const G(G0);
     long const t = static_cast<long>(blockIdx.x) * FB + threadIdx.x;
     int i, j, k;
     if (!nodeOf(G, QMOM, t, i, j, k))
@@ -492,7 +521,8 @@ __global__ __launch_bounds__(FB) void k_border(Geom G, double const* orig, doubl
 // A pure ghost node of a periodic patch holds the value of its periodic source node; computing
 // the stencil directly at the source gives that value bit for bit, so the producer kernels
 // below write their ghost layers themselves instead of a separate fill.
-__device__ __forceinline__ bool wrapNode(Geom const& G, int q, int* c)
+template<class GT>
+__device__ __forceinline__ bool wrapNode(GT const& G, int q, int* c)
 {
     bool ghost = false;
     for (int d = 0; d < G.dim; ++d)
@@ -511,12 +541,14 @@ __device__ __forceinline__ bool wrapNode(Geom const& G, int q, int* c)
     return ghost;
 }
 
-__device__ __forceinline__ bool inShape(Geom const& G, int q, int i, int j, int k)
+template<class GT>
+__device__ __forceinline__ bool inShape(GT const& G, int q, int i, int j, int k)
 {
     return i < G.shape[q][0] && j < G.shape[q][1] && k < G.shape[q][2];
 }
 
-__device__ __forceinline__ double faradayAt(Geom const& G, CVecPtr B, CVecPtr E, double dt, int c,
+template<class GT>
+__device__ __forceinline__ double faradayAt(GT const& G, CVecPtr B, CVecPtr E, double dt, int c,
                                             int i, int j, int k)
 {
     int const D = G.dim;
@@ -541,23 +573,26 @@ __device__ __forceinline__ double faradayAt(Geom const& G, CVecPtr B, CVecPtr E,
 }
 
 // Faraday into a separate output, ghost layers included (predictor: B, E -> Bpred)
-__global__ __launch_bounds__(FB) void k_faraday_fill(Geom G, CVecPtr B, CVecPtr E, VecPtr Bn, double dt)
+template<int DIM>
+__global__ __launch_bounds__(FB) void k_faraday_fill(Geom G0, CVecPtr B, CVecPtr E, VecPtr Bn, double dt)
 {
+    GeomD<DIM> const G(G0);
     long const t = static_cast<long>(blockIdx.x) * FB + threadIdx.x;
     int i, j, k;
     if (!nodeOf(G, QMOM, t, i, j, k))
         return;
-    for (int c = 0; c < 3; ++c)
+    int const c = static_cast<int>(blockIdx.y); // one component per grid row
     {
         if (!inShape(G, QBX + c, i, j, k))
-            continue;
+            return;
         int s[3] = {i, j, k};
         wrapNode(G, QBX + c, s);
         Bn.c[c][G.idx(QBX + c, i, j, k)] = faradayAt(G, B, E, dt, c, s[0], s[1], s[2]);
     }
 }
 
-__device__ __forceinline__ double ampereAt(Geom const& G, CVecPtr B, int c, int i, int j, int k)
+template<class GT>
+__device__ __forceinline__ double ampereAt(GT const& G, CVecPtr B, int c, int i, int j, int k)
 {
     int const D = G.dim;
     if (c == 0)
@@ -580,16 +615,18 @@ __device__ __forceinline__ double ampereAt(Geom const& G, CVecPtr B, int c, int 
 }
 
 // Ampere + fillCurrentGhosts
-__global__ __launch_bounds__(FB) void k_ampere_fill(Geom G, CVecPtr B, VecPtr J)
+template<int DIM>
+__global__ __launch_bounds__(FB) void k_ampere_fill(Geom G0, CVecPtr B, VecPtr J)
 {
+    GeomD<DIM> const G(G0);
     long const t = static_cast<long>(blockIdx.x) * FB + threadIdx.x;
     int i, j, k;
     if (!nodeOf(G, QMOM, t, i, j, k))
         return;
-    for (int c = 0; c < 3; ++c)
+    int const c = static_cast<int>(blockIdx.y); // one component per grid row
     {
         if (!inShape(G, QJX + c, i, j, k))
-            continue;
+            return;
         int s[3] = {i, j, k};
         wrapNode(G, QJX + c, s);
         J.c[c][G.idx(QJX + c, i, j, k)] = ampereAt(G, B, c, s[0], s[1], s[2]);
@@ -597,17 +634,19 @@ __global__ __launch_bounds__(FB) void k_ampere_fill(Geom G, CVecPtr B, VecPtr J)
 }
 
 // ghost fill of three components in one launch
-__global__ __launch_bounds__(FB) void k_fill3(Geom G, int q0, VecPtr F)
+template<int DIM>
+__global__ __launch_bounds__(FB) void k_fill3(Geom G0, int q0, VecPtr F)
 {
+    GeomD<DIM> const G(G0);
     long const t = static_cast<long>(blockIdx.x) * FB + threadIdx.x;
     int i, j, k;
     if (!nodeOf(G, QMOM, t, i, j, k))
         return;
-    for (int c = 0; c < 3; ++c)
+    int const c = static_cast<int>(blockIdx.y); // one component per grid row
     {
         int const q = q0 + c;
         if (!inShape(G, q, i, j, k))
-            continue;
+            return;
         int s[3] = {i, j, k};
         if (wrapNode(G, q, s))
             F.c[c][G.idx(q, i, j, k)] = F.c[c][G.idx(q, s[0], s[1], s[2])];
@@ -615,14 +654,16 @@ __global__ __launch_bounds__(FB) void k_fill3(Geom G, int q0, VecPtr F)
 }
 
 // average of E and B sets (algorithm.hpp:68-76) + fillElectricGhosts of Eavg
-__global__ __launch_bounds__(FB) void k_average_fill(Geom G, CVecPtr E, CVecPtr Ep, CVecPtr B,
+template<int DIM>
+__global__ __launch_bounds__(FB) void k_average_fill(Geom G0, CVecPtr E, CVecPtr Ep, CVecPtr B,
                                                      CVecPtr Bp, VecPtr Ea, VecPtr Ba)
 {
+    GeomD<DIM> const G(G0);
     long const t = static_cast<long>(blockIdx.x) * FB + threadIdx.x;
     int i, j, k;
     if (!nodeOf(G, QMOM, t, i, j, k))
         return;
-    for (int c = 0; c < 3; ++c)
+    int const c = static_cast<int>(blockIdx.y); // one component per grid row
     {
         if (inShape(G, QEX + c, i, j, k))
         {
@@ -648,66 +689,87 @@ struct MomSet
     int npop;
 };
 
+// periodic images of a node of the all-primal grid: image m (lexicographic over shifts
+// {-1,0,1}^D less the zero shift) overlaps the node when the node lies in the array shifted by
+// that many patch widths; its source is then at the linear offset -off[m]
+template<int D>
+struct Images
+{
+    static constexpr int N = D == 1 ? 2 : (D == 2 ? 8 : 26);
+    bool valid[N];
+    int off[N];
+};
+
+template<class GT>
+__device__ __forceinline__ void imagesOf(GT const& G, int const* c, Images<GT::dim>& I)
+{
+    constexpr int D     = GT::dim;
+    constexpr int total = D == 1 ? 3 : (D == 2 ? 9 : 27);
+    int m               = 0;
+#pragma unroll
+    for (int im = 0; im < total; ++im)
+    {
+        int kk[3] = {0, 0, 0};
+        int r     = im;
+#pragma unroll
+        for (int d = D - 1; d >= 0; --d)
+        {
+            kk[d] = (r % 3) - 1;
+            r /= 3;
+        }
+        if (kk[0] == 0 && kk[1] == 0 && kk[2] == 0)
+            continue;
+        bool in   = true;
+        int shift = 0;
+#pragma unroll
+        for (int d = 0; d < D; ++d)
+        {
+            int const end = G.ghostEnd(QMOM, d);
+            int const sh  = kk[d] * G.n[d];
+            int const lo  = sh > 0 ? sh : 0;
+            int const hi  = end + sh < end ? end + sh : end;
+            in            = in && c[d] >= lo && c[d] <= hi;
+            int const stride = d == 0 ? G.shape[QMOM][1] * G.shape[QMOM][2]
+                                      : (d == 1 ? G.shape[QMOM][2] : 1);
+            shift += sh * stride;
+        }
+        I.valid[m] = in;
+        I.off[m]   = shift;
+        ++m;
+    }
+}
+
 // fillFluxBorders + fillDensityBorders (border sum of every population moment) + updateIons
 // (totals into tmp) ; the max fill of the totals follows in k_max5
-__global__ __launch_bounds__(FB) void k_fold_totals(Geom G, MomSet M, double* trho, double* tmd,
+template<int DIM>
+__global__ __launch_bounds__(FB) void k_fold_totals(Geom G0, MomSet M, double* trho, double* tmd,
                                                     VecPtr tvi)
 {
+    GeomD<DIM> const G(G0);
     long const t = static_cast<long>(blockIdx.x) * FB + threadIdx.x;
     int i, j, k;
     if (!nodeOf(G, QMOM, t, i, j, k))
         return;
     int const c[3] = {i, j, k};
     int const n    = G.idx(QMOM, i, j, k);
-    int total      = 1;
-    for (int d = 0; d < G.dim; ++d)
-        total *= 3;
     // sources of this node: itself, then the overlapping periodic images in lexicographic order
-    int src[27];
-    int ns = 0;
-    for (int im = 0; im < total; ++im)
-    {
-        int kk[3] = {0, 0, 0};
-        int r     = im;
-        bool zero = true;
-        for (int d = G.dim - 1; d >= 0; --d)
-        {
-            kk[d] = (r % 3) - 1;
-            r /= 3;
-            if (kk[d])
-                zero = false;
-        }
-        if (zero)
-            continue;
-        bool in  = true;
-        int s[3] = {i, j, k};
-        for (int d = 0; d < G.dim; ++d)
-        {
-            int const end = G.ghostEnd(QMOM, d);
-            int const sh  = kk[d] * G.n[d];
-            int const lo  = sh > 0 ? sh : 0;
-            int const hi  = end + sh < end ? end + sh : end;
-            if (c[d] < lo || c[d] > hi)
-            {
-                in = false;
-                break;
-            }
-            s[d] = c[d] - sh;
-        }
-        if (in)
-            src[ns++] = G.idx(QMOM, s[0], s[1], s[2]);
-    }
+    Images<DIM> I;
+    imagesOf(G, c, I);
     double r = 0., m = 0., vx = 0., vy = 0., vz = 0.;
     for (int p = 0; p < M.npop; ++p)
     {
         double f[5];
+#pragma unroll
         for (int q = 0; q < 5; ++q)
         {
-            double v = M.raw[p][q][n];
-            for (int x = 0; x < ns; ++x)
-                v += M.raw[p][q][src[x]];
-            f[q]          = v;
-            M.mom[p][q][n] = v;
+            double const* raw = M.raw[p][q];
+            double v          = raw[n];
+#pragma unroll
+            for (int x = 0; x < Images<DIM>::N; ++x)
+                if (I.valid[x])
+                    v += raw[n - I.off[x]];
+            f[q]            = v;
+            M.mom[p][q][n]  = v;
             M.zero[p][q][n] = 0.;
         }
         r  = r + f[1];
@@ -730,60 +792,31 @@ struct Five
 };
 
 // fillIonBorders: max over the periodic images (snapshot in src)
-__global__ __launch_bounds__(FB) void k_max5(Geom G, Five F)
+template<int DIM>
+__global__ __launch_bounds__(FB) void k_max5(Geom G0, Five F)
 {
+    GeomD<DIM> const G(G0);
     long const t = static_cast<long>(blockIdx.x) * FB + threadIdx.x;
     int i, j, k;
     if (!nodeOf(G, QMOM, t, i, j, k))
         return;
     int const c[3] = {i, j, k};
     int const n    = G.idx(QMOM, i, j, k);
-    double v[5];
+    Images<DIM> I;
+    imagesOf(G, c, I);
+#pragma unroll
     for (int q = 0; q < 5; ++q)
-        v[q] = F.src[q][n];
-    int total = 1;
-    for (int d = 0; d < G.dim; ++d)
-        total *= 3;
-    for (int im = 0; im < total; ++im)
     {
-        int kk[3] = {0, 0, 0};
-        int r     = im;
-        bool zero = true;
-        for (int d = G.dim - 1; d >= 0; --d)
-        {
-            kk[d] = (r % 3) - 1;
-            r /= 3;
-            if (kk[d])
-                zero = false;
-        }
-        if (zero)
-            continue;
-        bool in  = true;
-        int s[3] = {i, j, k};
-        for (int d = 0; d < G.dim; ++d)
-        {
-            int const end = G.ghostEnd(QMOM, d);
-            int const sh  = kk[d] * G.n[d];
-            int const lo  = sh > 0 ? sh : 0;
-            int const hi  = end + sh < end ? end + sh : end;
-            if (c[d] < lo || c[d] > hi)
+        double v = F.src[q][n];
+#pragma unroll
+        for (int x = 0; x < Images<DIM>::N; ++x)
+            if (I.valid[x])
             {
-                in = false;
-                break;
+                double const o = F.src[q][n - I.off[x]];
+                v              = v < o ? o : v;
             }
-            s[d] = c[d] - sh;
-        }
-        if (!in)
-            continue;
-        int const sn = G.idx(QMOM, s[0], s[1], s[2]);
-        for (int q = 0; q < 5; ++q)
-        {
-            double const o = F.src[q][sn];
-            v[q]           = v[q] < o ? o : v[q];
-        }
+        F.dst[q][n] = v;
     }
-    for (int q = 0; q < 5; ++q)
-        F.dst[q][n] = v[q];
 }
 
 unsigned int gridFor(long n) { return static_cast<unsigned int>((n + FB - 1) / FB); }
@@ -792,16 +825,31 @@ unsigned int gridFor(long n) { return static_cast<unsigned int>((n + FB - 1) / F
 
 void launchFaraday(Geom const& G, CVecPtr B, CVecPtr E, VecPtr Bnew, double dt, hipStream_t s)
 {
-    k_faraday<<<gridFor(G.size(QMOM)), FB, 0, s>>>(G, B, E, Bnew, dt);
+    switch (G.dim)
+    {
+        case 1: k_faraday<1><<<gridFor(G.size(QMOM)), FB, 0, s>>>(G, B, E, Bnew, dt); break;
+        case 2: k_faraday<2><<<gridFor(G.size(QMOM)), FB, 0, s>>>(G, B, E, Bnew, dt); break;
+        case 3: k_faraday<3><<<gridFor(G.size(QMOM)), FB, 0, s>>>(G, B, E, Bnew, dt); break;
+    }
 }
 void launchAmpere(Geom const& G, CVecPtr B, VecPtr J, hipStream_t s)
 {
-    k_ampere<<<gridFor(G.size(QMOM)), FB, 0, s>>>(G, B, J);
+    switch (G.dim)
+    {
+        case 1: k_ampere<1><<<gridFor(G.size(QMOM)), FB, 0, s>>>(G, B, J); break;
+        case 2: k_ampere<2><<<gridFor(G.size(QMOM)), FB, 0, s>>>(G, B, J); break;
+        case 3: k_ampere<3><<<gridFor(G.size(QMOM)), FB, 0, s>>>(G, B, J); break;
+    }
 }
 void launchElectrons(Geom const& G, double Te, CVecPtr J, CVecPtr Vi, double const* Ne, VecPtr Ve,
                      double* Pe, hipStream_t s)
 {
-    k_electrons<<<gridFor(G.size(QMOM)), FB, 0, s>>>(G, Te, J, Vi, Ne, Ve, Pe);
+    switch (G.dim)
+    {
+        case 1: k_electrons<1><<<gridFor(G.size(QMOM)), FB, 0, s>>>(G, Te, J, Vi, Ne, Ve, Pe); break;
+        case 2: k_electrons<2><<<gridFor(G.size(QMOM)), FB, 0, s>>>(G, Te, J, Vi, Ne, Ve, Pe); break;
+        case 3: k_electrons<3><<<gridFor(G.size(QMOM)), FB, 0, s>>>(G, Te, J, Vi, Ne, Ve, Pe); break;
+    }
 }
 void launchOhm(Geom const& G, double eta, double nu, int hyper, int level, double const* n,
                CVecPtr Ve, double const* Pe, CVecPtr B, CVecPtr J, VecPtr E, hipStream_t s, int fill)
@@ -810,7 +858,12 @@ void launchOhm(Geom const& G, double eta, double nu, int hyper, int level, doubl
     for (int l = 0; l < level; ++l)
         lvlCoef *= 4.;
     lvlCoef = 1. / lvlCoef;
-    k_ohm<<<gridFor(G.size(QMOM)), FB, 0, s>>>(G, eta, nu, hyper, lvlCoef, n, Ve, Pe, B, J, E, fill);
+    switch (G.dim)
+    {
+        case 1: k_ohm<1><<<dim3(gridFor(G.size(QMOM)), 3), FB, 0, s>>>(G, eta, nu, hyper, lvlCoef, n, Ve, Pe, B, J, E, fill); break;
+        case 2: k_ohm<2><<<dim3(gridFor(G.size(QMOM)), 3), FB, 0, s>>>(G, eta, nu, hyper, lvlCoef, n, Ve, Pe, B, J, E, fill); break;
+        case 3: k_ohm<3><<<dim3(gridFor(G.size(QMOM)), 3), FB, 0, s>>>(G, eta, nu, hyper, lvlCoef, n, Ve, Pe, B, J, E, fill); break;
+    }
 }
 void launchAverage(double const* a, double const* b, double* out, long count, hipStream_t s)
 {
@@ -823,15 +876,30 @@ void launchCopy(double const* src, double* dst, long count, hipStream_t s)
 }
 void launchFillGhosts(Geom const& G, int q, double* f, hipStream_t s)
 {
-    k_fill_ghosts<<<gridFor(G.size(q)), FB, 0, s>>>(G, q, f);
+    switch (G.dim)
+    {
+        case 1: k_fill_ghosts<1><<<gridFor(G.size(q)), FB, 0, s>>>(G, q, f); break;
+        case 2: k_fill_ghosts<2><<<gridFor(G.size(q)), FB, 0, s>>>(G, q, f); break;
+        case 3: k_fill_ghosts<3><<<gridFor(G.size(q)), FB, 0, s>>>(G, q, f); break;
+    }
 }
 void launchBorderSum(Geom const& G, double const* orig, double* out, hipStream_t s)
 {
-    k_border<false><<<gridFor(G.size(QMOM)), FB, 0, s>>>(G, orig, out);
+    switch (G.dim)
+    {
+        case 1: k_border<1, false><<<gridFor(G.size(QMOM)), FB, 0, s>>>(G, orig, out); break;
+        case 2: k_border<2, false><<<gridFor(G.size(QMOM)), FB, 0, s>>>(G, orig, out); break;
+        case 3: k_border<3, false><<<gridFor(G.size(QMOM)), FB, 0, s>>>(G, orig, out); break;
+    }
 }
 void launchBorderMax(Geom const& G, double const* orig, double* inout, hipStream_t s)
 {
-    k_border<true><<<gridFor(G.size(QMOM)), FB, 0, s>>>(G, orig, inout);
+    switch (G.dim)
+    {
+        case 1: k_border<1, true><<<gridFor(G.size(QMOM)), FB, 0, s>>>(G, orig, inout); break;
+        case 2: k_border<2, true><<<gridFor(G.size(QMOM)), FB, 0, s>>>(G, orig, inout); break;
+        case 3: k_border<3, true><<<gridFor(G.size(QMOM)), FB, 0, s>>>(G, orig, inout); break;
+    }
 }
 void launchIonTotals(PopMoments const& P, double* rho, double* massDensity, VecPtr Vi, long count,
                      hipStream_t s)
@@ -841,20 +909,40 @@ void launchIonTotals(PopMoments const& P, double* rho, double* massDensity, VecP
 
 void launchFaradayFill(Geom const& G, CVecPtr B, CVecPtr E, VecPtr Bnew, double dt, hipStream_t s)
 {
-    k_faraday_fill<<<gridFor(G.size(QMOM)), FB, 0, s>>>(G, B, E, Bnew, dt);
+    switch (G.dim)
+    {
+        case 1: k_faraday_fill<1><<<dim3(gridFor(G.size(QMOM)), 3), FB, 0, s>>>(G, B, E, Bnew, dt); break;
+        case 2: k_faraday_fill<2><<<dim3(gridFor(G.size(QMOM)), 3), FB, 0, s>>>(G, B, E, Bnew, dt); break;
+        case 3: k_faraday_fill<3><<<dim3(gridFor(G.size(QMOM)), 3), FB, 0, s>>>(G, B, E, Bnew, dt); break;
+    }
 }
 void launchAmpereFill(Geom const& G, CVecPtr B, VecPtr J, hipStream_t s)
 {
-    k_ampere_fill<<<gridFor(G.size(QMOM)), FB, 0, s>>>(G, B, J);
+    switch (G.dim)
+    {
+        case 1: k_ampere_fill<1><<<dim3(gridFor(G.size(QMOM)), 3), FB, 0, s>>>(G, B, J); break;
+        case 2: k_ampere_fill<2><<<dim3(gridFor(G.size(QMOM)), 3), FB, 0, s>>>(G, B, J); break;
+        case 3: k_ampere_fill<3><<<dim3(gridFor(G.size(QMOM)), 3), FB, 0, s>>>(G, B, J); break;
+    }
 }
 void launchFill3(Geom const& G, int q0, VecPtr F, hipStream_t s)
 {
-    k_fill3<<<gridFor(G.size(QMOM)), FB, 0, s>>>(G, q0, F);
+    switch (G.dim)
+    {
+        case 1: k_fill3<1><<<dim3(gridFor(G.size(QMOM)), 3), FB, 0, s>>>(G, q0, F); break;
+        case 2: k_fill3<2><<<dim3(gridFor(G.size(QMOM)), 3), FB, 0, s>>>(G, q0, F); break;
+        case 3: k_fill3<3><<<dim3(gridFor(G.size(QMOM)), 3), FB, 0, s>>>(G, q0, F); break;
+    }
 }
 void launchAverageFill(Geom const& G, CVecPtr E, CVecPtr Ep, CVecPtr B, CVecPtr Bp, VecPtr Ea,
                        VecPtr Ba, hipStream_t s)
 {
-    k_average_fill<<<gridFor(G.size(QMOM)), FB, 0, s>>>(G, E, Ep, B, Bp, Ea, Ba);
+    switch (G.dim)
+    {
+        case 1: k_average_fill<1><<<dim3(gridFor(G.size(QMOM)), 3), FB, 0, s>>>(G, E, Ep, B, Bp, Ea, Ba); break;
+        case 2: k_average_fill<2><<<dim3(gridFor(G.size(QMOM)), 3), FB, 0, s>>>(G, E, Ep, B, Bp, Ea, Ba); break;
+        case 3: k_average_fill<3><<<dim3(gridFor(G.size(QMOM)), 3), FB, 0, s>>>(G, E, Ep, B, Bp, Ea, Ba); break;
+    }
 }
 void launchFoldTotals(Geom const& G, MomSetArgs const& A, double* trho, double* tmd, VecPtr tvi,
                       hipStream_t s)
@@ -871,7 +959,12 @@ void launchFoldTotals(Geom const& G, MomSetArgs const& A, double* trho, double* 
             M.zero[p][q] = A.zero[p][q];
         }
     }
-    k_fold_totals<<<gridFor(G.size(QMOM)), FB, 0, s>>>(G, M, trho, tmd, tvi);
+    switch (G.dim)
+    {
+        case 1: k_fold_totals<1><<<gridFor(G.size(QMOM)), FB, 0, s>>>(G, M, trho, tmd, tvi); break;
+        case 2: k_fold_totals<2><<<gridFor(G.size(QMOM)), FB, 0, s>>>(G, M, trho, tmd, tvi); break;
+        case 3: k_fold_totals<3><<<gridFor(G.size(QMOM)), FB, 0, s>>>(G, M, trho, tmd, tvi); break;
+    }
 }
 void launchMax5(Geom const& G, double const* const* src, double* const* dst, hipStream_t s)
 {
@@ -881,7 +974,12 @@ void launchMax5(Geom const& G, double const* const* src, double* const* dst, hip
         F.src[q] = src[q];
         F.dst[q] = dst[q];
     }
-    k_max5<<<gridFor(G.size(QMOM)), FB, 0, s>>>(G, F);
+    switch (G.dim)
+    {
+        case 1: k_max5<1><<<gridFor(G.size(QMOM)), FB, 0, s>>>(G, F); break;
+        case 2: k_max5<2><<<gridFor(G.size(QMOM)), FB, 0, s>>>(G, F); break;
+        case 3: k_max5<3><<<gridFor(G.size(QMOM)), FB, 0, s>>>(G, F); break;
+    }
 }
 
 } // namespace phx

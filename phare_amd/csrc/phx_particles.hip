@@ -297,35 +297,61 @@ __device__ __forceinline__ void accAdd(Acc<D, O>& A, DepositData<D, O> const& dd
     }
 }
 
+// the fields of one particle read from its slot (cell[0] is DEAD_CELL for an empty slot)
+template<int D>
+struct Part
+{
+    int cell[D];
+    double delta[D], v[3], w, q;
+};
+
+template<int D>
+__device__ __forceinline__ void loadPart(PView const& in, unsigned long long i, Part<D>& p)
+{
+    p.cell[0] = in.ic[0][i];
+    if (p.cell[0] == DEAD_CELL)
+        return;
+#pragma unroll
+    for (int d = 1; d < D; ++d)
+        p.cell[d] = in.ic[d][i];
+#pragma unroll
+    for (int d = 0; d < D; ++d)
+        p.delta[d] = in.de[d][i];
+    p.v[0] = in.v[0][i];
+    p.v[1] = in.v[1][i];
+    p.v[2] = in.v[2][i];
+    p.w    = in.w[i];
+    p.q    = in.q[i];
+}
+
 // One particle of updatePopulations: prePush, gather, Boris, postPush (boris.hpp:93-138), the
 // selection of ion_updater.hpp:169-295, the deposit and, in mode all, the write-back. Dead slots
 // (lost particles, empty slots of segmented storage) are skipped, and dropped by a compaction.
 template<int D, int O>
-__device__ __forceinline__ void pushParticle(PushArgs const& A, PView const& in, PView const& out,
-                                             unsigned long long i, TileView const& T, int copy,
-                                             Acc<D, O>& acc)
+__device__ __forceinline__ void pushLoaded(PushArgs const& A, PView const& out, bool outOfPlace,
+                                           unsigned long long i, Part<D> const& P,
+                                           TileView const& T, int copy, Acc<D, O>& acc)
 {
     using C       = Cfg<D, O>;
     Geom const& G = A.G;
-    int cell[D];
-    double delta[D], v[3];
-    cell[0] = in.ic[0][i];
-    if (cell[0] == DEAD_CELL)
+    if (P.cell[0] == DEAD_CELL)
     {
         if (A.pushOn && A.mode == MODE_ALL && A.status)
             A.status[i] = ST_DROP;
         return;
     }
-#pragma unroll
-    for (int d = 1; d < D; ++d)
-        cell[d] = in.ic[d][i];
+    int cell[D];
+    double delta[D], v[3];
 #pragma unroll
     for (int d = 0; d < D; ++d)
-        delta[d] = in.de[d][i];
-    v[0] = in.v[0][i];
-    v[1] = in.v[1][i];
-    v[2] = in.v[2][i];
-    double const w = in.w[i], q = in.q[i];
+    {
+        cell[d]  = P.cell[d];
+        delta[d] = P.delta[d];
+    }
+    v[0] = P.v[0];
+    v[1] = P.v[1];
+    v[2] = P.v[2];
+    double const w = P.w, q = P.q;
 
     bool deposit = true, sel1 = true, inDom = true;
     uint8_t st   = ST_KEEP;
@@ -344,9 +370,17 @@ __device__ __forceinline__ void pushParticle(PushArgs const& A, PView const& in,
         {
             Stencil<D, O> s;
             makeStencil<D, O>(G, cell, delta, s);
-            if (!stencilInside<D, O>(G, s, QEX) || !stencilInside<D, O>(G, s, QBX)
-                || !stencilInside<D, O>(G, s, QEY) || !stencilInside<D, O>(G, s, QBY)
-                || !stencilInside<D, O>(G, s, QEZ) || !stencilInside<D, O>(G, s, QBZ))
+            bool safe = true;
+#pragma unroll
+            for (int d = 0; d < D; ++d)
+            {
+                int const icl = cell[d] - (G.lower[d] - G.g);
+                safe          = safe && icl >= G.safeLo[d] && icl <= G.safeHi[d];
+            }
+            if (!safe
+                && (!stencilInside<D, O>(G, s, QEX) || !stencilInside<D, O>(G, s, QBX)
+                    || !stencilInside<D, O>(G, s, QEY) || !stencilInside<D, O>(G, s, QBY)
+                    || !stencilInside<D, O>(G, s, QEZ) || !stencilInside<D, O>(G, s, QBZ)))
             {
                 raiseError(A.err, 4u, 2u, i, delta[0], v[0], A.kind, A.pop);
                 return;
@@ -449,7 +483,7 @@ __device__ __forceinline__ void pushParticle(PushArgs const& A, PView const& in,
         out.v[0][i] = v[0];
         out.v[1][i] = v[1];
         out.v[2][i] = v[2];
-        if (out.w != in.w)
+        if (outOfPlace)
         {
             out.w[i] = w;
             out.q[i] = q;
@@ -457,6 +491,16 @@ __device__ __forceinline__ void pushParticle(PushArgs const& A, PView const& in,
         if (A.status)
             A.status[i] = st;
     }
+}
+
+template<int D, int O>
+__device__ __forceinline__ void pushParticle(PushArgs const& A, PView const& in, PView const& out,
+                                             unsigned long long i, TileView const& T, int copy,
+                                             Acc<D, O>& acc)
+{
+    Part<D> P;
+    loadPart<D>(in, i, P);
+    pushLoaded<D, O>(A, out, out.w != in.w, i, P, T, copy, acc);
 }
 
 __device__ __forceinline__ void zeroTile(double* lds, TileSpec const& S)
@@ -565,8 +609,9 @@ __global__ __launch_bounds__(BLOCK) void k_push(PushArgs A, PView in, PView out)
 #endif
 template<int D, int O>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(PHX_SEG_WAVES)))
-void k_push_seg(PushArgs A, PView in, PView out)
+void k_push_seg(PushArgs A, PView in)
 {
+    PView const& out = in; // segmented arrays are pushed in place
     using C = Cfg<D, O>;
     extern __shared__ double lds[];
     unsigned int const item = blockIdx.x;
@@ -613,9 +658,25 @@ void k_push_seg(PushArgs A, PView in, PView out)
     if constexpr (C::ACC)
         accZero(acc);
     unsigned int const rEnd = min(r0 + static_cast<unsigned int>(SEG_ROWS), R);
+#ifdef PHX_PREFETCH
+    // the next row's particle is read while this row's is pushed
+    unsigned int r = r0 + (t >> 6);
+    Part<D> next;
+    if (r < rEnd)
+        loadPart<D>(in, off + static_cast<unsigned long long>(r) * LANES + lane, next);
+    for (; r < rEnd; r += NW)
+    {
+        Part<D> const cur = next;
+        if (r + NW < rEnd)
+            loadPart<D>(in, off + static_cast<unsigned long long>(r + NW) * LANES + lane, next);
+        pushLoaded<D, O>(A, out, false, off + static_cast<unsigned long long>(r) * LANES + lane,
+                         cur, T, copy, acc);
+    }
+#else
     for (unsigned int r = r0 + (t >> 6); r < rEnd; r += NW)
         pushParticle<D, O>(A, in, out, off + static_cast<unsigned long long>(r) * LANES + lane, T,
                            copy, acc);
+#endif
     if constexpr (C::ACC)
         accFlush<D, O>(acc, G, T, copy, A.mom);
     __syncthreads();
@@ -1067,7 +1128,8 @@ static void launchPushT(PushArgs const& A, PView const& in, PView const& out, hi
 {
     if (A.segOn)
     {
-        k_push_seg<D, O><<<A.st.itemsCap, BLOCK, A.tile.bytes, s>>>(A, in, out);
+        // segmented arrays are pushed in place (out is in)
+        k_push_seg<D, O><<<A.st.itemsCap, BLOCK, A.tile.bytes, s>>>(A, in);
         return;
     }
     if (A.n == 0)
