@@ -86,7 +86,8 @@ struct phx_ctx
     hipStream_t stream = nullptr;
 
     std::vector<void*> allocs;
-    double* em[3][6] = {};
+    double* em[3][6] = {};    // E/B of each set, one block per set (em[set][0] is its base)
+    unsigned int emBytes = 0; // bytes of one set's block
     double* J[3]     = {};
     double* Ve[3]    = {};
     double* Pe = nullptr, *rho = nullptr, *massD = nullptr, *tmp = nullptr;
@@ -338,6 +339,9 @@ PushArgs pushArgs(phx_ctx* c, int pop, int kind, int mode, int emSet, double dt,
         A.em.E[i] = c->em[emSet][i];
         A.em.B[i] = c->em[emSet][3 + i];
     }
+    A.em.bytes = c->emBytes;
+    for (int k = 0; k < 6; ++k)
+        A.em.off[k] = static_cast<unsigned int>((c->em[emSet][k] - c->em[emSet][0]) * 8);
     for (int m = 0; m < 5; ++m)
         A.mom[m] = c->pops[pop].mom[m];
     for (int d = 0; d < 3; ++d)
@@ -742,6 +746,9 @@ std::vector<unsigned long long> storageKey(phx_ctx const* c)
 {
     std::vector<unsigned long long> k;
     for (int p = 0; p < c->cfg.n_pop; ++p)
+    {
+        // the segment tables a segmented push reads (reallocated when the storage grows)
+        k.push_back(reinterpret_cast<unsigned long long>(c->pops[p].seg.items));
         for (Store const& s : c->pops[p].arr)
         {
             // segmented storage: the kernels read their extent from the segment tables, so a
@@ -751,6 +758,7 @@ std::vector<unsigned long long> storageKey(phx_ctx const* c)
             k.push_back(s.cap);
             k.push_back(s.seg ? ~0ull : s.n);
         }
+    }
     return k;
 }
 
@@ -894,9 +902,23 @@ phx_status phx_create(phx_config const* cfg, phx_patch const* patch, size_t part
     }
     auto alloc = [&](double** p, int q) { return devAlloc(c, (void**)p, static_cast<size_t>(G.size(q)) * 8); };
     phx_status s = PHX_OK;
+    // each E/B set is one block (Ex, Ey, Ez, Bx, By, Bz, each starting on a 256-byte boundary),
+    // so that the gather addresses all six components from one base with 32-bit offsets
+    size_t emWords = 0;
+    size_t emOff[6];
+    for (int k = 0; k < 6; ++k)
+    {
+        emOff[k] = emWords;
+        emWords += (static_cast<size_t>(G.size(k < 3 ? QEX + k : QBX + k - 3)) + 31) / 32 * 32;
+    }
     for (int set = 0; set < 3 && s == PHX_OK; ++set)
+    {
+        double* block = nullptr;
+        s             = devAlloc(c, (void**)&block, emWords * 8);
         for (int k = 0; k < 6 && s == PHX_OK; ++k)
-            s = alloc(&c->em[set][k], k < 3 ? QEX + k : QBX + k - 3);
+            c->em[set][k] = block + emOff[k];
+    }
+    c->emBytes = static_cast<unsigned int>(emWords * 8);
     for (int k = 0; k < 3 && s == PHX_OK; ++k)
     {
         s = alloc(&c->J[k], QJX + k);

@@ -267,6 +267,8 @@ struct EMPtrs
 {
     double const* E[3];
     double const* B[3];
+    unsigned int off[6]; // byte offsets of Ex..Bz from E[0] (one block per set)
+    unsigned int bytes;  // size of the block
 };
 
 // ---------------------------------------------------------------------------------------------

