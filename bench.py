@@ -29,10 +29,10 @@ from phare_amd import workloads as W  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 
-# HBM bytes per k_push dispatch (mode all) from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of
+# HBM bytes per push dispatch (k_push_seg, mode all) from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of
 # this script, FETCH_SIZE scaled x2 and WRITE_SIZE x1 by a 1 GiB device copy's readings; keyed by
 # workload (kernel variant, precision), measured for the default Harris workload only
-MEASURED_TRAFFIC = {("harris2d", "fp64"): 395.0e6}
+MEASURED_TRAFFIC = {("harris2d", "fp64"): 402.0e6}
 
 
 def _hip():
@@ -188,7 +188,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": MEASURED_TRAFFIC.get((args.workload, "fp64")),
-                     "kernel": "k_push (fused prePush+gather+Boris+postPush+deposit, mode all)",
+                     "kernel": "k_push_seg (fused prePush+gather+Boris+postPush+deposit on cell-segmented storage, mode all)",
                      "kernel_ms": ms_all, "kernel_ms_domain_only": ms_dom,
                      "algorithmic_bytes_per_launch": bytes_all},
         "step_hbm_fraction": step_bytes(wl) / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS,

@@ -604,6 +604,9 @@ phx_status upload(phx_ctx* c, Store& s, size_t n, int32_t const* ic, double cons
     int const D = c->cfg.dim;
     PHX_TRY(storeReserve(c, s, offset + n));
     s.seg = false;
+    // the copies below run on the null stream, which does not wait for the context's
+    // non-blocking stream: nothing queued there may still use the array
+    PHX_HIP(c, hipStreamSynchronize(c->stream));
     // appended particles start a fresh chunk alignment only when the array was empty
     std::vector<size_t> const ord = offset == 0 ? chunkInterleave(n, D, c->cfg.interp_order)
                                                 : chunkInterleave(0, D, c->cfg.interp_order);
