@@ -113,7 +113,7 @@ struct phx_ctx
     Store sortBuf[PHX_MAX_POP];
     SegGeom sg{};
     unsigned int* segWork = nullptr; // per-cell counts and per-segment work items of a sort
-    int sortInterval      = 25;
+    int sortInterval      = 50;
     int stepsSinceSort    = 0;
 
     // captured PPC steps, keyed on dt, the raw deposit set and the particle storage they address
@@ -720,8 +720,10 @@ phx_status sortDomain(phx_ctx* c)
         PHX_HIP(c, hipMemcpy(&slots, P.seg.off + c->sg.nsegs + 1, 8, hipMemcpyDeviceToHost));
         if (buf.cap < slots || buf.cap < dom.cap)
         {
+            // room for the empty slots of growing segments, so that later sorts reuse it
+            size_t const want = slots + slots / 8 + 4096;
             storeFree(c, buf);
-            PHX_TRY(storeAlloc(c, buf, std::max<size_t>(std::max<size_t>(slots, dom.cap), 1024)));
+            PHX_TRY(storeAlloc(c, buf, std::max(want, dom.cap)));
         }
         // work items for the larger of the two storages the sort alternates between
         unsigned int const need = segItemsCap(c->sg, std::max(buf.cap, dom.cap));
@@ -739,6 +741,15 @@ phx_status sortDomain(phx_ctx* c)
         buf.seg = true;
         std::swap(dom, buf);
         buf.seg = false;
+        buf.n   = 0;
+        if (buf.cap < dom.cap)
+        {
+            // the two storages the sorts alternate between keep one capacity: no reallocation
+            // (and no new captured step) at later sorts
+            PHX_HIP(c, hipStreamSynchronize(c->stream));
+            storeFree(c, buf);
+            PHX_TRY(storeAlloc(c, buf, dom.cap));
+        }
     }
     c->stepsSinceSort = 0;
     return PHX_OK;

@@ -23,7 +23,7 @@
 #define PHX_PPT_ND 2 // particles per thread of the linear 2D/3D push
 #endif
 #ifndef PHX_SEG_ROWS
-#define PHX_SEG_ROWS 32 // segment rows per workgroup of the segmented push
+#define PHX_SEG_ROWS 16 // segment rows per workgroup of the segmented push
 #endif
 
 namespace phx
