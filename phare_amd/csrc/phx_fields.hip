@@ -54,10 +54,11 @@ __device__ __forceinline__ bool inRange(GT const& G, int i, int j, int k, int co
     return true;
 }
 
-// GridLayout::deriv (gridlayout.hpp:571-632): operand centering decides next/prev
-template<class GT>
-__device__ __forceinline__ double deriv(GT const& G, double const* f, int q, int dir, int i, int j,
-                                        int k)
+// GridLayout::deriv (gridlayout.hpp:571-632): operand centering decides next/prev; f(i, j, k)
+// gives the operand at a node
+template<class GT, class F>
+__device__ __forceinline__ double derivFn(GT const& G, F const& f, int q, int dir, int i, int j,
+                                          int k)
 {
     int const c = centering(q, dir);
     int in = i, jn = j, kn = k, ip = i, jp = j, kp = k;
@@ -78,7 +79,14 @@ __device__ __forceinline__ double deriv(GT const& G, double const* f, int q, int
         kn = k + up;
         kp = k + dn;
     }
-    return G.inv[dir] * (f[G.idx(q, in, jn, kn)] - f[G.idx(q, ip, jp, kp)]);
+    return G.inv[dir] * (f(in, jn, kn) - f(ip, jp, kp));
+}
+
+template<class GT>
+__device__ __forceinline__ double deriv(GT const& G, double const* f, int q, int dir, int i, int j,
+                                        int k)
+{
+    return derivFn(G, [&](int a, int b, int cc) { return f[G.idx(q, a, b, cc)]; }, q, dir, i, j, k);
 }
 
 // GridLayout::laplacian (gridlayout.hpp:638-700)
@@ -106,9 +114,9 @@ enum : int
     P2D = 1,
     D2P = 2
 };
-template<class GT>
-__device__ __forceinline__ double project(GT const& G, double const* f, int q, int i, int j, int k,
-                                          int mx, int my, int mz)
+template<class GT, class F>
+__device__ __forceinline__ double projectFn(GT const& G, F const& f, int i, int j, int k, int mx,
+                                            int my, int mz)
 {
     int m[3] = {mx, my, mz};
     int off[3][2];
@@ -137,9 +145,29 @@ __device__ __forceinline__ double project(GT const& G, double const* f, int q, i
     for (int a = 0; a < np[0]; ++a)
         for (int b = 0; b < np[1]; ++b)
             for (int c = 0; c < np[2]; ++c)
-                r += cf[0][a] * cf[1][b] * cf[2][c]
-                     * f[G.idx(q, i + off[0][a], j + off[1][b], k + off[2][c])];
+                r += cf[0][a] * cf[1][b] * cf[2][c] * f(i + off[0][a], j + off[1][b], k + off[2][c]);
     return r;
+}
+
+template<class GT>
+__device__ __forceinline__ double project(GT const& G, double const* f, int q, int i, int j, int k,
+                                          int mx, int my, int mz)
+{
+    return projectFn(G, [&](int a, int b, int cc) { return f[G.idx(q, a, b, cc)]; }, i, j, k, mx,
+                     my, mz);
+}
+
+// Electrons::update at a primal node (electrons.hpp:102-128): component c of
+// Ve = Vi - J(projected dual->primal along c) / n_e, n_e the ion charge density
+template<class GT>
+__device__ __forceinline__ double veAt(GT const& G, CVecPtr J, CVecPtr Vi, double const* Ne, int c,
+                                       int i, int j, int k)
+{
+    int const n     = G.idx(QMOM, i, j, k);
+    double const jc = c == 0   ? project(G, J.c[0], QJX, i, j, k, D2P, PN, PN)
+                      : c == 1 ? project(G, J.c[1], QJY, i, j, k, PN, D2P, PN)
+                               : project(G, J.c[2], QJZ, i, j, k, PN, PN, D2P);
+    return Vi.c[c][n] - jc / Ne[n];
 }
 
 template<class GT>
@@ -276,6 +304,118 @@ __global__ __launch_bounds__(FB) void k_electrons(Geom G0, double Te, CVecPtr J,
 }
 
 // ---- Ohm --------------------------------------------------------------------------------------
+// component c of E at a node of its physical box (Ohm::operator(), ohm.hpp:48-269): ideal
+// -(Ve x B), pressure -grad(Pe)/n, resistive eta J, hyper-resistive (constant or spatial).
+// ve(d, i, j, k) and pe(i, j, k) give the electron fields at primal nodes.
+template<class GT, class VeF, class PeF>
+__device__ __forceinline__ double ohmAt(GT const& G, int c, int i, int j, int k, double eta,
+                                        double nu, int hyper, double lvlCoef, double const* nn,
+                                        VeF const& ve, PeF const& pe, CVecPtr B, CVecPtr J)
+{
+    int const D = G.dim;
+    auto veOf   = [&](int d) { return [&, d](int a, int b, int cc) { return ve(d, a, b, cc); }; };
+    // moments -> E_c: primal->dual along c
+    int const mx = c == 0 ? P2D : PN, my = c == 1 ? P2D : PN, mz = c == 2 ? P2D : PN;
+    double ideal, press = 0., bxE, byE, bzE;
+    if (c == 0)
+    {
+        double const vy = projectFn(G, veOf(1), i, j, k, mx, my, mz);
+        double const vz = projectFn(G, veOf(2), i, j, k, mx, my, mz);
+        byE             = project(G, B.c[1], QBY, i, j, k, PN, PN, D2P); // ByToEx
+        bzE             = project(G, B.c[2], QBZ, i, j, k, PN, D2P, PN); // BzToEx
+        ideal           = -vy * bzE + vz * byE;
+    }
+    else if (c == 1)
+    {
+        double const vx = projectFn(G, veOf(0), i, j, k, mx, my, mz);
+        double const vz = projectFn(G, veOf(2), i, j, k, mx, my, mz);
+        bxE             = project(G, B.c[0], QBX, i, j, k, PN, PN, D2P); // BxToEy
+        bzE             = project(G, B.c[2], QBZ, i, j, k, D2P, PN, PN); // BzToEy
+        ideal           = -vz * bxE + vx * bzE;
+    }
+    else
+    {
+        double const vx = projectFn(G, veOf(0), i, j, k, mx, my, mz);
+        double const vy = projectFn(G, veOf(1), i, j, k, mx, my, mz);
+        bxE             = project(G, B.c[0], QBX, i, j, k, PN, D2P, PN); // BxToEz
+        byE             = project(G, B.c[1], QBY, i, j, k, D2P, PN, PN); // ByToEz
+        ideal           = -vx * byE + vy * bxE;
+    }
+    if (c < D)
+    {
+        double const nOnE = project(G, nn, QMOM, i, j, k, mx, my, mz);
+        double const grad = derivFn(G, pe, QMOM, c, i, j, k);
+        press             = -grad / nOnE;
+    }
+    int const n       = G.idx(QEX + c, i, j, k);
+    double const res  = eta * (0. + 1.0 * J.c[c][n]);
+    double const lapJ = laplacian(G, J.c[c], QJX + c, i, j, k);
+    double hyp;
+    if (hyper == 0)
+        hyp = -nu * lapJ;
+    else
+    {
+        // BxToE_c, ByToE_c, BzToE_c and momentsToE_c (gridlayout_hybrid_yee.hpp:450-745)
+        double bx, by, bz;
+        if (c == 0)
+        {
+            bx = project(G, B.c[0], QBX, i, j, k, P2D, D2P, D2P);
+            by = project(G, B.c[1], QBY, i, j, k, PN, PN, D2P);
+            bz = project(G, B.c[2], QBZ, i, j, k, PN, D2P, PN);
+        }
+        else if (c == 1)
+        {
+            bx = project(G, B.c[0], QBX, i, j, k, PN, PN, D2P);
+            by = project(G, B.c[1], QBY, i, j, k, D2P, P2D, D2P);
+            bz = project(G, B.c[2], QBZ, i, j, k, D2P, PN, PN);
+        }
+        else
+        {
+            bx = project(G, B.c[0], QBX, i, j, k, PN, D2P, PN);
+            by = project(G, B.c[1], QBY, i, j, k, D2P, PN, PN);
+            bz = project(G, B.c[2], QBZ, i, j, k, D2P, D2P, P2D);
+        }
+        double const nOnE = project(G, nn, QMOM, i, j, k, mx, my, mz);
+        double const b    = sqrt(bx * bx + by * by + bz * bz);
+        hyp               = -nu * (b / (nOnE + 0.1) + 1) * lvlCoef * lapJ;
+    }
+    return ideal + press + res + hyp;
+}
+
+// node (i0, j0, k0) of E_c: false outside its array (or, without fill, outside its physical
+// box); else (i, j, k) is where Ohm is evaluated: the node itself, or for a ghost node its
+// periodic source
+template<class GT>
+__device__ __forceinline__ bool ohmNode(GT const& G, int c, int fill, int i0, int j0, int k0, int& i,
+                                        int& j, int& k)
+{
+    int const q = QEX + c;
+    i = i0;
+    j = j0;
+    k = k0;
+    if (fill)
+    {
+        // ghost layers too: a ghost node gets Ohm evaluated at its periodic source
+        if (i0 >= G.shape[q][0] || j0 >= G.shape[q][1] || k0 >= G.shape[q][2])
+            return false;
+        int s[3] = {i0, j0, k0};
+        for (int d = 0; d < G.dim; ++d)
+        {
+            if (s[d] < G.g)
+                s[d] += G.n[d];
+            else if (s[d] > G.physEnd(q, d))
+                s[d] -= G.n[d];
+        }
+        i = s[0];
+        j = s[1];
+        k = s[2];
+        return true;
+    }
+    int lo[3], hi[3];
+    physBox(G, q, lo, hi);
+    return inRange(G, i, j, k, lo, hi);
+}
+
 template<int DIM>
 __global__ __launch_bounds__(FB) void k_ohm(Geom G0, double eta, double nu, int hyper, double lvlCoef,
                                             double const* nn, CVecPtr Ve, double const* Pe, CVecPtr B,
@@ -286,103 +426,61 @@ __global__ __launch_bounds__(FB) void k_ohm(Geom G0, double eta, double nu, int 
     int i0, j0, k0;
     if (!nodeOf(G, QMOM, t, i0, j0, k0))
         return;
-    int const D = G.dim;
     int const c = static_cast<int>(blockIdx.y); // one component per grid row
+    int i, j, k;
+    if (!ohmNode(G, c, fill, i0, j0, k0, i, j, k))
+        return;
+    auto ve = [&](int d, int a, int b, int cc) { return Ve.c[d][G.idx(QMOM, a, b, cc)]; };
+    auto pe = [&](int a, int b, int cc) { return Pe[G.idx(QMOM, a, b, cc)]; };
+    E.c[c][G.idx(QEX + c, i0, j0, k0)] = ohmAt(G, c, i, j, k, eta, nu, hyper, lvlCoef, nn, ve, pe, B, J);
+}
+
+// the electron closure fused into Ohm (periodic level): Ve and Pe are evaluated where Ohm reads
+// them, by the formulas k_electrons applies (Ohm reads Ve only on physical primal nodes, where
+// k_electrons defines it), and the grid row of component 0 also stores them as k_electrons
+// does. AVG: the predictor's average follows, Eavg at every node from Ohm at its periodic
+// source and Bavg from Bpred (the Ohm result itself is stored on the physical nodes only).
+
+template<int DIM, bool AVG>
+__global__ __launch_bounds__(FB) void k_ohm_fused(Geom G0, OhmFused F)
+{
+    GeomD<DIM> const G(G0);
+    long const t = static_cast<long>(blockIdx.x) * FB + threadIdx.x;
+    int i0, j0, k0;
+    if (!nodeOf(G, QMOM, t, i0, j0, k0))
+        return;
+    int const c = static_cast<int>(blockIdx.y);
+    if (c == 0)
     {
-        int const q = QEX + c;
-        int i = i0, j = j0, k = k0;
-        int const dstNode = G.idx(q, i0, j0, k0);
-        if (fill)
-        {
-            // ghost layers too: a ghost node gets Ohm evaluated at its periodic source
-            if (i0 >= G.shape[q][0] || j0 >= G.shape[q][1] || k0 >= G.shape[q][2])
-                return;
-            int s[3] = {i0, j0, k0};
-            for (int d = 0; d < D; ++d)
-            {
-                if (s[d] < G.g)
-                    s[d] += G.n[d];
-                else if (s[d] > G.physEnd(q, d))
-                    s[d] -= G.n[d];
-            }
-            i = s[0];
-            j = s[1];
-            k = s[2];
-        }
-        else
-        {
-            int lo[3], hi[3];
-            physBox(G, q, lo, hi);
-            if (!inRange(G, i, j, k, lo, hi))
-                return;
-        }
-        // moments -> E_c: primal->dual along c
-        int const mx = c == 0 ? P2D : PN, my = c == 1 ? P2D : PN, mz = c == 2 ? P2D : PN;
-        double ideal, press = 0., bxE, byE, bzE;
-        if (c == 0)
-        {
-            double const vy = project(G, Ve.c[1], QMOM, i, j, k, mx, my, mz);
-            double const vz = project(G, Ve.c[2], QMOM, i, j, k, mx, my, mz);
-            byE             = project(G, B.c[1], QBY, i, j, k, PN, PN, D2P); // ByToEx
-            bzE             = project(G, B.c[2], QBZ, i, j, k, PN, D2P, PN); // BzToEx
-            ideal           = -vy * bzE + vz * byE;
-        }
-        else if (c == 1)
-        {
-            double const vx = project(G, Ve.c[0], QMOM, i, j, k, mx, my, mz);
-            double const vz = project(G, Ve.c[2], QMOM, i, j, k, mx, my, mz);
-            bxE             = project(G, B.c[0], QBX, i, j, k, PN, PN, D2P); // BxToEy
-            bzE             = project(G, B.c[2], QBZ, i, j, k, D2P, PN, PN); // BzToEy
-            ideal           = -vz * bxE + vx * bzE;
-        }
-        else
-        {
-            double const vx = project(G, Ve.c[0], QMOM, i, j, k, mx, my, mz);
-            double const vy = project(G, Ve.c[1], QMOM, i, j, k, mx, my, mz);
-            bxE             = project(G, B.c[0], QBX, i, j, k, PN, D2P, PN); // BxToEz
-            byE             = project(G, B.c[1], QBY, i, j, k, D2P, PN, PN); // ByToEz
-            ideal           = -vx * byE + vy * bxE;
-        }
-        if (c < D)
-        {
-            double const nOnE = project(G, nn, QMOM, i, j, k, mx, my, mz);
-            double const grad = deriv(G, Pe, QMOM, c, i, j, k);
-            press             = -grad / nOnE;
-        }
-        int const n       = G.idx(q, i, j, k);
-        double const res  = eta * (0. + 1.0 * J.c[c][n]);
-        double const lapJ = laplacian(G, J.c[c], QJX + c, i, j, k);
-        double hyp;
-        if (hyper == 0)
-            hyp = -nu * lapJ;
-        else
-        {
-            // BxToE_c, ByToE_c, BzToE_c and momentsToE_c (gridlayout_hybrid_yee.hpp:450-745)
-            double bx, by, bz;
-            if (c == 0)
-            {
-                bx = project(G, B.c[0], QBX, i, j, k, P2D, D2P, D2P);
-                by = project(G, B.c[1], QBY, i, j, k, PN, PN, D2P);
-                bz = project(G, B.c[2], QBZ, i, j, k, PN, D2P, PN);
-            }
-            else if (c == 1)
-            {
-                bx = project(G, B.c[0], QBX, i, j, k, PN, PN, D2P);
-                by = project(G, B.c[1], QBY, i, j, k, D2P, P2D, D2P);
-                bz = project(G, B.c[2], QBZ, i, j, k, D2P, PN, PN);
-            }
-            else
-            {
-                bx = project(G, B.c[0], QBX, i, j, k, PN, D2P, PN);
-                by = project(G, B.c[1], QBY, i, j, k, D2P, PN, PN);
-                bz = project(G, B.c[2], QBZ, i, j, k, D2P, D2P, P2D);
-            }
-            double const nOnE = project(G, nn, QMOM, i, j, k, mx, my, mz);
-            double const b    = sqrt(bx * bx + by * by + bz * bz);
-            hyp               = -nu * (b / (nOnE + 0.1) + 1) * lvlCoef * lapJ;
-        }
-        E.c[c][dstNode] = ideal + press + res + hyp;
+        // Electrons::update at this primal node: Pe everywhere, Ve on the physical box
+        int const n = G.idx(QMOM, i0, j0, k0);
+        F.Pe[n]     = F.nn[n] * F.Te;
+        int lo[3], hi[3];
+        physBox(G, QMOM, lo, hi);
+        if (inRange(G, i0, j0, k0, lo, hi))
+            for (int d = 0; d < 3; ++d)
+                F.Ve.c[d][n] = veAt(G, F.J, F.Vi, F.nn, d, i0, j0, k0);
     }
+    if constexpr (AVG)
+    {
+        if (i0 < G.shape[QBX + c][0] && j0 < G.shape[QBX + c][1] && k0 < G.shape[QBX + c][2])
+        {
+            int const n = G.idx(QBX + c, i0, j0, k0);
+            F.Bavg.c[c][n] = (F.Bold.c[c][n] + F.Bnew.c[c][n]) * .5;
+        }
+    }
+    int i, j, k;
+    if (!ohmNode(G, c, AVG ? 1 : F.fill, i0, j0, k0, i, j, k))
+        return;
+    auto ve = [&](int d, int a, int b, int cc) { return veAt(G, F.J, F.Vi, F.nn, d, a, b, cc); };
+    auto pe = [&](int a, int b, int cc) { return F.nn[G.idx(QMOM, a, b, cc)] * F.Te; };
+    double const e = ohmAt(G, c, i, j, k, F.eta, F.nu, F.hyper, F.lvlCoef, F.nn, ve, pe, F.B, F.J);
+    int const dst  = G.idx(QEX + c, i0, j0, k0);
+    bool const own = i == i0 && j == j0 && k == k0;
+    if (!AVG || own)
+        F.E.c[c][dst] = e;
+    if constexpr (AVG)
+        F.Eavg.c[c][dst] = (F.Eold.c[c][G.idx(QEX + c, i, j, k)] + e) * .5;
 }
 
 // ---- elementwise ------------------------------------------------------------------------------
@@ -785,6 +883,105 @@ __global__ __launch_bounds__(FB) void k_fold_totals(Geom G0, MomSet M, double* t
     tvi.c[2][n] = vz / m;
 }
 
+// the five totals (rho, massDensity, Vi) at node c from the folded population moments there
+// (the node's raw deposit plus its overlapping periodic images in lexicographic order, as
+// k_fold_totals sums them); `store` also writes the folded moments and clears the other raw set
+template<class GT>
+__device__ __forceinline__ void foldAt(GT const& G, MomSet const& M, int const* c, bool store,
+                                       double* tot)
+{
+    int const n = G.idx(QMOM, c[0], c[1], c[2]);
+    Images<GT::dim> I;
+    imagesOf(G, c, I);
+    double r = 0., m = 0., vx = 0., vy = 0., vz = 0.;
+    for (int p = 0; p < M.npop; ++p)
+    {
+        double f[5];
+#pragma unroll
+        for (int q = 0; q < 5; ++q)
+        {
+            double const* raw = M.raw[p][q];
+            double v          = raw[n];
+#pragma unroll
+            for (int x = 0; x < Images<GT::dim>::N; ++x)
+                if (I.valid[x])
+                    v += raw[n - I.off[x]];
+            f[q] = v;
+            if (store)
+            {
+                M.mom[p][q][n]  = v;
+                M.zero[p][q][n] = 0.;
+            }
+        }
+        r  = r + f[1];
+        m  = m + f[0] * M.mass[p];
+        vx = vx + f[2] * M.mass[p];
+        vy = vy + f[3] * M.mass[p];
+        vz = vz + f[4] * M.mass[p];
+    }
+    tot[0] = r;
+    tot[1] = m;
+    tot[2] = vx / m;
+    tot[3] = vy / m;
+    tot[4] = vz / m;
+}
+
+// k_fold_totals and k_max5 in one launch: the node's totals are compared with its periodic
+// images' totals, each recomputed from the raw deposits as k_fold_totals computes it
+template<int DIM>
+__global__ __launch_bounds__(FB) void k_fold_max(Geom G0, MomSet M, VecPtr tot3, double* rho,
+                                                 double* md)
+{
+    GeomD<DIM> const G(G0);
+    long const t = static_cast<long>(blockIdx.x) * FB + threadIdx.x;
+    int i, j, k;
+    if (!nodeOf(G, QMOM, t, i, j, k))
+        return;
+    int const c[3] = {i, j, k};
+    int const n    = G.idx(QMOM, i, j, k);
+    double v[5];
+    foldAt(G, M, c, true, v);
+    constexpr int total = DIM == 1 ? 3 : (DIM == 2 ? 9 : 27);
+#pragma unroll
+    for (int im = 0; im < total; ++im)
+    {
+        int kk[3] = {0, 0, 0};
+        int r     = im;
+#pragma unroll
+        for (int d = DIM - 1; d >= 0; --d)
+        {
+            kk[d] = (r % 3) - 1;
+            r /= 3;
+        }
+        if (kk[0] == 0 && kk[1] == 0 && kk[2] == 0)
+            continue;
+        bool in  = true;
+        int s[3] = {i, j, k};
+#pragma unroll
+        for (int d = 0; d < DIM; ++d)
+        {
+            int const end = G.ghostEnd(QMOM, d);
+            int const sh  = kk[d] * G.n[d];
+            int const lo  = sh > 0 ? sh : 0;
+            int const hi  = end + sh < end ? end + sh : end;
+            in            = in && c[d] >= lo && c[d] <= hi;
+            s[d]          = c[d] - sh;
+        }
+        if (!in)
+            continue;
+        double o[5];
+        foldAt(G, M, s, false, o);
+#pragma unroll
+        for (int q = 0; q < 5; ++q)
+            v[q] = v[q] < o[q] ? o[q] : v[q];
+    }
+    rho[n]         = v[0];
+    md[n]          = v[1];
+    tot3.c[0][n] = v[2];
+    tot3.c[1][n] = v[3];
+    tot3.c[2][n] = v[4];
+}
+
 struct Five
 {
     double const* src[5];
@@ -865,6 +1062,20 @@ void launchOhm(Geom const& G, double eta, double nu, int hyper, int level, doubl
         case 3: k_ohm<3><<<dim3(gridFor(G.size(QMOM)), 3), FB, 0, s>>>(G, eta, nu, hyper, lvlCoef, n, Ve, Pe, B, J, E, fill); break;
     }
 }
+void launchOhmFused(Geom const& G, OhmFused const& F, bool average, hipStream_t s)
+{
+    dim3 const grid(gridFor(G.size(QMOM)), 3);
+    switch (G.dim * 2 + (average ? 1 : 0))
+    {
+        case 2: k_ohm_fused<1, false><<<grid, FB, 0, s>>>(G, F); break;
+        case 3: k_ohm_fused<1, true><<<grid, FB, 0, s>>>(G, F); break;
+        case 4: k_ohm_fused<2, false><<<grid, FB, 0, s>>>(G, F); break;
+        case 5: k_ohm_fused<2, true><<<grid, FB, 0, s>>>(G, F); break;
+        case 6: k_ohm_fused<3, false><<<grid, FB, 0, s>>>(G, F); break;
+        case 7: k_ohm_fused<3, true><<<grid, FB, 0, s>>>(G, F); break;
+    }
+}
+
 void launchAverage(double const* a, double const* b, double* out, long count, hipStream_t s)
 {
     k_average<<<gridFor(count), FB, 0, s>>>(a, b, out, count);
@@ -966,6 +1177,29 @@ void launchFoldTotals(Geom const& G, MomSetArgs const& A, double* trho, double* 
         case 3: k_fold_totals<3><<<gridFor(G.size(QMOM)), FB, 0, s>>>(G, M, trho, tmd, tvi); break;
     }
 }
+void launchFoldMax(Geom const& G, MomSetArgs const& A, double* const* dst, hipStream_t s)
+{
+    MomSet M{};
+    M.npop = A.npop;
+    for (int p = 0; p < A.npop; ++p)
+    {
+        M.mass[p] = A.mass[p];
+        for (int q = 0; q < 5; ++q)
+        {
+            M.raw[p][q]  = A.raw[p][q];
+            M.mom[p][q]  = A.mom[p][q];
+            M.zero[p][q] = A.zero[p][q];
+        }
+    }
+    VecPtr vi{{dst[2], dst[3], dst[4]}};
+    switch (G.dim)
+    {
+        case 1: k_fold_max<1><<<gridFor(G.size(QMOM)), FB, 0, s>>>(G, M, vi, dst[0], dst[1]); break;
+        case 2: k_fold_max<2><<<gridFor(G.size(QMOM)), FB, 0, s>>>(G, M, vi, dst[0], dst[1]); break;
+        case 3: k_fold_max<3><<<gridFor(G.size(QMOM)), FB, 0, s>>>(G, M, vi, dst[0], dst[1]); break;
+    }
+}
+
 void launchMax5(Geom const& G, double const* const* src, double* const* dst, hipStream_t s)
 {
     Five F{};
