@@ -484,8 +484,10 @@ phx_status enqueueMoveIonsFast(phx_ctx* c, double dt, int mode)
         }
     }
     PHX_TRY(checkLaunch(c, "push"));
-    double* dst[5] = {c->rho, c->massD, c->Vi[0], c->Vi[1], c->Vi[2]};
-    launchFoldMax(c->G, M, dst, c->stream);
+    launchFoldTotals(c->G, M, c->tot5[0], c->tot5[1], vec(c->tot5 + 2), c->stream);
+    double const* src[5] = {c->tot5[0], c->tot5[1], c->tot5[2], c->tot5[3], c->tot5[4]};
+    double* dst[5]       = {c->rho, c->massD, c->Vi[0], c->Vi[1], c->Vi[2]};
+    launchMax5(c->G, src, dst, c->stream);
     c->rawCur = 1 - cur;
     return checkLaunch(c, "moment fills");
 }
@@ -546,62 +548,15 @@ void enqueueCorrector(phx_ctx* c, double dt)
               c->Pe, cvec(c->em[PHX_EM] + 3), cvec(c->J), vec(c->em[PHX_EM]), c->stream, 1);
 }
 
-// the electron closure + Ohm kernel of a periodic level (Ohm's ghost layers from periodic sources)
-OhmFused ohmFusedArgs(phx_ctx* c, int bset, int eset)
-{
-    OhmFused F{};
-    F.eta = c->cfg.eta;
-    F.nu  = c->cfg.nu;
-    double lvl = 1.;
-    for (int l = 0; l < c->patch.level; ++l)
-        lvl *= 4.;
-    F.lvlCoef = 1. / lvl;
-    F.Te      = c->cfg.Te;
-    F.hyper   = c->cfg.hyper_mode;
-    F.fill    = 1;
-    F.nn      = c->rho;
-    F.Vi      = cvec(c->Vi);
-    F.B       = cvec(c->em[bset] + 3);
-    F.J       = cvec(c->J);
-    F.Ve      = vec(c->Ve);
-    F.Pe      = c->Pe;
-    F.E       = vec(c->em[eset]);
-    return F;
-}
-
-// predictor + average of a periodic level: Faraday, Ampere, then one kernel for the electron
-// closure, Ohm (Epred on the physical nodes) and the average with its E ghost fill
-void enqueuePredictorAverageFast(phx_ctx* c, double dt, int eset)
-{
-    launchFaradayFill(c->G, cvec(c->em[PHX_EM] + 3), cvec(c->em[eset]), vec(c->em[PHX_EM_PRED] + 3), dt,
-                      c->stream);
-    launchAmpereFill(c->G, cvec(c->em[PHX_EM_PRED] + 3), vec(c->J), c->stream);
-    OhmFused F = ohmFusedArgs(c, PHX_EM_PRED, PHX_EM_PRED);
-    F.Eold     = cvec(c->em[PHX_EM]);
-    F.Bold     = cvec(c->em[PHX_EM] + 3);
-    F.Bnew     = cvec(c->em[PHX_EM_PRED] + 3);
-    F.Eavg     = vec(c->em[PHX_EM_AVG]);
-    F.Bavg     = vec(c->em[PHX_EM_AVG] + 3);
-    launchOhmFused(c->G, F, true, c->stream);
-}
-
-// corrector of a periodic level: in-place Faraday and its B fill, Ampere, electron closure + Ohm
-void enqueueCorrectorFast(phx_ctx* c, double dt)
-{
-    launchFaraday(c->G, cvec(c->em[PHX_EM] + 3), cvec(c->em[PHX_EM_AVG]), vec(c->em[PHX_EM] + 3), dt,
-                  c->stream);
-    launchFill3(c->G, QBX, vec(c->em[PHX_EM] + 3), c->stream);
-    launchAmpereFill(c->G, cvec(c->em[PHX_EM] + 3), vec(c->J), c->stream);
-    launchOhmFused(c->G, ohmFusedArgs(c, PHX_EM, PHX_EM), false, c->stream);
-}
-
 phx_status enqueueAdvanceFast(phx_ctx* c, double dt)
 {
-    enqueuePredictorAverageFast(c, dt, PHX_EM);
+    enqueuePredictor(c, dt, PHX_EM);
+    enqueueAverage(c);
     PHX_TRY(enqueueMoveIonsFast(c, dt, PHX_MODE_DOMAIN_ONLY));
-    enqueuePredictorAverageFast(c, dt, PHX_EM_AVG);
+    enqueuePredictor(c, dt, PHX_EM_AVG);
+    enqueueAverage(c);
     PHX_TRY(enqueueMoveIonsFast(c, dt, PHX_MODE_ALL));
-    enqueueCorrectorFast(c, dt);
+    enqueueCorrector(c, dt);
     return checkLaunch(c, "advance");
 }
 

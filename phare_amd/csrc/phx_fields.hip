@@ -157,19 +157,6 @@ __device__ __forceinline__ double project(GT const& G, double const* f, int q, i
                      my, mz);
 }
 
-// Electrons::update at a primal node (electrons.hpp:102-128): component c of
-// Ve = Vi - J(projected dual->primal along c) / n_e, n_e the ion charge density
-template<class GT>
-__device__ __forceinline__ double veAt(GT const& G, CVecPtr J, CVecPtr Vi, double const* Ne, int c,
-                                       int i, int j, int k)
-{
-    int const n     = G.idx(QMOM, i, j, k);
-    double const jc = c == 0   ? project(G, J.c[0], QJX, i, j, k, D2P, PN, PN)
-                      : c == 1 ? project(G, J.c[1], QJY, i, j, k, PN, D2P, PN)
-                               : project(G, J.c[2], QJZ, i, j, k, PN, PN, D2P);
-    return Vi.c[c][n] - jc / Ne[n];
-}
-
 template<class GT>
 __device__ __forceinline__ void physBox(GT const& G, int q, int* lo, int* hi)
 {
@@ -288,19 +275,25 @@ __global__ __launch_bounds__(FB) void k_electrons(Geom G0, double Te, CVecPtr J,
     int i, j, k;
     if (!nodeOf(G, QMOM, t, i, j, k))
         return;
-    int const n = G.idx(QMOM, i, j, k);
-    Pe[n]       = Ne[n] * Te;
+    // every load before the first store (a store that may alias a later load would serialize them)
+    int const n     = G.idx(QMOM, i, j, k);
+    double const ne = Ne[n];
     int lo[3], hi[3];
     physBox(G, QMOM, lo, hi);
     if (!inRange(G, i, j, k, lo, hi))
+    {
+        Pe[n] = ne * Te;
         return;
+    }
     // JxToMoments / JyToMoments / JzToMoments: dual->primal along the component's direction
     double const jx = project(G, J.c[0], QJX, i, j, k, D2P, PN, PN);
     double const jy = project(G, J.c[1], QJY, i, j, k, PN, D2P, PN);
     double const jz = project(G, J.c[2], QJZ, i, j, k, PN, PN, D2P);
-    Ve.c[0][n]      = Vi.c[0][n] - jx / Ne[n];
-    Ve.c[1][n]      = Vi.c[1][n] - jy / Ne[n];
-    Ve.c[2][n]      = Vi.c[2][n] - jz / Ne[n];
+    double const vx = Vi.c[0][n], vy = Vi.c[1][n], vz = Vi.c[2][n];
+    Pe[n]           = ne * Te;
+    Ve.c[0][n]      = vx - jx / ne;
+    Ve.c[1][n]      = vy - jy / ne;
+    Ve.c[2][n]      = vz - jz / ne;
 }
 
 // ---- Ohm --------------------------------------------------------------------------------------
@@ -433,54 +426,6 @@ __global__ __launch_bounds__(FB) void k_ohm(Geom G0, double eta, double nu, int 
     auto ve = [&](int d, int a, int b, int cc) { return Ve.c[d][G.idx(QMOM, a, b, cc)]; };
     auto pe = [&](int a, int b, int cc) { return Pe[G.idx(QMOM, a, b, cc)]; };
     E.c[c][G.idx(QEX + c, i0, j0, k0)] = ohmAt(G, c, i, j, k, eta, nu, hyper, lvlCoef, nn, ve, pe, B, J);
-}
-
-// the electron closure fused into Ohm (periodic level): Ve and Pe are evaluated where Ohm reads
-// them, by the formulas k_electrons applies (Ohm reads Ve only on physical primal nodes, where
-// k_electrons defines it), and the grid row of component 0 also stores them as k_electrons
-// does. AVG: the predictor's average follows, Eavg at every node from Ohm at its periodic
-// source and Bavg from Bpred (the Ohm result itself is stored on the physical nodes only).
-
-template<int DIM, bool AVG>
-__global__ __launch_bounds__(FB) void k_ohm_fused(Geom G0, OhmFused F)
-{
-    GeomD<DIM> const G(G0);
-    long const t = static_cast<long>(blockIdx.x) * FB + threadIdx.x;
-    int i0, j0, k0;
-    if (!nodeOf(G, QMOM, t, i0, j0, k0))
-        return;
-    int const c = static_cast<int>(blockIdx.y);
-    if (c == 0)
-    {
-        // Electrons::update at this primal node: Pe everywhere, Ve on the physical box
-        int const n = G.idx(QMOM, i0, j0, k0);
-        F.Pe[n]     = F.nn[n] * F.Te;
-        int lo[3], hi[3];
-        physBox(G, QMOM, lo, hi);
-        if (inRange(G, i0, j0, k0, lo, hi))
-            for (int d = 0; d < 3; ++d)
-                F.Ve.c[d][n] = veAt(G, F.J, F.Vi, F.nn, d, i0, j0, k0);
-    }
-    if constexpr (AVG)
-    {
-        if (i0 < G.shape[QBX + c][0] && j0 < G.shape[QBX + c][1] && k0 < G.shape[QBX + c][2])
-        {
-            int const n = G.idx(QBX + c, i0, j0, k0);
-            F.Bavg.c[c][n] = (F.Bold.c[c][n] + F.Bnew.c[c][n]) * .5;
-        }
-    }
-    int i, j, k;
-    if (!ohmNode(G, c, AVG ? 1 : F.fill, i0, j0, k0, i, j, k))
-        return;
-    auto ve = [&](int d, int a, int b, int cc) { return veAt(G, F.J, F.Vi, F.nn, d, a, b, cc); };
-    auto pe = [&](int a, int b, int cc) { return F.nn[G.idx(QMOM, a, b, cc)] * F.Te; };
-    double const e = ohmAt(G, c, i, j, k, F.eta, F.nu, F.hyper, F.lvlCoef, F.nn, ve, pe, F.B, F.J);
-    int const dst  = G.idx(QEX + c, i0, j0, k0);
-    bool const own = i == i0 && j == j0 && k == k0;
-    if (!AVG || own)
-        F.E.c[c][dst] = e;
-    if constexpr (AVG)
-        F.Eavg.c[c][dst] = (F.Eold.c[c][G.idx(QEX + c, i, j, k)] + e) * .5;
 }
 
 // ---- elementwise ------------------------------------------------------------------------------
@@ -838,7 +783,10 @@ __device__ __forceinline__ void imagesOf(GT const& G, int const* c, Images<GT::d
 }
 
 // fillFluxBorders + fillDensityBorders (border sum of every population moment) + updateIons
-// (totals into tmp) ; the max fill of the totals follows in k_max5
+// (totals into tmp) ; the max fill of the totals follows in k_max5. A node's sources are itself,
+// then its overlapping periodic images in lexicographic order. Loads are unconditional (an
+// image that does not overlap reads the node itself and is not added) and precede the
+// population's stores, so that they are in flight together.
 template<int DIM>
 __global__ __launch_bounds__(FB) void k_fold_totals(Geom G0, MomSet M, double* trho, double* tmd,
                                                     VecPtr tvi)
@@ -850,9 +798,12 @@ __global__ __launch_bounds__(FB) void k_fold_totals(Geom G0, MomSet M, double* t
         return;
     int const c[3] = {i, j, k};
     int const n    = G.idx(QMOM, i, j, k);
-    // sources of this node: itself, then the overlapping periodic images in lexicographic order
     Images<DIM> I;
     imagesOf(G, c, I);
+    int src[Images<DIM>::N];
+#pragma unroll
+    for (int x = 0; x < Images<DIM>::N; ++x)
+        src[x] = I.valid[x] ? n - I.off[x] : n;
     double r = 0., m = 0., vx = 0., vy = 0., vz = 0.;
     for (int p = 0; p < M.npop; ++p)
     {
@@ -864,10 +815,16 @@ __global__ __launch_bounds__(FB) void k_fold_totals(Geom G0, MomSet M, double* t
             double v          = raw[n];
 #pragma unroll
             for (int x = 0; x < Images<DIM>::N; ++x)
-                if (I.valid[x])
-                    v += raw[n - I.off[x]];
-            f[q]            = v;
-            M.mom[p][q][n]  = v;
+            {
+                double const o = raw[src[x]];
+                v              = I.valid[x] ? v + o : v;
+            }
+            f[q] = v;
+        }
+#pragma unroll
+        for (int q = 0; q < 5; ++q)
+        {
+            M.mom[p][q][n]  = f[q];
             M.zero[p][q][n] = 0.;
         }
         r  = r + f[1];
@@ -881,105 +838,6 @@ __global__ __launch_bounds__(FB) void k_fold_totals(Geom G0, MomSet M, double* t
     tvi.c[0][n] = vx / m;
     tvi.c[1][n] = vy / m;
     tvi.c[2][n] = vz / m;
-}
-
-// the five totals (rho, massDensity, Vi) at node c from the folded population moments there
-// (the node's raw deposit plus its overlapping periodic images in lexicographic order, as
-// k_fold_totals sums them); `store` also writes the folded moments and clears the other raw set
-template<class GT>
-__device__ __forceinline__ void foldAt(GT const& G, MomSet const& M, int const* c, bool store,
-                                       double* tot)
-{
-    int const n = G.idx(QMOM, c[0], c[1], c[2]);
-    Images<GT::dim> I;
-    imagesOf(G, c, I);
-    double r = 0., m = 0., vx = 0., vy = 0., vz = 0.;
-    for (int p = 0; p < M.npop; ++p)
-    {
-        double f[5];
-#pragma unroll
-        for (int q = 0; q < 5; ++q)
-        {
-            double const* raw = M.raw[p][q];
-            double v          = raw[n];
-#pragma unroll
-            for (int x = 0; x < Images<GT::dim>::N; ++x)
-                if (I.valid[x])
-                    v += raw[n - I.off[x]];
-            f[q] = v;
-            if (store)
-            {
-                M.mom[p][q][n]  = v;
-                M.zero[p][q][n] = 0.;
-            }
-        }
-        r  = r + f[1];
-        m  = m + f[0] * M.mass[p];
-        vx = vx + f[2] * M.mass[p];
-        vy = vy + f[3] * M.mass[p];
-        vz = vz + f[4] * M.mass[p];
-    }
-    tot[0] = r;
-    tot[1] = m;
-    tot[2] = vx / m;
-    tot[3] = vy / m;
-    tot[4] = vz / m;
-}
-
-// k_fold_totals and k_max5 in one launch: the node's totals are compared with its periodic
-// images' totals, each recomputed from the raw deposits as k_fold_totals computes it
-template<int DIM>
-__global__ __launch_bounds__(FB) void k_fold_max(Geom G0, MomSet M, VecPtr tot3, double* rho,
-                                                 double* md)
-{
-    GeomD<DIM> const G(G0);
-    long const t = static_cast<long>(blockIdx.x) * FB + threadIdx.x;
-    int i, j, k;
-    if (!nodeOf(G, QMOM, t, i, j, k))
-        return;
-    int const c[3] = {i, j, k};
-    int const n    = G.idx(QMOM, i, j, k);
-    double v[5];
-    foldAt(G, M, c, true, v);
-    constexpr int total = DIM == 1 ? 3 : (DIM == 2 ? 9 : 27);
-#pragma unroll
-    for (int im = 0; im < total; ++im)
-    {
-        int kk[3] = {0, 0, 0};
-        int r     = im;
-#pragma unroll
-        for (int d = DIM - 1; d >= 0; --d)
-        {
-            kk[d] = (r % 3) - 1;
-            r /= 3;
-        }
-        if (kk[0] == 0 && kk[1] == 0 && kk[2] == 0)
-            continue;
-        bool in  = true;
-        int s[3] = {i, j, k};
-#pragma unroll
-        for (int d = 0; d < DIM; ++d)
-        {
-            int const end = G.ghostEnd(QMOM, d);
-            int const sh  = kk[d] * G.n[d];
-            int const lo  = sh > 0 ? sh : 0;
-            int const hi  = end + sh < end ? end + sh : end;
-            in            = in && c[d] >= lo && c[d] <= hi;
-            s[d]          = c[d] - sh;
-        }
-        if (!in)
-            continue;
-        double o[5];
-        foldAt(G, M, s, false, o);
-#pragma unroll
-        for (int q = 0; q < 5; ++q)
-            v[q] = v[q] < o[q] ? o[q] : v[q];
-    }
-    rho[n]         = v[0];
-    md[n]          = v[1];
-    tot3.c[0][n] = v[2];
-    tot3.c[1][n] = v[3];
-    tot3.c[2][n] = v[4];
 }
 
 struct Five
@@ -1001,19 +859,22 @@ __global__ __launch_bounds__(FB) void k_max5(Geom G0, Five F)
     int const n    = G.idx(QMOM, i, j, k);
     Images<DIM> I;
     imagesOf(G, c, I);
+    // unconditional loads (a non-overlapping image reads the node itself), stores after all loads
+    double v[5];
 #pragma unroll
     for (int q = 0; q < 5; ++q)
     {
-        double v = F.src[q][n];
+        v[q] = F.src[q][n];
 #pragma unroll
         for (int x = 0; x < Images<DIM>::N; ++x)
-            if (I.valid[x])
-            {
-                double const o = F.src[q][n - I.off[x]];
-                v              = v < o ? o : v;
-            }
-        F.dst[q][n] = v;
+        {
+            double const o = F.src[q][I.valid[x] ? n - I.off[x] : n];
+            v[q]           = I.valid[x] && v[q] < o ? o : v[q];
+        }
     }
+#pragma unroll
+    for (int q = 0; q < 5; ++q)
+        F.dst[q][n] = v[q];
 }
 
 unsigned int gridFor(long n) { return static_cast<unsigned int>((n + FB - 1) / FB); }
@@ -1062,20 +923,6 @@ void launchOhm(Geom const& G, double eta, double nu, int hyper, int level, doubl
         case 3: k_ohm<3><<<dim3(gridFor(G.size(QMOM)), 3), FB, 0, s>>>(G, eta, nu, hyper, lvlCoef, n, Ve, Pe, B, J, E, fill); break;
     }
 }
-void launchOhmFused(Geom const& G, OhmFused const& F, bool average, hipStream_t s)
-{
-    dim3 const grid(gridFor(G.size(QMOM)), 3);
-    switch (G.dim * 2 + (average ? 1 : 0))
-    {
-        case 2: k_ohm_fused<1, false><<<grid, FB, 0, s>>>(G, F); break;
-        case 3: k_ohm_fused<1, true><<<grid, FB, 0, s>>>(G, F); break;
-        case 4: k_ohm_fused<2, false><<<grid, FB, 0, s>>>(G, F); break;
-        case 5: k_ohm_fused<2, true><<<grid, FB, 0, s>>>(G, F); break;
-        case 6: k_ohm_fused<3, false><<<grid, FB, 0, s>>>(G, F); break;
-        case 7: k_ohm_fused<3, true><<<grid, FB, 0, s>>>(G, F); break;
-    }
-}
-
 void launchAverage(double const* a, double const* b, double* out, long count, hipStream_t s)
 {
     k_average<<<gridFor(count), FB, 0, s>>>(a, b, out, count);
@@ -1175,28 +1022,6 @@ void launchFoldTotals(Geom const& G, MomSetArgs const& A, double* trho, double* 
         case 1: k_fold_totals<1><<<gridFor(G.size(QMOM)), FB, 0, s>>>(G, M, trho, tmd, tvi); break;
         case 2: k_fold_totals<2><<<gridFor(G.size(QMOM)), FB, 0, s>>>(G, M, trho, tmd, tvi); break;
         case 3: k_fold_totals<3><<<gridFor(G.size(QMOM)), FB, 0, s>>>(G, M, trho, tmd, tvi); break;
-    }
-}
-void launchFoldMax(Geom const& G, MomSetArgs const& A, double* const* dst, hipStream_t s)
-{
-    MomSet M{};
-    M.npop = A.npop;
-    for (int p = 0; p < A.npop; ++p)
-    {
-        M.mass[p] = A.mass[p];
-        for (int q = 0; q < 5; ++q)
-        {
-            M.raw[p][q]  = A.raw[p][q];
-            M.mom[p][q]  = A.mom[p][q];
-            M.zero[p][q] = A.zero[p][q];
-        }
-    }
-    VecPtr vi{{dst[2], dst[3], dst[4]}};
-    switch (G.dim)
-    {
-        case 1: k_fold_max<1><<<gridFor(G.size(QMOM)), FB, 0, s>>>(G, M, vi, dst[0], dst[1]); break;
-        case 2: k_fold_max<2><<<gridFor(G.size(QMOM)), FB, 0, s>>>(G, M, vi, dst[0], dst[1]); break;
-        case 3: k_fold_max<3><<<gridFor(G.size(QMOM)), FB, 0, s>>>(G, M, vi, dst[0], dst[1]); break;
     }
 }
 

@@ -193,24 +193,5 @@ void launchAverageFill(Geom const& G, CVecPtr E, CVecPtr Ep, CVecPtr B, CVecPtr 
 void launchFoldTotals(Geom const& G, MomSetArgs const& A, double* trho, double* tmd, VecPtr tvi,
                       hipStream_t s);
 void launchMax5(Geom const& G, double const* const* src, double* const* dst, hipStream_t s);
-// the fold kernel and the border max in one: every node also folds its periodic images' moments
-// and takes the max of their totals
-void launchFoldMax(Geom const& G, MomSetArgs const& A, double* const* dst, hipStream_t s);
-
-// arguments of the fused electron closure + Ohm (+ predictor average) kernel
-struct OhmFused
-{
-    double eta, nu, lvlCoef, Te;
-    int hyper, fill;
-    double const* nn; // ion charge density (n_e)
-    CVecPtr Vi, B, J;
-    VecPtr Ve;
-    double* Pe;
-    VecPtr E;              // Ohm's result
-    CVecPtr Eold, Bold;    // AVG: the state E, B
-    CVecPtr Bnew;          // AVG: Bpred
-    VecPtr Eavg, Bavg;     // AVG: averaged set
-};
-void launchOhmFused(Geom const& G, OhmFused const& F, bool average, hipStream_t s);
 
 } // namespace phx
