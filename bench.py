@@ -83,21 +83,58 @@ def workload(name):
     raise SystemExit(f"unknown workload {name}")
 
 
-def push_bytes(wl, mode):
-    """algorithmic HBM bytes of one fused push+deposit launch (FP64 SoA record R = 12D+40 B per
-    particle read; mode all also writes the mutable part M = 12D+24 B; plus 6 E/B component
-    reads and 5 moment writes per cell)"""
+def record_bytes(wl, precision):
+    """(R, M): bytes of a particle record read, and of its mutable part written back. FP64 SoA:
+    iCell i32 x D, delta f64 x D, v f64 x 3, weight and charge f64; mixed precision: iCell i32 x D,
+    delta f32 x D, v f32 x 3, weight f32 (the charge is the population's)"""
     D = wl.dim
-    R, M = 12 * D + 40, 12 * D + 24
+    if precision == "fp32":
+        return 8 * D + 16, 8 * D + 12
+    return 12 * D + 40, 12 * D + 24
+
+
+def push_bytes(wl, mode, precision="fp64"):
+    """algorithmic HBM bytes of one fused push+deposit launch: the record read, in mode all the
+    write-back of its mutable part, plus 6 E/B component reads and 5 moment writes per cell"""
+    R, M = record_bytes(wl, precision)
     per_particle = R + (M if mode == phx.MODE_ALL else 0)
     return wl.n_particles * per_particle + wl.n_cells * 11 * 8
 
 
-def step_bytes(wl):
+def step_bytes(wl, precision="fp64"):
     """algorithmic bytes of one PPC step: N_p (2R+M) + N_c * 8 * (189 + 20 npop)"""
-    D = wl.dim
-    R, M = 12 * D + 40, 12 * D + 24
+    R, M = record_bytes(wl, precision)
     return wl.n_particles * (2 * R + M) + wl.n_cells * 8 * (189 + 20 * len(wl.masses))
+
+
+def run_family(wl, precision, steps, warmup, reps):
+    """steps/s of one precision family on the workload, and its push kernel's time"""
+    prec = phx.PRECISION_FP32 if precision == "fp32" else phx.PRECISION_FP64
+    lvl = phx.Level(wl.dim, wl.order, wl.ncells, wl.dx, masses=wl.masses, Te=wl.Te,
+                    capacity=wl.n_particles, precision=prec)
+    wl.setup(lvl, phx.F_B)
+    diag0 = lvl.diagnostics()
+    lvl.advance(wl.dt, warmup)
+    lvl.sync()
+    device_sync()
+    t0 = time.perf_counter()
+    lvl.advance(wl.dt, steps)
+    lvl.sync()
+    device_sync()
+    elapsed = time.perf_counter() - t0
+    diag1 = lvl.diagnostics()
+    # dominant kernel: the fused push + gather + deposit, captured alone in a graph and replayed
+    # back to back between two HIP events on the level's stream
+    ms_all = lvl.profile_push(wl.dt, phx.MODE_ALL, reps)
+    ms_dom = lvl.profile_push(wl.dt, phx.MODE_DOMAIN_ONLY, reps)
+    lvl.close()
+    e0 = diag0["kinetic"] + diag0["magnetic"]
+    e1 = diag1["kinetic"] + diag1["magnetic"]
+    return dict(value=wl.n_particles * steps / elapsed, ms_step=1e3 * elapsed / steps,
+                ms_all=ms_all, ms_dom=ms_dom,
+                energy_drift=(e1 - e0) / e0,
+                charge_drift=(diag1["charge"] - diag0["charge"]) / diag0["charge"],
+                steps_done=warmup + steps)
 
 
 def cpu_baseline(wl, steps):
@@ -126,6 +163,7 @@ def main():
     ap.add_argument("--workload", default="harris2d")
     ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision family")
     ap.add_argument("--kernel-reps", type=int, default=20)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES (counter calibration)")
@@ -139,42 +177,17 @@ def main():
         sys.exit(2)
 
     wl = workload(args.workload)
-    lvl = phx.Level(wl.dim, wl.order, wl.ncells, wl.dx, masses=wl.masses, Te=wl.Te,
-                    capacity=wl.n_particles)
-    wl.setup(lvl, phx.F_B)
-    diag0 = lvl.diagnostics()
-
-    lvl.advance(wl.dt, args.warmup)
-    lvl.sync()
-    device_sync()
-    t0 = time.perf_counter()
-    lvl.advance(wl.dt, args.steps)
-    lvl.sync()
-    device_sync()
-    elapsed = time.perf_counter() - t0
-    diag1 = lvl.diagnostics()
-
-    ms_step = 1e3 * elapsed / args.steps
-    value = wl.n_particles * args.steps / elapsed
-
-    # dominant kernel: the fused push + gather + deposit, captured alone in a graph and replayed
-    # back to back between two HIP events on the level's stream
-    ms_all = lvl.profile_push(wl.dt, phx.MODE_ALL, args.kernel_reps)
-    ms_dom = lvl.profile_push(wl.dt, phx.MODE_DOMAIN_ONLY, args.kernel_reps)
+    f64 = run_family(wl, "fp64", args.steps, args.warmup, args.kernel_reps)
     bytes_all = push_bytes(wl, phx.MODE_ALL)
-    achieved = bytes_all / (ms_all * 1e-3) / 1e9
-    lvl.close()
-
-    e0 = diag0["kinetic"] + diag0["magnetic"]
-    e1 = diag1["kinetic"] + diag1["magnetic"]
+    achieved = bytes_all / (f64["ms_all"] * 1e-3) / 1e9
     out = {
         "metric": "particle-steps/s (push+gather+deposit)",
-        "value": value,
+        "value": f64["value"],
         "unit": "particle-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_step,
+        "ms_per_step": f64["ms_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -189,12 +202,32 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": MEASURED_TRAFFIC.get((args.workload, "fp64")),
                      "kernel": "k_push_seg (fused prePush+gather+Boris+postPush+deposit on cell-segmented storage, mode all)",
-                     "kernel_ms": ms_all, "kernel_ms_domain_only": ms_dom,
+                     "kernel_ms": f64["ms_all"], "kernel_ms_domain_only": f64["ms_dom"],
                      "algorithmic_bytes_per_launch": bytes_all},
-        "step_hbm_fraction": step_bytes(wl) / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
-        "energy_drift": (e1 - e0) / e0,
-        "charge_drift": (diag1["charge"] - diag0["charge"]) / diag0["charge"],
+        "step_hbm_fraction": step_bytes(wl) / (f64["ms_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "energy_drift": f64["energy_drift"],
+        "charge_drift": f64["charge_drift"],
     }
+    if not args.no_mixed:
+        # the mixed-precision family on the same workload (FP32 particles, FP64 fields)
+        f32 = run_family(wl, "fp32", args.steps, args.warmup, args.kernel_reps)
+        b32 = push_bytes(wl, phx.MODE_ALL, "fp32")
+        a32 = b32 / (f32["ms_all"] * 1e-3) / 1e9
+        out["mixed_fp32"] = {
+            "value": f32["value"], "unit": "particle-steps/s", "ms_per_step": f32["ms_step"],
+            "speedup_vs_fp64": f32["value"] / f64["value"],
+            "roofline": {"bound": "hbm", "achieved": a32, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": a32 / HBM_PEAK_GBS,
+                         "traffic": MEASURED_TRAFFIC.get((args.workload, "fp32")),
+                         "kernel": "k_push_seg32 (mixed precision, mode all)",
+                         "kernel_ms": f32["ms_all"], "kernel_ms_domain_only": f32["ms_dom"],
+                         "algorithmic_bytes_per_launch": b32},
+            "step_hbm_fraction": step_bytes(wl, "fp32") / (f32["ms_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "energy_drift": f32["energy_drift"],
+            "energy_drift_minus_fp64": f32["energy_drift"] - f64["energy_drift"],
+            "charge_drift": f32["charge_drift"],
+            "drift_steps": f32["steps_done"],
+        }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(wl, args.cpu_steps)
     print(json.dumps(out), flush=True)

@@ -41,34 +41,42 @@ using namespace phx;
 namespace
 {
 
+// particle arrays of one kind: int32 cells, and delta, velocity, weight (and charge) as FP64
+// (real = 8) or, in the mixed-precision family, FP32 without a charge array (real = 4)
 struct Store
 {
     size_t n = 0, cap = 0;
     bool seg = false; // cell-segmented by the sort (the population's segment tables describe it)
+    int real = 8;
     int32_t* ic[3] = {nullptr, nullptr, nullptr};
-    double* de[3]  = {nullptr, nullptr, nullptr};
-    double* v[3]   = {nullptr, nullptr, nullptr};
-    double* w      = nullptr;
-    double* q      = nullptr;
+    void* de[3]    = {nullptr, nullptr, nullptr};
+    void* v[3]     = {nullptr, nullptr, nullptr};
+    void* w        = nullptr;
+    void* q        = nullptr;
 
-    PView view() const
+    template<class R>
+    PViewT<R> viewT() const
     {
-        PView p;
+        PViewT<R> p;
         for (int d = 0; d < 3; ++d)
         {
             p.ic[d] = ic[d];
-            p.de[d] = de[d];
-            p.v[d]  = v[d];
+            p.de[d] = static_cast<R*>(de[d]);
+            p.v[d]  = static_cast<R*>(v[d]);
         }
-        p.w = w;
-        p.q = q;
+        p.w = static_cast<R*>(w);
+        p.q = static_cast<R*>(q);
         return p;
     }
+    PView view() const { return viewT<double>(); }
+    PView32 view32() const { return viewT<float>(); }
 };
 
 struct Pop
 {
     double mass = 1.;
+    double charge = 0.; // the particles' charge (mixed-precision family: one per population)
+    bool hasCharge = false;
     Store arr[3]; // domain, patch ghost, level ghost
     double* mom[5] = {};
     SegTables seg{}; // tables of the segmented domain array (items sized for itemsAlloc items)
@@ -175,17 +183,22 @@ void devFree(phx_ctx* c, void* p)
     (void)hipFree(p);
 }
 
+bool mixed(phx_ctx const* c) { return c->cfg.precision == PHX_PRECISION_FP32; }
+
 phx_status storeAlloc(phx_ctx* c, Store& s, size_t cap)
 {
     int const D = c->cfg.dim;
+    s.real      = mixed(c) ? 4 : 8;
+    size_t const r = static_cast<size_t>(s.real);
     for (int d = 0; d < 3; ++d)
     {
         PHX_TRY(devAlloc(c, (void**)&s.ic[d], d < D ? cap * 4 : 8));
-        PHX_TRY(devAlloc(c, (void**)&s.de[d], d < D ? cap * 8 : 8));
-        PHX_TRY(devAlloc(c, (void**)&s.v[d], cap * 8));
+        PHX_TRY(devAlloc(c, &s.de[d], d < D ? cap * r : 8));
+        PHX_TRY(devAlloc(c, &s.v[d], cap * r));
     }
-    PHX_TRY(devAlloc(c, (void**)&s.w, cap * 8));
-    PHX_TRY(devAlloc(c, (void**)&s.q, cap * 8));
+    PHX_TRY(devAlloc(c, &s.w, cap * r));
+    if (s.real == 8)
+        PHX_TRY(devAlloc(c, &s.q, cap * r));
     s.cap = cap;
     return PHX_OK;
 }
@@ -215,18 +228,20 @@ phx_status storeReserve(phx_ctx* c, Store& s, size_t need)
     size_t cap = std::max(need, s.cap + s.cap / 2 + 1024);
     Store ns;
     PHX_TRY(storeAlloc(c, ns, cap));
-    int const D = c->cfg.dim;
+    int const D    = c->cfg.dim;
+    size_t const r = static_cast<size_t>(s.real);
     if (s.n)
     {
         for (int d = 0; d < D; ++d)
         {
             PHX_HIP(c, hipMemcpyAsync(ns.ic[d], s.ic[d], s.n * 4, hipMemcpyDeviceToDevice, c->stream));
-            PHX_HIP(c, hipMemcpyAsync(ns.de[d], s.de[d], s.n * 8, hipMemcpyDeviceToDevice, c->stream));
+            PHX_HIP(c, hipMemcpyAsync(ns.de[d], s.de[d], s.n * r, hipMemcpyDeviceToDevice, c->stream));
         }
         for (int d = 0; d < 3; ++d)
-            PHX_HIP(c, hipMemcpyAsync(ns.v[d], s.v[d], s.n * 8, hipMemcpyDeviceToDevice, c->stream));
-        PHX_HIP(c, hipMemcpyAsync(ns.w, s.w, s.n * 8, hipMemcpyDeviceToDevice, c->stream));
-        PHX_HIP(c, hipMemcpyAsync(ns.q, s.q, s.n * 8, hipMemcpyDeviceToDevice, c->stream));
+            PHX_HIP(c, hipMemcpyAsync(ns.v[d], s.v[d], s.n * r, hipMemcpyDeviceToDevice, c->stream));
+        PHX_HIP(c, hipMemcpyAsync(ns.w, s.w, s.n * r, hipMemcpyDeviceToDevice, c->stream));
+        if (s.q)
+            PHX_HIP(c, hipMemcpyAsync(ns.q, s.q, s.n * r, hipMemcpyDeviceToDevice, c->stream));
         PHX_HIP(c, hipStreamSynchronize(c->stream));
     }
     ns.n = s.n;
@@ -358,6 +373,7 @@ PushArgs pushArgs(phx_ctx* c, int pop, int kind, int mode, int emSet, double dt,
     A.status       = nullptr;
     A.dropped      = c->dropped;
     A.pushOn       = 1;
+    A.charge       = c->pops[pop].charge;
     // tile shape from the density of particles per cell of this array
     double ppc = static_cast<double>(A.n);
     for (int d = 0; d < c->cfg.dim; ++d)
@@ -373,7 +389,8 @@ PushArgs pushArgs(phx_ctx* c, int pop, int kind, int mode, int emSet, double dt,
         A.sg          = c->sg;
         A.st          = P.seg;
         A.st.itemsCap = segItemsCap(c->sg, s.cap);
-        A.tile        = segTileShape(c->cfg.dim, c->cfg.interp_order, c->sg, PHX_SEG_COPIES);
+        A.tile        = mixed(c) ? segTileShape32(c->cfg.dim, c->cfg.interp_order, c->sg)
+                                 : segTileShape(c->cfg.dim, c->cfg.interp_order, c->sg, PHX_SEG_COPIES);
     }
     return A;
 }
@@ -473,8 +490,15 @@ phx_status enqueueMoveIonsFast(phx_ctx* c, double dt, int mode)
         for (int m = 0; m < 5; ++m)
             A.mom[m] = c->raw[cur][p][m];
         A.periodicWrap = 1;
-        PView v        = c->pops[p].arr[PHX_DOMAIN].view();
-        launchPush(A, v, v, c->stream);
+        Store const& dom = c->pops[p].arr[PHX_DOMAIN];
+        if (mixed(c))
+        {
+            if (!A.segOn)
+                return fail(c, PHX_ERR_STATE, "mixed-precision push on unsorted storage");
+            launchPush32(A, dom.view32(), c->stream);
+        }
+        else
+            launchPush(A, dom.view(), dom.view(), c->stream);
         M.mass[p] = c->pops[p].mass;
         for (int m = 0; m < 5; ++m)
         {
@@ -598,10 +622,51 @@ std::vector<size_t> chunkInterleave(size_t n, int dim, int order)
     return ord;
 }
 
-phx_status upload(phx_ctx* c, Store& s, size_t n, int32_t const* ic, double const* de, double const* v,
-                  double const* w, double const* q, size_t offset)
+// FP64 host values into a device array of the store's precision
+phx_status putReals(phx_ctx* c, Store const& s, void* dst, size_t offset, std::vector<double> const& src)
+{
+    size_t const n = src.size();
+    if (s.real == 8)
+    {
+        PHX_HIP(c, hipMemcpy(static_cast<double*>(dst) + offset, src.data(), n * 8, hipMemcpyHostToDevice));
+        return PHX_OK;
+    }
+    std::vector<float> f(n);
+    for (size_t i = 0; i < n; ++i)
+        f[i] = static_cast<float>(src[i]);
+    PHX_HIP(c, hipMemcpy(static_cast<float*>(dst) + offset, f.data(), n * 4, hipMemcpyHostToDevice));
+    return PHX_OK;
+}
+
+phx_status getReals(phx_ctx* c, Store const& s, void const* src, std::vector<double>& dst)
+{
+    size_t const n = dst.size();
+    if (s.real == 8)
+    {
+        PHX_HIP(c, hipMemcpy(dst.data(), src, n * 8, hipMemcpyDeviceToHost));
+        return PHX_OK;
+    }
+    std::vector<float> f(n);
+    PHX_HIP(c, hipMemcpy(f.data(), src, n * 4, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; ++i)
+        dst[i] = static_cast<double>(f[i]);
+    return PHX_OK;
+}
+
+phx_status upload(phx_ctx* c, Pop& P, Store& s, size_t n, int32_t const* ic, double const* de,
+                  double const* v, double const* w, double const* q, size_t offset)
 {
     int const D = c->cfg.dim;
+    if (s.real == 4 && n)
+    {
+        // mixed precision: one charge per population, kept on the host
+        double const q0 = P.hasCharge && offset > 0 ? P.charge : q[0];
+        for (size_t i = 0; i < n; ++i)
+            if (q[i] != q0)
+                return fail(c, PHX_ERR_ARG, "the mixed-precision family keeps one charge per population");
+        P.charge    = q0;
+        P.hasCharge = true;
+    }
     PHX_TRY(storeReserve(c, s, offset + n));
     s.seg = false;
     // the copies below run on the null stream, which does not wait for the context's
@@ -626,33 +691,42 @@ phx_status upload(phx_ctx* c, Store& s, size_t n, int32_t const* ic, double cons
             ib[i] = ic[o[i] * D + d];
         PHX_HIP(c, hipMemcpy(s.ic[d] + offset, ib.data(), n * 4, hipMemcpyHostToDevice));
         for (size_t i = 0; i < n; ++i)
+        {
             db[i] = de[o[i] * D + d];
-        PHX_HIP(c, hipMemcpy(s.de[d] + offset, db.data(), n * 8, hipMemcpyHostToDevice));
+            // an FP32 delta must stay below 1: the largest float under 1 for a delta that rounds up
+            if (s.real == 4 && static_cast<float>(db[i]) >= 1.f)
+                db[i] = static_cast<double>(std::nextafter(1.f, 0.f));
+        }
+        PHX_TRY(putReals(c, s, s.de[d], offset, db));
     }
     for (int d = 0; d < 3; ++d)
     {
         for (size_t i = 0; i < n; ++i)
             db[i] = v[o[i] * 3 + d];
-        PHX_HIP(c, hipMemcpy(s.v[d] + offset, db.data(), n * 8, hipMemcpyHostToDevice));
+        PHX_TRY(putReals(c, s, s.v[d], offset, db));
     }
     for (size_t i = 0; i < n; ++i)
         db[i] = w[o[i]];
-    PHX_HIP(c, hipMemcpy(s.w + offset, db.data(), n * 8, hipMemcpyHostToDevice));
-    for (size_t i = 0; i < n; ++i)
-        db[i] = q[o[i]];
-    PHX_HIP(c, hipMemcpy(s.q + offset, db.data(), n * 8, hipMemcpyHostToDevice));
+    PHX_TRY(putReals(c, s, s.w, offset, db));
+    if (s.q)
+    {
+        for (size_t i = 0; i < n; ++i)
+            db[i] = q[o[i]];
+        PHX_TRY(putReals(c, s, s.q, offset, db));
+    }
     s.n = offset + n;
     return PHX_OK;
 }
 
-// live (not lost) particles of a store, downloaded to host SoA buffers
-phx_status download(phx_ctx* c, Store const& s, std::vector<int32_t>& ic, std::vector<double>& de,
-                    std::vector<double>& v, std::vector<double>& w, std::vector<double>& q)
+// live (not lost) particles of a store, downloaded to host SoA buffers (FP64)
+phx_status download(phx_ctx* c, Pop const& P, Store const& s, std::vector<int32_t>& ic,
+                    std::vector<double>& de, std::vector<double>& v, std::vector<double>& w,
+                    std::vector<double>& q)
 {
     int const D  = c->cfg.dim;
     size_t const n = s.n;
     std::vector<int32_t> icd(n * D);
-    std::vector<double> ded(n * D), vd(n * 3), wd(n), qd(n);
+    std::vector<double> ded(n * D), vd(n * 3), wd(n), qd(n, P.charge);
     std::vector<int32_t> ib(n);
     std::vector<double> db(n);
     PHX_HIP(c, hipStreamSynchronize(c->stream));
@@ -661,18 +735,19 @@ phx_status download(phx_ctx* c, Store const& s, std::vector<int32_t>& ic, std::v
         PHX_HIP(c, hipMemcpy(ib.data(), s.ic[d], n * 4, hipMemcpyDeviceToHost));
         for (size_t i = 0; i < n; ++i)
             icd[i * D + d] = ib[i];
-        PHX_HIP(c, hipMemcpy(db.data(), s.de[d], n * 8, hipMemcpyDeviceToHost));
+        PHX_TRY(getReals(c, s, s.de[d], db));
         for (size_t i = 0; i < n; ++i)
             ded[i * D + d] = db[i];
     }
     for (int d = 0; d < 3; ++d)
     {
-        PHX_HIP(c, hipMemcpy(db.data(), s.v[d], n * 8, hipMemcpyDeviceToHost));
+        PHX_TRY(getReals(c, s, s.v[d], db));
         for (size_t i = 0; i < n; ++i)
             vd[i * 3 + d] = db[i];
     }
-    PHX_HIP(c, hipMemcpy(wd.data(), s.w, n * 8, hipMemcpyDeviceToHost));
-    PHX_HIP(c, hipMemcpy(qd.data(), s.q, n * 8, hipMemcpyDeviceToHost));
+    PHX_TRY(getReals(c, s, s.w, wd));
+    if (s.q)
+        PHX_TRY(getReals(c, s, s.q, qd));
     ic.clear();
     de.clear();
     v.clear();
@@ -713,7 +788,10 @@ phx_status sortDomain(phx_ctx* c)
         Pop& P     = c->pops[p];
         Store& dom = P.arr[PHX_DOMAIN];
         Store& buf = c->sortBuf[p];
-        launchSegCount(c->sg, dom.view(), dom.n, c->segWork, P.seg, c->stream);
+        if (mixed(c))
+            launchSegCount(c->sg, dom.view32(), dom.n, c->segWork, P.seg, c->stream);
+        else
+            launchSegCount(c->sg, dom.view(), dom.n, c->segWork, P.seg, c->stream);
         PHX_TRY(checkLaunch(c, "particle sort (count)"));
         unsigned long long slots = 0;
         PHX_HIP(c, hipStreamSynchronize(c->stream));
@@ -734,8 +812,12 @@ phx_status sortDomain(phx_ctx* c)
             PHX_TRY(devAlloc(c, (void**)&P.seg.items, static_cast<size_t>(need) * 8));
             P.itemsAlloc = need;
         }
-        launchSegScatter(c->sg, dom.view(), buf.view(), dom.n, c->segWork, P.seg, c->cfg.dim,
-                         c->stream);
+        if (mixed(c))
+            launchSegScatter(c->sg, dom.view32(), buf.view32(), dom.n, c->segWork, P.seg, c->cfg.dim,
+                             c->stream);
+        else
+            launchSegScatter(c->sg, dom.view(), buf.view(), dom.n, c->segWork, P.seg, c->cfg.dim,
+                             c->stream);
         PHX_TRY(checkLaunch(c, "particle sort (scatter)"));
         buf.n   = slots;
         buf.seg = true;
@@ -834,7 +916,11 @@ phx_status phx_create(phx_config const* cfg, phx_patch const* patch, size_t part
         return PHX_ERR_ARG;
     *out = nullptr;
     if (cfg->dim < 1 || cfg->dim > 3 || cfg->interp_order < 1 || cfg->interp_order > 3
-        || cfg->n_pop < 1 || cfg->n_pop > PHX_MAX_POP || cfg->precision != PHX_PRECISION_FP64)
+        || cfg->n_pop < 1 || cfg->n_pop > PHX_MAX_POP
+        || (cfg->precision != PHX_PRECISION_FP64 && cfg->precision != PHX_PRECISION_FP32))
+        return PHX_ERR_ARG;
+    // the mixed-precision family runs the resident path of a periodic level
+    if (cfg->precision == PHX_PRECISION_FP32 && !patch->periodic)
         return PHX_ERR_ARG;
     if (patch->n_nonlevel_ghost_boxes < 0 || patch->n_nonlevel_ghost_boxes > MAX_NLGB)
         return PHX_ERR_ARG;
@@ -1090,8 +1176,10 @@ phx_status phx_set_particles(phx_ctx* c, int pop, int kind, size_t n, int32_t co
     PHX_TRY(validPop(c, pop, kind));
     if (n && (!icell || !delta || !v || !weight || !charge))
         return fail(c, PHX_ERR_ARG, "null particle buffer");
+    if (n && kind != PHX_DOMAIN && mixed(c))
+        return fail(c, PHX_ERR_STATE, "the mixed-precision family holds domain particles only");
     PHX_HIP(c, hipStreamSynchronize(c->stream));
-    return upload(c, c->pops[pop].arr[kind], n, icell, delta, v, weight, charge, 0);
+    return upload(c, c->pops[pop], c->pops[pop].arr[kind], n, icell, delta, v, weight, charge, 0);
 }
 
 phx_status phx_particle_count(phx_ctx* c, int pop, int kind, size_t* n)
@@ -1110,7 +1198,7 @@ phx_status phx_particle_count(phx_ctx* c, int pop, int kind, size_t* n)
         {
             std::vector<int32_t> ic;
             std::vector<double> de, v, w, q;
-            PHX_TRY(download(c, s, ic, de, v, w, q));
+            PHX_TRY(download(c, c->pops[pop], s, ic, de, v, w, q));
             *n = w.size();
             return PHX_OK;
         }
@@ -1125,7 +1213,7 @@ phx_status phx_get_particles(phx_ctx* c, int pop, int kind, int32_t* icell, doub
     PHX_TRY(validPop(c, pop, kind));
     std::vector<int32_t> ic;
     std::vector<double> de, vv, w, q;
-    PHX_TRY(download(c, c->pops[pop].arr[kind], ic, de, vv, w, q));
+    PHX_TRY(download(c, c->pops[pop], c->pops[pop].arr[kind], ic, de, vv, w, q));
     std::copy(ic.begin(), ic.end(), icell);
     std::copy(de.begin(), de.end(), delta);
     std::copy(vv.begin(), vv.end(), v);
@@ -1167,7 +1255,7 @@ phx_status phx_get_particles_aos(phx_ctx* c, int pop, int kind, void* records)
     size_t const oV  = oDe + 8 * D;
     std::vector<int32_t> ic;
     std::vector<double> de, v, w, q;
-    PHX_TRY(download(c, c->pops[pop].arr[kind], ic, de, v, w, q));
+    PHX_TRY(download(c, c->pops[pop], c->pops[pop].arr[kind], ic, de, v, w, q));
     auto* bytes = static_cast<unsigned char*>(records);
     for (size_t i = 0; i < w.size(); ++i)
     {
@@ -1233,7 +1321,7 @@ phx_status phx_load_maxwellian(phx_ctx* c, int pop, double const* n, double cons
     }
     Store& s = c->pops[pop].arr[PHX_DOMAIN];
     PHX_HIP(c, hipStreamSynchronize(c->stream));
-    return upload(c, s, w.size(), ic.data(), de.data(), v.data(), w.data(), q.data(), s.n);
+    return upload(c, c->pops[pop], s, w.size(), ic.data(), de.data(), v.data(), w.data(), q.data(), s.n);
 }
 
 phx_status phx_update_populations(phx_ctx* c, int em_set, double dt, int mode)
@@ -1242,6 +1330,9 @@ phx_status phx_update_populations(phx_ctx* c, int em_set, double dt, int mode)
         return PHX_ERR_ARG;
     if (em_set < 0 || em_set > 2 || (mode != PHX_MODE_DOMAIN_ONLY && mode != PHX_MODE_ALL))
         return fail(c, PHX_ERR_ARG, "bad em set or mode");
+    if (mixed(c))
+        return fail(c, PHX_ERR_STATE,
+                    "the mixed-precision family runs the resident level path (phx_move_ions, phx_advance)");
     zeroMoments(c);
     int const D = c->cfg.dim;
     Wrap noWrap{};
@@ -1483,7 +1574,13 @@ phx_status phx_initialize(phx_ctx* c)
     for (int p = 0; p < c->cfg.n_pop; ++p)
     {
         PushArgs A = pushArgs(c, p, KIND_DOMAIN, PHX_MODE_ALL, PHX_EM, 0., true);
-        launchDeposit(A, c->pops[p].arr[PHX_DOMAIN].view(), c->stream);
+        if (mixed(c))
+        {
+            A.pushOn = 0;
+            launchPush32(A, c->pops[p].arr[PHX_DOMAIN].view32(), c->stream);
+        }
+        else
+            launchDeposit(A, c->pops[p].arr[PHX_DOMAIN].view(), c->stream);
     }
     PHX_TRY(checkLaunch(c, "initial deposit"));
     PHX_TRY(fillMomentBorders(c));
@@ -1544,7 +1641,7 @@ phx_status phx_diagnostics(phx_ctx* c, double out[4])
     {
         std::vector<int32_t> ic;
         std::vector<double> de, v, w, q;
-        PHX_TRY(download(c, c->pops[p].arr[PHX_DOMAIN], ic, de, v, w, q));
+        PHX_TRY(download(c, c->pops[p], c->pops[p].arr[PHX_DOMAIN], ic, de, v, w, q));
         double k = 0.;
         for (size_t i = 0; i < w.size(); ++i)
         {
@@ -1592,12 +1689,17 @@ phx_status phx_profile_push(phx_ctx* c, double dt, int mode, int reps, double* a
     // pairs around direct launches read high on this runtime)
     PushArgs A     = pushArgs(c, 0, KIND_DOMAIN, mode, PHX_EM_AVG, dt, true);
     A.periodicWrap = 1;
-    PView v        = c->pops[0].arr[PHX_DOMAIN].view();
+    Store const& dom = c->pops[0].arr[PHX_DOMAIN];
+    if (mixed(c) && !A.segOn)
+        return fail(c, PHX_ERR_STATE, "mixed-precision push on unsorted storage");
     PHX_HIP(c, hipStreamSynchronize(c->stream));
     hipGraph_t g = nullptr;
     hipGraphExec_t ge = nullptr;
     PHX_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    launchPush(A, v, v, c->stream);
+    if (mixed(c))
+        launchPush32(A, dom.view32(), c->stream);
+    else
+        launchPush(A, dom.view(), dom.view(), c->stream);
     PHX_HIP(c, hipStreamEndCapture(c->stream, &g));
     PHX_HIP(c, hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
     (void)hipGraphDestroy(g);

@@ -33,15 +33,20 @@ enum : uint8_t
     ST_EXPORT = 3  // level-ghost particle that entered the domain -> domain
 };
 
-// structure-of-arrays view of a particle array
-struct PView
+// structure-of-arrays view of a particle array: R = double for the plain FP64 family (per-particle
+// charge), float for the mixed-precision family (FP32 delta, velocity and weight; the charge is
+// the population's, q is null)
+template<class R>
+struct PViewT
 {
     int32_t* ic[3];
-    double* de[3];
-    double* v[3];
-    double* w;
-    double* q;
+    R* de[3];
+    R* v[3];
+    R* w;
+    R* q;
 };
+using PView   = PViewT<double>;
+using PView32 = PViewT<float>;
 
 struct Boxes
 {
@@ -109,6 +114,7 @@ struct PushArgs
     int segOn; // the array is cell-segmented (sg, st valid)
     SegGeom sg;
     SegTables st;
+    double charge; // the population's charge (mixed-precision family)
 };
 
 struct Wrap
@@ -138,6 +144,15 @@ void launchSegCount(SegGeom const& sg, PView const& src, unsigned long long n, u
 // second half, once dst can hold the slot count: work items, scatter and empty-slot marking
 void launchSegScatter(SegGeom const& sg, PView const& src, PView const& dst, unsigned long long n,
                       unsigned int* work, SegTables const& tables, int dim, hipStream_t s);
+// mixed-precision family: the same sort over FP32 arrays
+void launchSegCount(SegGeom const& sg, PView32 const& src, unsigned long long n, unsigned int* work,
+                    SegTables const& tables, hipStream_t s);
+void launchSegScatter(SegGeom const& sg, PView32 const& src, PView32 const& dst, unsigned long long n,
+                      unsigned int* work, SegTables const& tables, int dim, hipStream_t s);
+// mixed-precision push + gather + deposit over cell-segmented FP32 storage, in place (A.pushOn 0:
+// deposit only); the LDS tile of segTileShape32
+void launchPush32(PushArgs const& A, PView32 const& v, hipStream_t s);
+TileSpec segTileShape32(int dim, int order, SegGeom const& sg);
 void launchCompact(PView const& src, PView const& dst, uint8_t const* status, unsigned long long n,
                    uint8_t want, unsigned int* blockCounts, unsigned long long* total,
                    unsigned long long dstOffset, int dim, Wrap const& wrap, hipStream_t s);

@@ -15,15 +15,12 @@
 // tile with one HBM atomic per touched node and moment; the gather reads E/B through the L1/L2
 // caches, leaving the LDS to the deposit. Particles whose stencil leaves the tile deposit
 // straight into HBM.
-#include "phx_kernels.hpp"
+#include "phx_push_common.hpp"
 
 #include <algorithm>
 
 #ifndef PHX_PPT_ND
 #define PHX_PPT_ND 2 // particles per thread of the linear 2D/3D push
-#endif
-#ifndef PHX_SEG_ROWS
-#define PHX_SEG_ROWS 16 // segment rows per workgroup of the segmented push
 #endif
 
 namespace phx
@@ -31,15 +28,13 @@ namespace phx
 
 namespace
 {
-constexpr int BLOCK    = 256;
-constexpr int NW       = BLOCK / 64; // wavefronts per workgroup
-constexpr int LANES    = 64;
-constexpr int SEG_ROWS = PHX_SEG_ROWS;
-constexpr unsigned int TILE_BYTES_MAX = 65536;
-#ifndef PHX_TILE_SPREAD
-#define PHX_TILE_SPREAD 1
-#endif
-constexpr int TILE_SPREAD = PHX_TILE_SPREAD; // cells of drift since the sort in a segment tile
+using pc::BLOCK;
+using pc::inBox;
+using pc::inNLGB;
+using pc::LANES;
+using pc::NW;
+using pc::SEG_ROWS;
+using pc::TILE_BYTES_MAX;
 
 // Per (dim, order): whether the deposit is accumulated in registers over a union window, the
 // particles per thread of the linear push and the window. With accumulation, a thread's
@@ -56,22 +51,6 @@ struct Cfg
     static constexpr int WB    = O == 1 ? 0 : 1;  // window base = local cell - WB
     static constexpr int WN    = D == 1 ? W : (D == 2 ? W * W : W * W * W);
 };
-
-__device__ __forceinline__ bool inBox(int const* c, int const* lo, int const* hi, int D)
-{
-    for (int d = 0; d < D; ++d)
-        if (c[d] < lo[d] || c[d] > hi[d])
-            return false;
-    return true;
-}
-
-__device__ __forceinline__ bool inNLGB(int const* c, Boxes const& b, int D)
-{
-    for (int i = 0; i < b.nnlgb; ++i)
-        if (inBox(c, b.nlgb_lo[i], b.nlgb_hi[i], D))
-            return true;
-    return false;
-}
 
 // linear storage: (thread, step) -> slot in the chunk; consecutive threads read consecutive slots
 // (coalesced). The loader's arrays are stored chunk-interleaved (chunkInterleave on the host):
@@ -686,9 +665,6 @@ __global__ __launch_bounds__(BLOCK) void k_push(PushArgs A, PView in, PView out)
 // fused push + deposit over cell-segmented storage: one workgroup per work item (a segment and
 // SEG_ROWS of its rows); wavefront w takes rows first + w, first + w + NW, ...
 // ---------------------------------------------------------------------------------------------
-#ifndef PHX_SEG_WAVES
-#define PHX_SEG_WAVES 4 // waves per SIMD the register allocation of the segmented push targets
-#endif
 template<int D, int O>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(PHX_SEG_WAVES)))
 void k_push_seg(PushArgs A, PView in)
@@ -714,26 +690,7 @@ void k_push_seg(PushArgs A, PView in)
     // motion, drift since the sort); the overflow segment has no cell range
     TileView T;
     tileView(T, lds, A.tile);
-    if (seg < static_cast<unsigned int>(S.nsegs))
-    {
-        int rem = static_cast<int>(seg);
-        int sidx[3];
-#pragma unroll
-        for (int d = D - 1; d >= 0; --d)
-        {
-            sidx[d] = rem % S.nseg[d];
-            rem /= S.nseg[d];
-        }
-#pragma unroll
-        for (int d = 0; d < D; ++d)
-            T.org[d] = S.lo[d] + sidx[d] * S.seg[d] - (G.lower[d] - G.g) - A.tile.lo[d];
-    }
-    else
-    {
-#pragma unroll
-        for (int d = 0; d < D; ++d)
-            T.org[d] = -0x20000000;
-    }
+    pc::segTileOrigin<D>(G, S, A.tile, seg, T.org);
     __syncthreads();
 
     Acc<D, O> acc;
@@ -867,7 +824,8 @@ __global__ __launch_bounds__(BLOCK) void k_scatter_status(PView src, PView dst,
     dst.v[1][j] = src.v[1][i];
     dst.v[2][j] = src.v[2][i];
     dst.w[j]    = src.w[i];
-    dst.q[j]    = src.q[i];
+    if (src.q)
+        dst.q[j] = src.q[i];
 }
 // ---------------------------------------------------------------------------------------------
 // counting sort of a domain array into cell-segmented storage (dead particles are dropped)
@@ -878,8 +836,9 @@ constexpr unsigned int DEAD_KEY = 0xffffffffu;
 
 // key of a particle: its row-major cell index in the domain box, ncells outside the box, DEAD_KEY
 // for a lost particle; c receives the cell relative to the box
-__device__ __forceinline__ unsigned int cellKey(SegGeom const& S, PView const& v,
-                                                unsigned long long i, int* c)
+template<class V>
+__device__ __forceinline__ unsigned int cellKey(SegGeom const& S, V const& v, unsigned long long i,
+                                                int* c)
 {
     if (v.ic[0][i] == DEAD_CELL)
         return DEAD_KEY;
@@ -943,7 +902,8 @@ __global__ __launch_bounds__(BLOCK) void k_seg_zero(unsigned int* __restrict__ p
         p[i] = 0u;
 }
 
-__global__ __launch_bounds__(BLOCK) void k_seg_count(SegGeom S, PView v, unsigned long long n,
+template<class V>
+__global__ __launch_bounds__(BLOCK) void k_seg_count(SegGeom S, V v, unsigned long long n,
                                                      unsigned int* __restrict__ counts)
 {
     unsigned long long const i = static_cast<unsigned long long>(blockIdx.x) * BLOCK + threadIdx.x;
@@ -1044,7 +1004,8 @@ __global__ __launch_bounds__(BLOCK) void k_seg_items(unsigned int const* __restr
     }
 }
 
-__global__ __launch_bounds__(BLOCK) void k_seg_scatter(SegGeom S, PView src, PView dst,
+template<class V>
+__global__ __launch_bounds__(BLOCK) void k_seg_scatter(SegGeom S, V src, V dst,
                                                        unsigned long long n,
                                                        unsigned int* __restrict__ cursor,
                                                        unsigned long long const* __restrict__ off)
@@ -1076,11 +1037,13 @@ __global__ __launch_bounds__(BLOCK) void k_seg_scatter(SegGeom S, PView src, PVi
     dst.v[1][j] = src.v[1][i];
     dst.v[2][j] = src.v[2][i];
     dst.w[j]    = src.w[i];
-    dst.q[j]    = src.q[i];
+    if (src.q)
+        dst.q[j] = src.q[i];
 }
 
 // marks the empty slots of every segment dead (one workgroup per segment)
-__global__ __launch_bounds__(BLOCK) void k_seg_pad(SegGeom S, PView dst,
+template<class V>
+__global__ __launch_bounds__(BLOCK) void k_seg_pad(SegGeom S, V dst,
                                                    unsigned int const* __restrict__ rows,
                                                    unsigned long long const* __restrict__ off,
                                                    unsigned int const* __restrict__ counts)
@@ -1102,8 +1065,9 @@ __global__ __launch_bounds__(BLOCK) void k_seg_pad(SegGeom S, PView dst,
     }
 }
 
-void launchSegCount(SegGeom const& sg, PView const& src, unsigned long long n, unsigned int* work,
-                    SegTables const& tables, hipStream_t s)
+template<class V>
+static void segCountT(SegGeom const& sg, V const& src, unsigned long long n, unsigned int* work,
+                      SegTables const& tables, hipStream_t s)
 {
     unsigned int const nc = static_cast<unsigned int>(sg.ncells) + 1;
     unsigned int const m  = static_cast<unsigned int>(sg.nsegs) + 1;
@@ -1111,14 +1075,15 @@ void launchSegCount(SegGeom const& sg, PView const& src, unsigned long long n, u
     unsigned int* itemCount = work + nc;
     k_seg_zero<<<(nc + BLOCK - 1) / BLOCK, BLOCK, 0, s>>>(counts, nc);
     if (n)
-        k_seg_count<<<static_cast<unsigned int>((n + BLOCK - 1) / BLOCK), BLOCK, 0, s>>>(sg, src, n,
-                                                                                       counts);
+        k_seg_count<V><<<static_cast<unsigned int>((n + BLOCK - 1) / BLOCK), BLOCK, 0, s>>>(sg, src, n,
+                                                                                          counts);
     k_seg_rows<<<(m + BLOCK - 1) / BLOCK, BLOCK, 0, s>>>(sg, counts, tables.rows, itemCount);
     k_seg_scan<<<1, 1024, 0, s>>>(tables.rows, tables.off, itemCount, tables.nItems, m);
 }
 
-void launchSegScatter(SegGeom const& sg, PView const& src, PView const& dst, unsigned long long n,
-                      unsigned int* work, SegTables const& tables, int, hipStream_t s)
+template<class V>
+static void segScatterT(SegGeom const& sg, V const& src, V const& dst, unsigned long long n,
+                        unsigned int* work, SegTables const& tables, hipStream_t s)
 {
     unsigned int const nc = static_cast<unsigned int>(sg.ncells) + 1;
     unsigned int const m  = static_cast<unsigned int>(sg.nsegs) + 1;
@@ -1127,9 +1092,30 @@ void launchSegScatter(SegGeom const& sg, PView const& src, PView const& dst, uns
     k_seg_items<<<(m + BLOCK - 1) / BLOCK, BLOCK, 0, s>>>(tables.rows, itemCount, tables.items, m);
     k_seg_zero<<<(nc + BLOCK - 1) / BLOCK, BLOCK, 0, s>>>(counts, nc);
     if (n)
-        k_seg_scatter<<<static_cast<unsigned int>((n + BLOCK - 1) / BLOCK), BLOCK, 0, s>>>(
+        k_seg_scatter<V><<<static_cast<unsigned int>((n + BLOCK - 1) / BLOCK), BLOCK, 0, s>>>(
             sg, src, dst, n, counts, tables.off);
-    k_seg_pad<<<m, BLOCK, 0, s>>>(sg, dst, tables.rows, tables.off, counts);
+    k_seg_pad<V><<<m, BLOCK, 0, s>>>(sg, dst, tables.rows, tables.off, counts);
+}
+
+void launchSegCount(SegGeom const& sg, PView const& src, unsigned long long n, unsigned int* work,
+                    SegTables const& tables, hipStream_t s)
+{
+    segCountT(sg, src, n, work, tables, s);
+}
+void launchSegCount(SegGeom const& sg, PView32 const& src, unsigned long long n, unsigned int* work,
+                    SegTables const& tables, hipStream_t s)
+{
+    segCountT(sg, src, n, work, tables, s);
+}
+void launchSegScatter(SegGeom const& sg, PView const& src, PView const& dst, unsigned long long n,
+                      unsigned int* work, SegTables const& tables, int, hipStream_t s)
+{
+    segScatterT(sg, src, dst, n, work, tables, s);
+}
+void launchSegScatter(SegGeom const& sg, PView32 const& src, PView32 const& dst, unsigned long long n,
+                      unsigned int* work, SegTables const& tables, int, hipStream_t s)
+{
+    segScatterT(sg, src, dst, n, work, tables, s);
 }
 
 SegGeom segGeom(int dim, int const* lo, int const* hi)
@@ -1169,10 +1155,8 @@ TileSpec segTileShape(int dim, int order, SegGeom const& sg, int copies)
 {
     TileSpec T{};
     T.copies = copies;
-    // node range of a particle's deposit around its cell: [-1, 2] (orders 2, 3), [0, 1]
-    // (order 1); one cell of motion and TILE_SPREAD cells of drift since the sort on each side
-    int const mlo = (order >= 2 ? 1 : 0) + 1 + TILE_SPREAD;
-    int const mhi = (order == 1 ? 1 : 2) + 1 + TILE_SPREAD;
+    int const mlo = pc::segMarginLo(order);
+    int const mhi = pc::segMarginHi(order);
     for (int d = 0; d < 3; ++d)
     {
         T.lo[d]  = d < dim ? mlo : 0;

@@ -1,0 +1,79 @@
+// phx_push_common.hpp -- launch geometry and box predicates shared by the FP64 and the
+// mixed-precision particle kernels (the two must agree on the work items the sort builds).
+#pragma once
+
+#include "phx_kernels.hpp"
+
+#ifndef PHX_SEG_ROWS
+#define PHX_SEG_ROWS 16 // segment rows per workgroup of the segmented push
+#endif
+#ifndef PHX_TILE_SPREAD
+#define PHX_TILE_SPREAD 1
+#endif
+#ifndef PHX_SEG_WAVES
+#define PHX_SEG_WAVES 4 // waves per SIMD the register allocation of the segmented push targets
+#endif
+
+namespace phx
+{
+namespace pc
+{
+constexpr int BLOCK    = 256;
+constexpr int NW       = BLOCK / 64; // wavefronts per workgroup
+constexpr int LANES    = 64;
+constexpr int SEG_ROWS = PHX_SEG_ROWS;
+constexpr unsigned int TILE_BYTES_MAX = 65536;
+constexpr int TILE_SPREAD = PHX_TILE_SPREAD; // cells of drift since the sort in a segment tile
+
+__device__ __forceinline__ bool inBox(int const* c, int const* lo, int const* hi, int D)
+{
+    for (int d = 0; d < D; ++d)
+        if (c[d] < lo[d] || c[d] > hi[d])
+            return false;
+    return true;
+}
+
+// UpdaterSelectionBoxing::nonLevelGhostBox membership (ion_updater.hpp:128-163)
+__device__ __forceinline__ bool inNLGB(int const* c, Boxes const& b, int D)
+{
+    for (int i = 0; i < b.nnlgb; ++i)
+        if (inBox(c, b.nlgb_lo[i], b.nlgb_hi[i], D))
+            return true;
+    return false;
+}
+
+// local node index of the LDS tile origin of segment `seg` (the segment's first cell less the
+// tile's low margin); the overflow segment has no cell range and gets an unreachable origin
+template<int D>
+__device__ __forceinline__ void segTileOrigin(Geom const& G, SegGeom const& S, TileSpec const& tile,
+                                              unsigned int seg, int* org)
+{
+    if (seg < static_cast<unsigned int>(S.nsegs))
+    {
+        int rem = static_cast<int>(seg);
+        int sidx[3];
+#pragma unroll
+        for (int d = D - 1; d >= 0; --d)
+        {
+            sidx[d] = rem % S.nseg[d];
+            rem /= S.nseg[d];
+        }
+#pragma unroll
+        for (int d = 0; d < D; ++d)
+            org[d] = S.lo[d] + sidx[d] * S.seg[d] - (G.lower[d] - G.g) - tile.lo[d];
+    }
+    else
+    {
+#pragma unroll
+        for (int d = 0; d < D; ++d)
+            org[d] = -0x20000000;
+    }
+}
+
+// node margins of a segment tile: a particle's deposit spans nodes [-1, 2] of its cell (orders
+// 2, 3) or [0, 1] (order 1); one cell of motion and TILE_SPREAD cells of drift since the sort
+inline int segMarginLo(int order) { return (order >= 2 ? 1 : 0) + 1 + TILE_SPREAD; }
+inline int segMarginHi(int order) { return (order == 1 ? 1 : 2) + 1 + TILE_SPREAD; }
+
+} // namespace pc
+} // namespace phx
