@@ -77,6 +77,7 @@ struct Pop
     double mass = 1.;
     double charge = 0.; // the particles' charge (mixed-precision family: one per population)
     bool hasCharge = false;
+    bool uniformCharge = true; // every particle uploaded so far has charge `charge`
     Store arr[3]; // domain, patch ghost, level ghost
     double* mom[5] = {};
     SegTables seg{}; // tables of the segmented domain array (items sized for itemsAlloc items)
@@ -391,6 +392,7 @@ PushArgs pushArgs(phx_ctx* c, int pop, int kind, int mode, int emSet, double dt,
         A.st.itemsCap = segItemsCap(c->sg, s.cap);
         A.tile        = mixed(c) ? segTileShape32(c->cfg.dim, c->cfg.interp_order, c->sg)
                                  : segTileShape(c->cfg.dim, c->cfg.interp_order, c->sg, PHX_SEG_COPIES);
+        A.rhoFromN    = !mixed(c) && P.hasCharge && P.uniformCharge ? 1 : 0;
     }
     return A;
 }
@@ -657,6 +659,19 @@ phx_status upload(phx_ctx* c, Pop& P, Store& s, size_t n, int32_t const* ic, dou
                   double const* v, double const* w, double const* q, size_t offset)
 {
     int const D = c->cfg.dim;
+    if (s.real == 8 && n)
+    {
+        // FP64 family: note whether the population keeps a single charge (the segmented push
+        // then derives the charge density from n)
+        if (!P.hasCharge)
+        {
+            P.charge        = q[0];
+            P.hasCharge     = true;
+            P.uniformCharge = true;
+        }
+        for (size_t i = 0; i < n && P.uniformCharge; ++i)
+            P.uniformCharge = q[i] == P.charge;
+    }
     if (s.real == 4 && n)
     {
         // mixed precision: one charge per population, kept on the host

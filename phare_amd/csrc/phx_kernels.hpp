@@ -114,7 +114,9 @@ struct PushArgs
     int segOn; // the array is cell-segmented (sg, st valid)
     SegGeom sg;
     SegTables st;
-    double charge; // the population's charge (mixed-precision family)
+    double charge; // the population's charge (mixed-precision family; FP64 family with rhoFromN)
+    int rhoFromN;  // FP64 segmented push: every particle has charge `charge`, so the LDS tile
+                   // leaves the charge density out and its flush takes charge x n
 };
 
 struct Wrap

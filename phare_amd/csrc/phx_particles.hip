@@ -205,8 +205,9 @@ __device__ __forceinline__ void gatherEB(Geom const& G, EMPtrs const& em, Stenci
 }
 
 // scatter-add the 5 moments of one particle: into the lane's copy of the LDS tile when its
-// whole stencil lies in the tile, else straight into the HBM moment arrays
-template<int D, int O>
+// whole stencil lies in the tile, else straight into the HBM moment arrays. RN: the population
+// has one charge, the tile skips the charge density and its flush derives it from n
+template<int D, int O, bool RN = false>
 __device__ __forceinline__ void depositOne(DepositData<D, O> const& dd, Geom const& G,
                                            TileView const& T, int copy, double* const* mom)
 {
@@ -235,7 +236,8 @@ __device__ __forceinline__ void depositOne(DepositData<D, O> const& dd, Geom con
                                             D > 2 ? dd.start[2] + c - T.org[2] : 0);
 #pragma unroll
                     for (int m = 0; m < 5; ++m)
-                        atomicAdd(slab[m] + node, nodeWeight<D, O>(dd, dd.val[m], a, b, c));
+                        if (!RN || m != 1)
+                            atomicAdd(slab[m] + node, nodeWeight<D, O>(dd, dd.val[m], a, b, c));
                 }
     }
     else
@@ -388,7 +390,7 @@ __device__ __forceinline__ void loadPart(PView const& in, unsigned long long i, 
 // One particle of updatePopulations: prePush, gather, Boris, postPush (boris.hpp:93-138), the
 // selection of ion_updater.hpp:169-295, the deposit and, in mode all, the write-back. Dead slots
 // (lost particles, empty slots of segmented storage) are skipped, and dropped by a compaction.
-template<int D, int O>
+template<int D, int O, bool RN = false>
 __device__ __forceinline__ void pushLoaded(PushArgs const& A, PView const& out, bool outOfPlace,
                                            unsigned long long i, Part<D> const& P,
                                            TileView const& T, int copy, Acc<D, O>& acc)
@@ -508,7 +510,7 @@ __device__ __forceinline__ void pushLoaded(PushArgs const& A, PView const& out, 
                 accAdd<D, O>(acc, dd);
             }
             else
-                depositOne<D, O>(dd, G, T, copy, A.mom);
+                depositOne<D, O, RN>(dd, G, T, copy, A.mom);
         }
     }
 
@@ -554,14 +556,14 @@ __device__ __forceinline__ void pushLoaded(PushArgs const& A, PView const& out, 
     }
 }
 
-template<int D, int O>
+template<int D, int O, bool RN = false>
 __device__ __forceinline__ void pushParticle(PushArgs const& A, PView const& in, PView const& out,
                                              unsigned long long i, TileView const& T, int copy,
                                              Acc<D, O>& acc)
 {
     Part<D> P;
     loadPart<D>(in, i, P);
-    pushLoaded<D, O>(A, out, out.w != in.w, i, P, T, copy, acc);
+    pushLoaded<D, O, RN>(A, out, out.w != in.w, i, P, T, copy, acc);
 }
 
 __device__ __forceinline__ void zeroTile(double* lds, TileSpec const& S)
@@ -570,10 +572,11 @@ __device__ __forceinline__ void zeroTile(double* lds, TileSpec const& S)
         lds[x] = 0.;
 }
 
-// flush the tile: the copies of each node summed, one HBM atomic per touched node and moment
-template<int D>
+// flush the tile: the copies of each node summed, one HBM atomic per touched node and moment;
+// RN: the charge density is the population's charge times n
+template<int D, bool RN = false>
 __device__ __forceinline__ void flushTile(Geom const& G, TileView const& T, int nodes,
-                                          double* const* mom)
+                                          double* const* mom, double charge = 0.)
 {
     int const e12 = T.ext[1] * T.ext[2];
     for (int x = threadIdx.x; x < nodes; x += BLOCK)
@@ -583,8 +586,15 @@ __device__ __forceinline__ void flushTile(Geom const& G, TileView const& T, int 
         for (int m = 0; m < 5; ++m)
         {
             double val = 0.;
-            for (int c = 0; c < T.copies; ++c)
-                val += T.mom(m, c)[x];
+            if (RN && m == 1)
+            {
+                for (int c = 0; c < T.copies; ++c)
+                    val += T.mom(0, c)[x];
+                val *= charge;
+            }
+            else
+                for (int c = 0; c < T.copies; ++c)
+                    val += T.mom(m, c)[x];
             if (val != 0.)
             {
                 if (node < 0)
@@ -665,7 +675,7 @@ __global__ __launch_bounds__(BLOCK) void k_push(PushArgs A, PView in, PView out)
 // fused push + deposit over cell-segmented storage: one workgroup per work item (a segment and
 // SEG_ROWS of its rows); wavefront w takes rows first + w, first + w + NW, ...
 // ---------------------------------------------------------------------------------------------
-template<int D, int O>
+template<int D, int O, bool RN>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(PHX_SEG_WAVES)))
 void k_push_seg(PushArgs A, PView in)
 {
@@ -708,18 +718,18 @@ void k_push_seg(PushArgs A, PView in)
         Part<D> const cur = next;
         if (r + NW < rEnd)
             loadPart<D>(in, off + static_cast<unsigned long long>(r + NW) * LANES + lane, next);
-        pushLoaded<D, O>(A, out, false, off + static_cast<unsigned long long>(r) * LANES + lane,
-                         cur, T, copy, acc);
+        pushLoaded<D, O, RN>(A, out, false, off + static_cast<unsigned long long>(r) * LANES + lane,
+                             cur, T, copy, acc);
     }
 #else
     for (unsigned int r = r0 + (t >> 6); r < rEnd; r += NW)
-        pushParticle<D, O>(A, in, out, off + static_cast<unsigned long long>(r) * LANES + lane, T,
-                           copy, acc);
+        pushParticle<D, O, RN>(A, in, out, off + static_cast<unsigned long long>(r) * LANES + lane, T,
+                               copy, acc);
 #endif
     if constexpr (C::ACC)
         accFlush<D, O>(acc, G, T, copy, A.mom);
     __syncthreads();
-    flushTile<D>(G, T, A.tile.nodes, A.mom);
+    flushTile<D, RN>(G, T, A.tile.nodes, A.mom, A.charge);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1195,7 +1205,10 @@ static void launchPushT(PushArgs const& A, PView const& in, PView const& out, hi
     if (A.segOn)
     {
         // segmented arrays are pushed in place (out is in)
-        k_push_seg<D, O><<<A.st.itemsCap, BLOCK, A.tile.bytes, s>>>(A, in);
+        if (A.rhoFromN)
+            k_push_seg<D, O, true><<<A.st.itemsCap, BLOCK, A.tile.bytes, s>>>(A, in);
+        else
+            k_push_seg<D, O, false><<<A.st.itemsCap, BLOCK, A.tile.bytes, s>>>(A, in);
         return;
     }
     if (A.n == 0)
