@@ -29,10 +29,11 @@ from phare_amd import workloads as W  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 
-# HBM bytes per push dispatch (k_push_seg, mode all) from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of
+# HBM bytes per push dispatch (mode all) from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of
 # this script, FETCH_SIZE scaled x2 and WRITE_SIZE x1 by a 1 GiB device copy's readings; keyed by
-# workload (kernel variant, precision), measured for the default Harris workload only
-MEASURED_TRAFFIC = {("harris2d", "fp64"): 402.0e6}
+# workload and precision family (k_push_seg / k_push_seg32), measured for the default Harris
+# workload only
+MEASURED_TRAFFIC = {("harris2d", "fp64"): 426.6e6, ("harris2d", "fp32"): 254.0e6}
 
 
 def _hip():

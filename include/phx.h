@@ -42,7 +42,9 @@ typedef enum {
 
 /* arithmetic of the particle path */
 enum { PHX_PRECISION_FP64 = 0, /* plain: the reference method in FP64 */
-       PHX_PRECISION_FP32 = 1  /* mixed: FP32 particles and moments, FP64 fields */ };
+       PHX_PRECISION_FP32 = 1  /* mixed: FP32 delta, velocity and weight (one charge per
+                                  population), FP64 fields and moments; resident path of a
+                                  periodic level only (phx_update_populations: PHX_ERR_STATE) */ };
 
 /* UpdaterMode (src/core/numerics/ion_updater/ion_updater.hpp:22) */
 enum { PHX_MODE_DOMAIN_ONLY = 1, PHX_MODE_ALL = 2 };

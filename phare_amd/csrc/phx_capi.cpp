@@ -860,6 +860,12 @@ std::vector<unsigned long long> storageKey(phx_ctx const* c)
     {
         // the segment tables a segmented push reads (reallocated when the storage grows)
         k.push_back(reinterpret_cast<unsigned long long>(c->pops[p].seg.items));
+        // the population's charge, which the push takes as an argument when it is uniform
+        double const q = c->pops[p].charge;
+        unsigned long long qb;
+        std::memcpy(&qb, &q, 8);
+        k.push_back(qb);
+        k.push_back(c->pops[p].uniformCharge ? 1ull : 0ull);
         for (Store const& s : c->pops[p].arr)
         {
             // segmented storage: the kernels read their extent from the segment tables, so a
