@@ -63,9 +63,10 @@ struct Boxes
 // up to 16.
 struct TileSpec
 {
-    int lo[3];  // margin below the lowest cell of the chunk (nodes)
-    int ext[3]; // extents
-    int nodes;  // ext[0]*ext[1]*ext[2]
+    int lo[3];   // margin below the lowest cell of the chunk (nodes)
+    int ext[3];  // extents
+    int pitch[3]; // node (i, j, k) sits at (i * pitch[1] + j) * pitch[2] + k (pitch[d] >= ext[d])
+    int nodes;  // ext[0]*pitch[1]*pitch[2]
     int copies; // moment copies (power of two): lane l accumulates into copy l % copies
     int stride;
     unsigned int bytes; // dynamic LDS of the launch

@@ -64,9 +64,10 @@ struct TileView
     double* lds;
     int org[3];
     int ext[3];
+    int pitch[3];
     int copies, stride;
     __device__ double* mom(int m, int copy) const { return lds + (m * copies + copy) * stride; }
-    __device__ int node(int i, int j, int k) const { return (i * ext[1] + j) * ext[2] + k; }
+    __device__ int node(int i, int j, int k) const { return (i * pitch[1] + j) * pitch[2] + k; }
     // the O+1 nodes from start lie in the tile along d
     template<int O>
     __device__ bool holds(int d, int start) const
@@ -578,7 +579,7 @@ template<int D, bool RN = false>
 __device__ __forceinline__ void flushTile(Geom const& G, TileView const& T, int nodes,
                                           double* const* mom, double charge = 0.)
 {
-    int const e12 = T.ext[1] * T.ext[2];
+    int const e12 = T.pitch[1] * T.pitch[2];
     for (int x = threadIdx.x; x < nodes; x += BLOCK)
     {
         int node = -1;
@@ -601,8 +602,8 @@ __device__ __forceinline__ void flushTile(Geom const& G, TileView const& T, int 
                 {
                     int const ti = x / e12;
                     int const r  = x - ti * e12;
-                    int const tj = r / T.ext[2];
-                    int const tk = r - tj * T.ext[2];
+                    int const tj = r / T.pitch[2];
+                    int const tk = r - tj * T.pitch[2];
                     node = G.idx(QMOM, T.org[0] + ti, D > 1 ? T.org[1] + tj : 0,
                                  D > 2 ? T.org[2] + tk : 0);
                 }
@@ -620,7 +621,8 @@ __device__ __forceinline__ void tileView(TileView& T, double* lds, TileSpec cons
 #pragma unroll
     for (int d = 0; d < 3; ++d)
     {
-        T.ext[d] = S.ext[d];
+        T.ext[d]   = S.ext[d];
+        T.pitch[d] = S.pitch[d];
         T.org[d] = 0;
     }
 }
@@ -1172,14 +1174,18 @@ TileSpec segTileShape(int dim, int order, SegGeom const& sg, int copies)
         T.lo[d]  = d < dim ? mlo : 0;
         T.ext[d] = d < dim ? sg.seg[d] + mlo + mhi : 1;
     }
-    T.nodes  = T.ext[0] * T.ext[1] * T.ext[2];
+    pc::tilePitch(T, dim);
+    T.nodes  = T.ext[0] * T.pitch[1] * T.pitch[2];
     T.stride = (T.nodes + 15) / 16 * 16; // copies 16 words apart: neighbouring lanes' banks differ
     T.bytes  = static_cast<unsigned int>(8 * 5 * copies * T.stride);
     if (T.bytes > TILE_BYTES_MAX)
     {
         // no room: every deposit goes straight to HBM
         for (int d = 0; d < 3; ++d)
-            T.ext[d] = 1;
+        {
+            T.ext[d]   = 1;
+            T.pitch[d] = 1;
+        }
         T.lo[0]  = -0x10000000;
         T.nodes  = 1;
         T.stride = 16;
@@ -1270,6 +1276,8 @@ TileSpec tileShape(int dim, int order, double ppc, int copies)
         others *= T.ext[d];
     int const maxFast = cap / others;
     T.ext[fast]       = want < maxFast ? want : maxFast;
+    for (int d = 0; d < 3; ++d)
+        T.pitch[d] = T.ext[d];
     T.nodes           = T.ext[0] * T.ext[1] * T.ext[2];
     // copies one word past a multiple of 16: lanes of one cell adding into different copies of
     // a node hit different banks

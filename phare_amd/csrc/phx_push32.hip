@@ -249,10 +249,10 @@ __device__ __forceinline__ bool advancePosition32(int* cell, float* delta, float
 struct Tile32
 {
     double* lds;
-    int org[3], ext[3];
+    int org[3], ext[3], pitch[3];
     int stride;
     __device__ double* mom(int m) const { return lds + m * stride; }
-    __device__ int node(int i, int j, int k) const { return (i * ext[1] + j) * ext[2] + k; }
+    __device__ int node(int i, int j, int k) const { return (i * pitch[1] + j) * pitch[2] + k; }
     template<int O>
     __device__ bool holds(int d, int start) const
     {
@@ -493,8 +493,11 @@ __device__ __forceinline__ void pushSlot32(PushArgs const& A, Push32 const& K, P
 // fused push + deposit over cell-segmented FP32 storage: one workgroup per work item (a segment
 // and SEG_ROWS of its rows), in place
 // ---------------------------------------------------------------------------------------------
+#ifndef PHX_SEG_WAVES32
+#define PHX_SEG_WAVES32 5 // waves per SIMD of the mixed-precision push (fewer registers than FP64)
+#endif
 template<int D, int O>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(PHX_SEG_WAVES)))
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(PHX_SEG_WAVES32)))
 void k_push_seg32(PushArgs A, PView32 v)
 {
     extern __shared__ double lds32[];
@@ -518,7 +521,8 @@ void k_push_seg32(PushArgs A, PView32 v)
 #pragma unroll
     for (int d = 0; d < 3; ++d)
     {
-        T.ext[d] = A.tile.ext[d];
+        T.ext[d]   = A.tile.ext[d];
+        T.pitch[d] = A.tile.pitch[d];
         T.org[d] = 0;
     }
     pc::segTileOrigin<D>(G, A.sg, A.tile, seg, T.org);
@@ -536,7 +540,7 @@ void k_push_seg32(PushArgs A, PView32 v)
     __syncthreads();
 
     // flush: one HBM atomic per touched node and moment
-    int const e12 = T.ext[1] * T.ext[2];
+    int const e12 = T.pitch[1] * T.pitch[2];
     for (int x = t; x < A.tile.nodes; x += BLOCK)
     {
         int node = -1;
@@ -550,8 +554,8 @@ void k_push_seg32(PushArgs A, PView32 v)
                 {
                     int const ti = x / e12;
                     int const rr = x - ti * e12;
-                    int const tj = rr / T.ext[2];
-                    int const tk = rr - tj * T.ext[2];
+                    int const tj = rr / T.pitch[2];
+                    int const tk = rr - tj * T.pitch[2];
                     node = G.idx(QMOM, T.org[0] + ti, D > 1 ? T.org[1] + tj : 0,
                                  D > 2 ? T.org[2] + tk : 0);
                 }
@@ -600,14 +604,18 @@ TileSpec segTileShape32(int dim, int order, SegGeom const& sg)
         T.lo[d]  = d < dim ? mlo : 0;
         T.ext[d] = d < dim ? sg.seg[d] + mlo + mhi : 1;
     }
-    T.nodes  = T.ext[0] * T.ext[1] * T.ext[2];
+    pc::tilePitch(T, dim);
+    T.nodes  = T.ext[0] * T.pitch[1] * T.pitch[2];
     T.stride = (T.nodes + 15) / 16 * 16;
     T.bytes  = static_cast<unsigned int>(8 * 4 * T.stride);
     if (T.bytes > pc::TILE_BYTES_MAX)
     {
         // no room: every deposit goes straight to HBM
         for (int d = 0; d < 3; ++d)
-            T.ext[d] = 1;
+        {
+            T.ext[d]   = 1;
+            T.pitch[d] = 1;
+        }
         T.lo[0]  = -0x10000000;
         T.nodes  = 1;
         T.stride = 16;

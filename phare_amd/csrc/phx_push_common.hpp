@@ -72,6 +72,20 @@ __device__ __forceinline__ void segTileOrigin(Geom const& G, SegGeom const& S, T
 
 // node margins of a segment tile: a particle's deposit spans nodes [-1, 2] of its cell (orders
 // 2, 3) or [0, 1] (order 1); one cell of motion and TILE_SPREAD cells of drift since the sort
+#ifndef PHX_TILE_ROW_PAD
+#define PHX_TILE_ROW_PAD 16
+#endif
+// node pitches of a segment tile: in 2D the rows (fastest dimension) are padded to a multiple of
+// PHX_TILE_ROW_PAD nodes, so that rows of the tile start on the same LDS bank and lanes of one
+// wavefront whose stencils start on neighbouring rows do not collide in banks; 1D/3D unpadded
+inline void tilePitch(TileSpec& T, int dim)
+{
+    for (int d = 0; d < 3; ++d)
+        T.pitch[d] = T.ext[d];
+    if (dim == 2 && PHX_TILE_ROW_PAD > 1)
+        T.pitch[1] = (T.ext[1] + PHX_TILE_ROW_PAD - 1) / PHX_TILE_ROW_PAD * PHX_TILE_ROW_PAD;
+}
+
 inline int segMarginLo(int order) { return (order >= 2 ? 1 : 0) + 1 + TILE_SPREAD; }
 inline int segMarginHi(int order) { return (order == 1 ? 1 : 2) + 1 + TILE_SPREAD; }
 
