@@ -96,6 +96,7 @@ struct phx_ctx
 
     std::vector<void*> allocs;
     double* em[3][6] = {};    // E/B of each set, one block per set (em[set][0] is its base)
+    float* em32[6]   = {};    // mixed-precision family: FP32 copy of the averaged set (gather)
     unsigned int emBytes = 0; // bytes of one set's block
     double* J[3]     = {};
     double* Ve[3]    = {};
@@ -357,6 +358,8 @@ PushArgs pushArgs(phx_ctx* c, int pop, int kind, int mode, int emSet, double dt,
     }
     A.em.bytes = c->emBytes;
     for (int k = 0; k < 6; ++k)
+        A.em32[k] = c->em32[k];
+    for (int k = 0; k < 6; ++k)
         A.em.off[k] = static_cast<unsigned int>((c->em[emSet][k] - c->em[emSet][0]) * 8);
     for (int m = 0; m < 5; ++m)
         A.mom[m] = c->pops[pop].mom[m];
@@ -549,9 +552,10 @@ void enqueueAverage(phx_ctx* c)
                           count(c, k < 3 ? QEX + k : QBX + k - 3), c->stream);
         return;
     }
+    // the mixed-precision family's push gathers from an FP32 copy of the averaged set
     launchAverageFill(c->G, cvec(c->em[PHX_EM]), cvec(c->em[PHX_EM_PRED]), cvec(c->em[PHX_EM] + 3),
                       cvec(c->em[PHX_EM_PRED] + 3), vec(c->em[PHX_EM_AVG]), vec(c->em[PHX_EM_AVG] + 3),
-                      c->stream);
+                      c->stream, mixed(c) ? c->em32 : nullptr);
 }
 
 // SolverPPC::corrector_ (solver_ppc.hpp:433-479): in-place Faraday, then the B ghost fill;
@@ -1040,6 +1044,13 @@ phx_status phx_create(phx_config const* cfg, phx_patch const* patch, size_t part
             c->em[set][k] = block + emOff[k];
     }
     c->emBytes = static_cast<unsigned int>(emWords * 8);
+    if (s == PHX_OK && cfg->precision == PHX_PRECISION_FP32)
+    {
+        float* block = nullptr;
+        s            = devAlloc(c, (void**)&block, emWords * 4);
+        for (int k = 0; k < 6 && s == PHX_OK; ++k)
+            c->em32[k] = block + emOff[k];
+    }
     for (int k = 0; k < 3 && s == PHX_OK; ++k)
     {
         s = alloc(&c->J[k], QJX + k);
@@ -1557,6 +1568,9 @@ phx_status phx_move_ions(phx_ctx* c, double dt, int mode)
         return fail(c, PHX_ERR_ARG, "bad mode");
     if (fastPath(c))
     {
+        // the averaged set may have been set or computed outside the average kernel
+        if (mixed(c))
+            launchEmToF32(c->G, c->em[PHX_EM_AVG], c->em32, c->stream);
         PHX_TRY(enqueueMoveIonsFast(c, dt, mode));
         return checkDeviceError(c);
     }
@@ -1713,6 +1727,8 @@ phx_status phx_profile_push(phx_ctx* c, double dt, int mode, int reps, double* a
     Store const& dom = c->pops[0].arr[PHX_DOMAIN];
     if (mixed(c) && !A.segOn)
         return fail(c, PHX_ERR_STATE, "mixed-precision push on unsorted storage");
+    if (mixed(c))
+        launchEmToF32(c->G, c->em[PHX_EM_AVG], c->em32, c->stream);
     PHX_HIP(c, hipStreamSynchronize(c->stream));
     hipGraph_t g = nullptr;
     hipGraphExec_t ge = nullptr;

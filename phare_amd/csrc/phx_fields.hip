@@ -8,6 +8,8 @@
 // stencils read neighbours straight from HBM through the L1/L2 caches.
 #include "phx_kernels.hpp"
 
+#include <algorithm>
+
 namespace phx
 {
 
@@ -697,9 +699,14 @@ __global__ __launch_bounds__(FB) void k_fill3(Geom G0, int q0, VecPtr F)
 }
 
 // average of E and B sets (algorithm.hpp:68-76) + fillElectricGhosts of Eavg
-template<int DIM>
+struct Mirror
+{
+    float* c[6];
+};
+
+template<int DIM, bool M32>
 __global__ __launch_bounds__(FB) void k_average_fill(Geom G0, CVecPtr E, CVecPtr Ep, CVecPtr B,
-                                                     CVecPtr Bp, VecPtr Ea, VecPtr Ba)
+                                                     CVecPtr Bp, VecPtr Ea, VecPtr Ba, Mirror m32)
 {
     GeomD<DIM> const G(G0);
     long const t = static_cast<long>(blockIdx.x) * FB + threadIdx.x;
@@ -712,13 +719,20 @@ __global__ __launch_bounds__(FB) void k_average_fill(Geom G0, CVecPtr E, CVecPtr
         {
             int s[3] = {i, j, k};
             wrapNode(G, QEX + c, s);
-            int const src = G.idx(QEX + c, s[0], s[1], s[2]);
-            Ea.c[c][G.idx(QEX + c, i, j, k)] = (E.c[c][src] + Ep.c[c][src]) * .5;
+            int const src  = G.idx(QEX + c, s[0], s[1], s[2]);
+            int const n    = G.idx(QEX + c, i, j, k);
+            double const v = (E.c[c][src] + Ep.c[c][src]) * .5;
+            Ea.c[c][n]     = v;
+            if constexpr (M32)
+                m32.c[c][n] = static_cast<float>(v);
         }
         if (inShape(G, QBX + c, i, j, k))
         {
-            int const n = G.idx(QBX + c, i, j, k);
-            Ba.c[c][n]  = (B.c[c][n] + Bp.c[c][n]) * .5;
+            int const n    = G.idx(QBX + c, i, j, k);
+            double const v = (B.c[c][n] + Bp.c[c][n]) * .5;
+            Ba.c[c][n]     = v;
+            if constexpr (M32)
+                m32.c[3 + c][n] = static_cast<float>(v);
         }
     }
 }
@@ -993,14 +1007,52 @@ void launchFill3(Geom const& G, int q0, VecPtr F, hipStream_t s)
     }
 }
 void launchAverageFill(Geom const& G, CVecPtr E, CVecPtr Ep, CVecPtr B, CVecPtr Bp, VecPtr Ea,
-                       VecPtr Ba, hipStream_t s)
+                       VecPtr Ba, hipStream_t s, float* const* mirror)
 {
-    switch (G.dim)
+    Mirror m{};
+    for (int k = 0; k < 6 && mirror; ++k)
+        m.c[k] = mirror[k];
+    dim3 const grid(gridFor(G.size(QMOM)), 3);
+    switch (G.dim * 2 + (mirror ? 1 : 0))
     {
-        case 1: k_average_fill<1><<<dim3(gridFor(G.size(QMOM)), 3), FB, 0, s>>>(G, E, Ep, B, Bp, Ea, Ba); break;
-        case 2: k_average_fill<2><<<dim3(gridFor(G.size(QMOM)), 3), FB, 0, s>>>(G, E, Ep, B, Bp, Ea, Ba); break;
-        case 3: k_average_fill<3><<<dim3(gridFor(G.size(QMOM)), 3), FB, 0, s>>>(G, E, Ep, B, Bp, Ea, Ba); break;
+        case 2: k_average_fill<1, false><<<grid, FB, 0, s>>>(G, E, Ep, B, Bp, Ea, Ba, m); break;
+        case 3: k_average_fill<1, true><<<grid, FB, 0, s>>>(G, E, Ep, B, Bp, Ea, Ba, m); break;
+        case 4: k_average_fill<2, false><<<grid, FB, 0, s>>>(G, E, Ep, B, Bp, Ea, Ba, m); break;
+        case 5: k_average_fill<2, true><<<grid, FB, 0, s>>>(G, E, Ep, B, Bp, Ea, Ba, m); break;
+        case 6: k_average_fill<3, false><<<grid, FB, 0, s>>>(G, E, Ep, B, Bp, Ea, Ba, m); break;
+        case 7: k_average_fill<3, true><<<grid, FB, 0, s>>>(G, E, Ep, B, Bp, Ea, Ba, m); break;
     }
+}
+
+// FP32 copy of the 6 E/B components (one grid row per component)
+__global__ __launch_bounds__(FB) void k_em_to_f32(EMPtrs src, Mirror dst, int n0, int n1, int n2,
+                                                  int n3, int n4, int n5)
+{
+    int const c  = static_cast<int>(blockIdx.y);
+    int const nc = c == 0 ? n0 : (c == 1 ? n1 : (c == 2 ? n2 : (c == 3 ? n3 : (c == 4 ? n4 : n5))));
+    long const t = static_cast<long>(blockIdx.x) * FB + threadIdx.x;
+    if (t >= nc)
+        return;
+    double const* s = c < 3 ? src.E[c] : src.B[c - 3];
+    dst.c[c][t]     = static_cast<float>(s[t]);
+}
+
+void launchEmToF32(Geom const& G, double const* const* src, float* const* dst, hipStream_t s)
+{
+    EMPtrs e{};
+    Mirror m{};
+    int n[6];
+    for (int k = 0; k < 6; ++k)
+    {
+        if (k < 3)
+            e.E[k] = src[k];
+        else
+            e.B[k - 3] = src[k];
+        m.c[k] = dst[k];
+        n[k]   = G.size(k < 3 ? QEX + k : QBX + k - 3);
+    }
+    int const mx = *std::max_element(n, n + 6);
+    k_em_to_f32<<<dim3(gridFor(mx), 6), FB, 0, s>>>(e, m, n[0], n[1], n[2], n[3], n[4], n[5]);
 }
 void launchFoldTotals(Geom const& G, MomSetArgs const& A, double* trho, double* tmd, VecPtr tvi,
                       hipStream_t s)

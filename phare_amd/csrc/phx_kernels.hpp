@@ -115,6 +115,7 @@ struct PushArgs
     int segOn; // the array is cell-segmented (sg, st valid)
     SegGeom sg;
     SegTables st;
+    float const* em32[6]; // mixed-precision family: FP32 copy of the E/B set (Ex..Bz)
     double charge; // the population's charge (mixed-precision family; FP64 family with rhoFromN)
     int rhoFromN;  // FP64 segmented push: every particle has charge `charge`, so the LDS tile
                    // leaves the charge density out and its flush takes charge x n
@@ -206,8 +207,12 @@ struct MomSetArgs
 void launchFaradayFill(Geom const& G, CVecPtr B, CVecPtr E, VecPtr Bnew, double dt, hipStream_t s);
 void launchAmpereFill(Geom const& G, CVecPtr B, VecPtr J, hipStream_t s);
 void launchFill3(Geom const& G, int q0, VecPtr F, hipStream_t s);
+// mirror (nullable): 6 FP32 arrays (Ex..Bz) that also receive the averaged set, for the
+// mixed-precision push's gather
 void launchAverageFill(Geom const& G, CVecPtr E, CVecPtr Ep, CVecPtr B, CVecPtr Bp, VecPtr Ea,
-                       VecPtr Ba, hipStream_t s);
+                       VecPtr Ba, hipStream_t s, float* const* mirror = nullptr);
+// FP32 copy of an E/B set (6 components, Ex..Bz)
+void launchEmToF32(Geom const& G, double const* const* src, float* const* dst, hipStream_t s);
 void launchFoldTotals(Geom const& G, MomSetArgs const& A, double* trho, double* tmd, VecPtr tvi,
                       hipStream_t s);
 void launchMax5(Geom const& G, double const* const* src, double* const* dst, hipStream_t s);

@@ -17,8 +17,9 @@
 //    is several times faster than an FP32 one (ds_add_f64 / ds_add_f32), and FP64 sums of FP32
 //    terms are more accurate than compensated FP32 sums (a TwoSum-compensated FP32 tile, whose
 //    adds must return the old value, measured slower still);
-//  - E and B are read as FP64 and interpolated in FP32; in 1D and 2D a component's stencil values
-//    are loaded together before the sums, so that their loads are in flight at once.
+//  - E and B are gathered from an FP32 copy of the averaged set (written beside it by the average
+//    kernel) and interpolated in FP32; in 1D and 2D a component's stencil values are loaded
+//    together before the sums, so that their loads are in flight at once.
 #include "phx_push_common.hpp"
 
 namespace phx
@@ -113,9 +114,9 @@ __device__ __forceinline__ bool stencilInside32(Geom const& G, Stencil32<D, O> c
     return true;
 }
 
-// FP64 field, FP32 weights and sums, in MeshToParticle's order of sums
+// FP32 copy of a field, FP32 weights and sums, in MeshToParticle's order of sums
 template<int D, int O>
-__device__ __forceinline__ float interp32(Geom const& G, double const* __restrict__ f, int q,
+__device__ __forceinline__ float interp32(Geom const& G, float const* __restrict__ f, int q,
                                           Stencil32<D, O> const& s)
 {
     constexpr int S = O + 1;
@@ -132,18 +133,18 @@ __device__ __forceinline__ float interp32(Geom const& G, double const* __restric
     float r      = 0.f;
     if constexpr (D == 1)
     {
-        double vals[S];
+        float vals[S];
 #pragma unroll
         for (int i = 0; i < S; ++i)
             vals[i] = f[st[0] + i];
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int i = 0; i < S; ++i)
-            r = __builtin_fmaf(static_cast<float>(vals[i]), w[0][i], r);
+            r = __builtin_fmaf(vals[i], w[0][i], r);
     }
     else if constexpr (D == 2)
     {
-        double vals[S][S];
+        float vals[S][S];
 #pragma unroll
         for (int i = 0; i < S; ++i)
 #pragma unroll
@@ -156,7 +157,7 @@ __device__ __forceinline__ float interp32(Geom const& G, double const* __restric
             float yi = 0.f;
 #pragma unroll
             for (int j = 0; j < S; ++j)
-                yi = __builtin_fmaf(static_cast<float>(vals[i][j]), w[1][j], yi);
+                yi = __builtin_fmaf(vals[i][j], w[1][j], yi);
             r = __builtin_fmaf(yi, w[0][i], r);
         }
     }
@@ -169,11 +170,11 @@ __device__ __forceinline__ float interp32(Geom const& G, double const* __restric
 #pragma unroll
             for (int j = 0; j < S; ++j)
             {
-                double const* pencil = f + ((st[0] + i) * s1 + st[1] + j) * s2 + st[2];
-                float zi             = 0.f;
+                float const* pencil = f + ((st[0] + i) * s1 + st[1] + j) * s2 + st[2];
+                float zi            = 0.f;
 #pragma unroll
                 for (int k = 0; k < S; ++k)
-                    zi = __builtin_fmaf(static_cast<float>(pencil[k]), w[2][k], zi);
+                    zi = __builtin_fmaf(pencil[k], w[2][k], zi);
                 yi = __builtin_fmaf(zi, w[1][j], yi);
             }
             r = __builtin_fmaf(yi, w[0][i], r);
@@ -398,12 +399,12 @@ __device__ __forceinline__ void pushSlot32(PushArgs const& A, Push32 const& K, P
                 return;
             }
             float E[3], B[3];
-            E[0] = interp32<D, O>(G, A.em.E[0], QEX, s);
-            E[1] = interp32<D, O>(G, A.em.E[1], QEY, s);
-            E[2] = interp32<D, O>(G, A.em.E[2], QEZ, s);
-            B[0] = interp32<D, O>(G, A.em.B[0], QBX, s);
-            B[1] = interp32<D, O>(G, A.em.B[1], QBY, s);
-            B[2] = interp32<D, O>(G, A.em.B[2], QBZ, s);
+            E[0] = interp32<D, O>(G, A.em32[0], QEX, s);
+            E[1] = interp32<D, O>(G, A.em32[1], QEY, s);
+            E[2] = interp32<D, O>(G, A.em32[2], QEZ, s);
+            B[0] = interp32<D, O>(G, A.em32[3], QBX, s);
+            B[1] = interp32<D, O>(G, A.em32[4], QBY, s);
+            B[2] = interp32<D, O>(G, A.em32[5], QBZ, s);
             borisAccelerate32(vel, E, B, K.coef1);
             if (!advancePosition32<D>(cell, delta, vel, K.hdtdx, bd, bv))
             {
