@@ -8,9 +8,14 @@ sheet, 256x128 cells, 100 particles per cell (3.28M particles), interpolation or
 FP64.
 
 Prints one JSON line: particle-steps/s, the roofline of the dominant kernel (the fused
-push+gather+deposit kernel, algorithmic bytes per launch / its mean duration from HIP events)
-and the CPU baseline (the FP64 oracle restatement of the reference method, 1 thread, timed on
-a bounded sample of the same workload).
+push+gather+deposit kernel, algorithmic bytes per launch / its mean duration over the timed
+steps), the same for the mixed-precision family, and the CPU baseline (the FP64 oracle
+restatement of the reference method, 1 thread, timed on a bounded sample of the same workload).
+
+The push's duration is taken inside the timed, graph-replayed steps: each launch stamps the
+device wall clock at its first workgroup's start and its last workgroup's end (a HIP event
+recorded inside a captured step would keep only the last replay's time); rocprofv3's kernel
+trace of the same command agrees (profiles/).
 """
 import argparse
 import ctypes
@@ -108,26 +113,31 @@ def step_bytes(wl, precision="fp64"):
     return wl.n_particles * (2 * R + M) + wl.n_cells * 8 * (189 + 20 * len(wl.masses))
 
 
-def run_family(wl, precision, steps, warmup, reps):
-    """steps/s of one precision family on the workload, and its push kernel's time"""
+def run_family(wl, precision, steps, warmup):
+    """steps/s of one precision family on the workload, and its push kernel's time measured
+    inside the timed steps"""
     prec = phx.PRECISION_FP32 if precision == "fp32" else phx.PRECISION_FP64
     lvl = phx.Level(wl.dim, wl.order, wl.ncells, wl.dx, masses=wl.masses, Te=wl.Te,
                     capacity=wl.n_particles, precision=prec)
     wl.setup(lvl, phx.F_B)
     diag0 = lvl.diagnostics()
+    # the pushes of the captured steps time themselves (first workgroup start to last workgroup
+    # end, device wall clock); enabled before the warm-up so that the timed steps replay the
+    # step captured with it, and reset right before the timed steps
+    lvl.set_push_timing(True)
     lvl.advance(wl.dt, warmup)
     lvl.sync()
     device_sync()
+    lvl.set_push_timing(True)
     t0 = time.perf_counter()
     lvl.advance(wl.dt, steps)
     lvl.sync()
     device_sync()
     elapsed = time.perf_counter() - t0
+    ms_all, n_all = lvl.push_timing(phx.MODE_ALL)
+    ms_dom, n_dom = lvl.push_timing(phx.MODE_DOMAIN_ONLY)
+    assert n_all == steps and n_dom == steps, (n_all, n_dom, steps)
     diag1 = lvl.diagnostics()
-    # dominant kernel: the fused push + gather + deposit, captured alone in a graph and replayed
-    # back to back between two HIP events on the level's stream
-    ms_all = lvl.profile_push(wl.dt, phx.MODE_ALL, reps)
-    ms_dom = lvl.profile_push(wl.dt, phx.MODE_DOMAIN_ONLY, reps)
     lvl.close()
     e0 = diag0["kinetic"] + diag0["magnetic"]
     e1 = diag1["kinetic"] + diag1["magnetic"]
@@ -165,7 +175,6 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision family")
-    ap.add_argument("--kernel-reps", type=int, default=20)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES (counter calibration)")
     args = ap.parse_args()
@@ -178,7 +187,7 @@ def main():
         sys.exit(2)
 
     wl = workload(args.workload)
-    f64 = run_family(wl, "fp64", args.steps, args.warmup, args.kernel_reps)
+    f64 = run_family(wl, "fp64", args.steps, args.warmup)
     bytes_all = push_bytes(wl, phx.MODE_ALL)
     achieved = bytes_all / (f64["ms_all"] * 1e-3) / 1e9
     out = {
@@ -211,7 +220,7 @@ def main():
     }
     if not args.no_mixed:
         # the mixed-precision family on the same workload (FP32 particles, FP64 fields)
-        f32 = run_family(wl, "fp32", args.steps, args.warmup, args.kernel_reps)
+        f32 = run_family(wl, "fp32", args.steps, args.warmup)
         b32 = push_bytes(wl, phx.MODE_ALL, "fp32")
         a32 = b32 / (f32["ms_all"] * 1e-3) / 1e9
         out["mixed_fp32"] = {

@@ -190,6 +190,13 @@ void* phx_stream(phx_ctx* ctx);
    on the context stream, and returns the mean kernel time in milliseconds. Mode PHX_MODE_ALL
    advances the particles in place. */
 phx_status phx_profile_push(phx_ctx* ctx, double dt, int mode, int reps, double* avg_ms);
+/* in-kernel timing of the fused pushes inside the steps phx_advance / phx_move_ions run on a
+   periodic level: each launch's span from its first workgroup's start to its last workgroup's
+   end (device wall clock), summed per population and mode. on = 1 enables and resets the sums
+   (a change of the setting re-captures the step once); phx_push_timing returns the mean launch
+   time in milliseconds and the number of launches summed. */
+phx_status phx_set_push_timing(phx_ctx* ctx, int on);
+phx_status phx_push_timing(phx_ctx* ctx, int pop, int mode, double* avg_ms, int* launches);
 
 #ifdef __cplusplus
 }

@@ -136,6 +136,9 @@ struct phx_ctx
     };
     GraphEntry graphs[2];
     int graphNext = 0;
+    // in-kernel timing of the pushes of the fast path, per (mode, population)
+    KTime* kt   = nullptr;
+    bool timing = false;
     std::string lastError;
 };
 
@@ -495,6 +498,7 @@ phx_status enqueueMoveIonsFast(phx_ctx* c, double dt, int mode)
         for (int m = 0; m < 5; ++m)
             A.mom[m] = c->raw[cur][p][m];
         A.periodicWrap = 1;
+        A.kt           = c->timing ? c->kt + (mode == PHX_MODE_ALL ? PHX_MAX_POP : 0) + p : nullptr;
         Store const& dom = c->pops[p].arr[PHX_DOMAIN];
         if (mixed(c))
         {
@@ -870,6 +874,7 @@ std::vector<unsigned long long> storageKey(phx_ctx const* c)
         std::memcpy(&qb, &q, 8);
         k.push_back(qb);
         k.push_back(c->pops[p].uniformCharge ? 1ull : 0ull);
+        k.push_back(c->timing ? 1ull : 0ull);
         for (Store const& s : c->pops[p].arr)
         {
             // segmented storage: the kernels read their extent from the segment tables, so a
@@ -1710,6 +1715,43 @@ phx_status phx_diagnostics(phx_ctx* c, double out[4])
     out[1] = 0.5 * mag * vol;
     out[2] = charge * vol;
     out[3] = cnt;
+    return PHX_OK;
+}
+
+phx_status phx_set_push_timing(phx_ctx* c, int on)
+{
+    if (!c)
+        return PHX_ERR_ARG;
+    PHX_HIP(c, hipStreamSynchronize(c->stream));
+    if (on && !c->kt)
+        PHX_TRY(devAlloc(c, (void**)&c->kt, sizeof(KTime) * 2 * PHX_MAX_POP));
+    if (on)
+    {
+        KTime z[2 * PHX_MAX_POP];
+        for (auto& k : z)
+            k = KTime{~0ull, 0ull, 0ull, 0u, 0u};
+        PHX_HIP(c, hipMemcpy(c->kt, z, sizeof(z), hipMemcpyHostToDevice));
+    }
+    c->timing = on != 0;
+    return PHX_OK;
+}
+
+phx_status phx_push_timing(phx_ctx* c, int pop, int mode, double* avg_ms, int* launches)
+{
+    if (!c || !avg_ms || !launches || pop < 0 || pop >= c->cfg.n_pop
+        || (mode != PHX_MODE_DOMAIN_ONLY && mode != PHX_MODE_ALL))
+        return PHX_ERR_ARG;
+    if (!c->kt)
+        return fail(c, PHX_ERR_STATE, "push timing was never enabled");
+    PHX_HIP(c, hipStreamSynchronize(c->stream));
+    KTime k{};
+    PHX_HIP(c, hipMemcpy(&k, c->kt + (mode == PHX_MODE_ALL ? PHX_MAX_POP : 0) + pop, sizeof(k),
+                         hipMemcpyDeviceToHost));
+    int dev = 0, khz = 0;
+    PHX_HIP(c, hipGetDevice(&dev));
+    PHX_HIP(c, hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+    *launches = static_cast<int>(k.count);
+    *avg_ms   = k.count && khz ? static_cast<double>(k.total) / k.count / khz : 0.;
     return PHX_OK;
 }
 

@@ -96,6 +96,18 @@ struct SegTables
     unsigned int itemsCap;    // work items the storage's capacity allows (grid of the push)
 };
 
+// in-kernel timing of one push launch site (a node of the captured step): the earliest
+// workgroup start and the latest workgroup end of a launch, in wall-clock ticks; the launch's last
+// workgroup adds end - start to the total and re-arms the record for the next launch
+struct KTime
+{
+    unsigned long long start; // ~0 between launches
+    unsigned long long end;
+    unsigned long long total;
+    unsigned int done;  // workgroups of the current launch that have finished
+    unsigned int count; // launches in total
+};
+
 struct PushArgs
 {
     Geom G;
@@ -116,6 +128,7 @@ struct PushArgs
     SegGeom sg;
     SegTables st;
     float const* em32[6]; // mixed-precision family: FP32 copy of the E/B set (Ex..Bz)
+    KTime* kt;            // timing record of this launch site (null: not timed)
     double charge; // the population's charge (mixed-precision family; FP64 family with rhoFromN)
     int rhoFromN;  // FP64 segmented push: every particle has charge `charge`, so the LDS tile
                    // leaves the charge density out and its flush takes charge x n

@@ -685,8 +685,10 @@ void k_push_seg(PushArgs A, PView in)
     using C = Cfg<D, O>;
     extern __shared__ double lds[];
     unsigned int const item = blockIdx.x;
-    if (item >= *A.st.nItems)
+    unsigned int const nItems = *A.st.nItems;
+    if (item >= nItems)
         return;
+    pc::ktBegin(A.kt);
     Geom const& G          = A.G;
     SegGeom const& S       = A.sg;
     unsigned int const seg = A.st.items[2 * item];
@@ -732,6 +734,11 @@ void k_push_seg(PushArgs A, PView in)
         accFlush<D, O>(acc, G, T, copy, A.mom);
     __syncthreads();
     flushTile<D, RN>(G, T, A.tile.nodes, A.mom, A.charge);
+    if (A.kt)
+    {
+        __syncthreads();
+        pc::ktEnd(A.kt, nItems);
+    }
 }
 
 // ---------------------------------------------------------------------------------------------

@@ -503,8 +503,10 @@ void k_push_seg32(PushArgs A, PView32 v)
 {
     extern __shared__ double lds32[];
     unsigned int const item = blockIdx.x;
-    if (item >= *A.st.nItems)
+    unsigned int const nItems = *A.st.nItems;
+    if (item >= nItems)
         return;
+    pc::ktBegin(A.kt);
     Geom const& G          = A.G;
     unsigned int const seg = A.st.items[2 * item];
     unsigned int const r0  = A.st.items[2 * item + 1];
@@ -569,6 +571,11 @@ void k_push_seg32(PushArgs A, PView32 v)
                     atomicAdd(A.mom[m + 1] + node, val);
             }
         }
+    }
+    if (A.kt)
+    {
+        __syncthreads();
+        pc::ktEnd(A.kt, nItems);
     }
 }
 

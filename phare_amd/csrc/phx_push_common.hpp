@@ -42,6 +42,29 @@ __device__ __forceinline__ bool inNLGB(int const* c, Boxes const& b, int D)
     return false;
 }
 
+// timing record of a launch (KTime): workgroup 0 (dispatched first) stamps the start; every
+// workgroup with work counts itself out at its end, and the last of the `working` ones stamps
+// the end, accumulates the launch and re-arms the record. One relaxed atomic per workgroup and
+// no fences (a device-scope fence writes back the L2: far too slow to pay per workgroup); the
+// stamps are exact to the spread of the first workgroups' dispatch.
+__device__ __forceinline__ void ktBegin(KTime* kt)
+{
+    if (kt && blockIdx.x == 0 && threadIdx.x == 0)
+        atomicExch(&kt->start, static_cast<unsigned long long>(wall_clock64()));
+}
+__device__ __forceinline__ void ktEnd(KTime* kt, unsigned int working)
+{
+    if (!kt || threadIdx.x != 0)
+        return;
+    if (atomicAdd(&kt->done, 1u) != working - 1)
+        return;
+    unsigned long long const e = static_cast<unsigned long long>(wall_clock64());
+    unsigned long long const s = atomicAdd(&kt->start, 0ull);
+    atomicAdd(&kt->total, e - s);
+    atomicAdd(&kt->count, 1u);
+    atomicExch(&kt->done, 0u);
+}
+
 // local node index of the LDS tile origin of segment `seg` (the segment's first cell less the
 // tile's low margin); the overflow segment has no cell range and gets an unreachable origin
 template<int D>

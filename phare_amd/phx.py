@@ -87,6 +87,8 @@ def lib():
         L.phx_advance.argtypes = [vp, C.c_double, C.c_int]
         L.phx_diagnostics.argtypes = [vp, _dp]
         L.phx_profile_push.argtypes = [vp, C.c_double, C.c_int, C.c_int, C.POINTER(C.c_double)]
+        L.phx_set_push_timing.argtypes = [vp, C.c_int]
+        L.phx_push_timing.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int)]
         _lib = L
     return _lib
 
@@ -98,7 +100,8 @@ EXPORTED = ["phx_create", "phx_destroy", "phx_last_error", "phx_synchronize", "p
             "phx_fill_moment_borders", "phx_fill_ion_borders", "phx_fill_ion_ghost_particles",
             "phx_fill_ghosts", "phx_faraday", "phx_ampere", "phx_electrons_update", "phx_ohm",
             "phx_average", "phx_move_ions", "phx_initialize", "phx_advance", "phx_diagnostics",
-            "phx_stream", "phx_profile_push", "phx_sort_particles", "phx_set_sort_interval"]
+            "phx_stream", "phx_profile_push", "phx_sort_particles", "phx_set_sort_interval",
+            "phx_set_push_timing", "phx_push_timing"]
 
 
 class Level:
@@ -251,6 +254,14 @@ class Level:
         ms = C.c_double(0.0)
         self._rc(lib().phx_profile_push(self.h, dt, mode, reps, C.byref(ms)))
         return ms.value
+
+    def set_push_timing(self, on):
+        self._rc(lib().phx_set_push_timing(self.h, int(on)))
+
+    def push_timing(self, mode, pop=0):
+        ms, n = C.c_double(0.0), C.c_int(0)
+        self._rc(lib().phx_push_timing(self.h, pop, mode, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
 
     def diagnostics(self):
         out = np.zeros(4)
