@@ -487,7 +487,9 @@ bool fastPath(phx_ctx* c)
 // enqueue the particle stage of SolverPPC::moveIons_ on the fast path (no host sync): the push
 // deposits into the current raw set, then one kernel border-sums every population moment,
 // clears the other raw set and computes the totals, and one kernel applies their border max
-phx_status enqueueMoveIonsFast(phx_ctx* c, double dt, int mode)
+// borderMax false: the border max of the totals is left to the next electron closure
+// (launchElectronsMax), which a captured step runs before anything reads the totals
+phx_status enqueueMoveIonsFast(phx_ctx* c, double dt, int mode, bool borderMax = true)
 {
     int const cur = c->rawCur;
     MomSetArgs M{};
@@ -518,9 +520,12 @@ phx_status enqueueMoveIonsFast(phx_ctx* c, double dt, int mode)
     }
     PHX_TRY(checkLaunch(c, "push"));
     launchFoldTotals(c->G, M, c->tot5[0], c->tot5[1], vec(c->tot5 + 2), c->stream);
-    double const* src[5] = {c->tot5[0], c->tot5[1], c->tot5[2], c->tot5[3], c->tot5[4]};
-    double* dst[5]       = {c->rho, c->massD, c->Vi[0], c->Vi[1], c->Vi[2]};
-    launchMax5(c->G, src, dst, c->stream);
+    if (borderMax)
+    {
+        double const* src[5] = {c->tot5[0], c->tot5[1], c->tot5[2], c->tot5[3], c->tot5[4]};
+        double* dst[5]       = {c->rho, c->massD, c->Vi[0], c->Vi[1], c->Vi[2]};
+        launchMax5(c->G, src, dst, c->stream);
+    }
     c->rawCur = 1 - cur;
     return checkLaunch(c, "moment fills");
 }
@@ -582,15 +587,56 @@ void enqueueCorrector(phx_ctx* c, double dt)
               c->Pe, cvec(c->em[PHX_EM] + 3), cvec(c->J), vec(c->em[PHX_EM]), c->stream, 1);
 }
 
+// the electron closure, with the border max of the totals first when the last moveIons left it
+void electronsAfterFold(phx_ctx* c, bool borderMax)
+{
+    if (!borderMax)
+    {
+        electrons(c);
+        return;
+    }
+    double const* src[5] = {c->tot5[0], c->tot5[1], c->tot5[2], c->tot5[3], c->tot5[4]};
+    double* dst[5]       = {c->rho, c->massD, c->Vi[0], c->Vi[1], c->Vi[2]};
+    launchElectronsMax(c->G, c->cfg.Te, cvec(c->J), src, dst, vec(c->Ve), c->Pe, c->stream);
+}
+
+// predictor + average of a periodic level: Faraday and Ampere with their ghost layers, the
+// electron closure (with the pending border max), then Ohm and the average in one kernel
+void enqueuePredictorAverageFast(phx_ctx* c, double dt, int eset, bool borderMax)
+{
+    launchFaradayFill(c->G, cvec(c->em[PHX_EM] + 3), cvec(c->em[eset]), vec(c->em[PHX_EM_PRED] + 3), dt,
+                      c->stream);
+    launchAmpereFill(c->G, cvec(c->em[PHX_EM_PRED] + 3), vec(c->J), c->stream);
+    electronsAfterFold(c, borderMax);
+    launchOhmAverage(c->G, c->cfg.eta, c->cfg.nu, c->cfg.hyper_mode, c->patch.level, c->rho,
+                     cvec(c->Ve), c->Pe, cvec(c->em[PHX_EM_PRED] + 3), cvec(c->J),
+                     vec(c->em[PHX_EM_PRED]), cvec(c->em[PHX_EM]), cvec(c->em[PHX_EM] + 3),
+                     vec(c->em[PHX_EM_AVG]), vec(c->em[PHX_EM_AVG] + 3), c->stream,
+                     mixed(c) ? c->em32 : nullptr);
+}
+
+// SolverPPC::corrector_ of a periodic level: in-place Faraday and its B fill, Ampere, the
+// electron closure with the pending border max, Ohm with its ghost layers
+void enqueueCorrectorFast(phx_ctx* c, double dt)
+{
+    launchFaraday(c->G, cvec(c->em[PHX_EM] + 3), cvec(c->em[PHX_EM_AVG]), vec(c->em[PHX_EM] + 3), dt,
+                  c->stream);
+    launchFill3(c->G, QBX, vec(c->em[PHX_EM] + 3), c->stream);
+    launchAmpereFill(c->G, cvec(c->em[PHX_EM] + 3), vec(c->J), c->stream);
+    electronsAfterFold(c, true);
+    launchOhm(c->G, c->cfg.eta, c->cfg.nu, c->cfg.hyper_mode, c->patch.level, c->rho, cvec(c->Ve),
+              c->Pe, cvec(c->em[PHX_EM] + 3), cvec(c->J), vec(c->em[PHX_EM]), c->stream, 1);
+}
+
+// one PPC step of a periodic level, as captured: each moveIons leaves the border max of the
+// totals to the electron closure that follows it (the first reader of the totals)
 phx_status enqueueAdvanceFast(phx_ctx* c, double dt)
 {
-    enqueuePredictor(c, dt, PHX_EM);
-    enqueueAverage(c);
-    PHX_TRY(enqueueMoveIonsFast(c, dt, PHX_MODE_DOMAIN_ONLY));
-    enqueuePredictor(c, dt, PHX_EM_AVG);
-    enqueueAverage(c);
-    PHX_TRY(enqueueMoveIonsFast(c, dt, PHX_MODE_ALL));
-    enqueueCorrector(c, dt);
+    enqueuePredictorAverageFast(c, dt, PHX_EM, false);
+    PHX_TRY(enqueueMoveIonsFast(c, dt, PHX_MODE_DOMAIN_ONLY, false));
+    enqueuePredictorAverageFast(c, dt, PHX_EM_AVG, true);
+    PHX_TRY(enqueueMoveIonsFast(c, dt, PHX_MODE_ALL, false));
+    enqueueCorrectorFast(c, dt);
     return checkLaunch(c, "advance");
 }
 

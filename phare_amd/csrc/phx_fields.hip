@@ -430,6 +430,63 @@ __global__ __launch_bounds__(FB) void k_ohm(Geom G0, double eta, double nu, int 
     E.c[c][G.idx(QEX + c, i0, j0, k0)] = ohmAt(G, c, i, j, k, eta, nu, hyper, lvlCoef, nn, ve, pe, B, J);
 }
 
+struct Mirror
+{
+    float* c[6];
+};
+
+// Ohm of a periodic level's predictor and SolverPPC::average_ (solver_ppc.hpp:483-510) in one
+// launch: Ohm is evaluated at every E node (a ghost node at its periodic source) and stored as
+// Epred on the physical nodes; Eavg = (E + Epred) / 2 at every node from its source node and
+// Bavg = (B + Bpred) / 2, as k_ohm followed by k_average_fill computes them. M32: the averaged
+// set is also written in FP32 (the mixed-precision push gathers from it). Every load precedes
+// the first store.
+template<int DIM, bool M32>
+__global__ __launch_bounds__(FB) void k_ohm_avg(Geom G0, double eta, double nu, int hyper,
+                                                double lvlCoef, double const* nn, CVecPtr Ve,
+                                                double const* Pe, CVecPtr Bp, CVecPtr J, VecPtr Ep,
+                                                CVecPtr E, CVecPtr B, VecPtr Ea, VecPtr Ba,
+                                                Mirror m32)
+{
+    GeomD<DIM> const G(G0);
+    long const t = static_cast<long>(blockIdx.x) * FB + threadIdx.x;
+    int i0, j0, k0;
+    if (!nodeOf(G, QMOM, t, i0, j0, k0))
+        return;
+    int const c = static_cast<int>(blockIdx.y); // one component per grid row
+    int i, j, k;
+    bool const onE = ohmNode(G, c, 1, i0, j0, k0, i, j, k);
+    bool const onB = i0 < G.shape[QBX + c][0] && j0 < G.shape[QBX + c][1] && k0 < G.shape[QBX + c][2];
+    double ev = 0., eo = 0., bs = 0.;
+    if (onE)
+    {
+        auto ve = [&](int d, int a, int b, int cc) { return Ve.c[d][G.idx(QMOM, a, b, cc)]; };
+        auto pe = [&](int a, int b, int cc) { return Pe[G.idx(QMOM, a, b, cc)]; };
+        ev = ohmAt(G, c, i, j, k, eta, nu, hyper, lvlCoef, nn, ve, pe, Bp, J);
+        eo = E.c[c][G.idx(QEX + c, i, j, k)];
+    }
+    int const nb = onB ? G.idx(QBX + c, i0, j0, k0) : 0;
+    if (onB)
+        bs = B.c[c][nb] + Bp.c[c][nb];
+    if (onB)
+    {
+        double const v = bs * .5;
+        Ba.c[c][nb]    = v;
+        if constexpr (M32)
+            m32.c[3 + c][nb] = static_cast<float>(v);
+    }
+    if (onE)
+    {
+        int const n = G.idx(QEX + c, i0, j0, k0);
+        if (i == i0 && j == j0 && k == k0)
+            Ep.c[c][n] = ev;
+        double const v = (eo + ev) * .5;
+        Ea.c[c][n]     = v;
+        if constexpr (M32)
+            m32.c[c][n] = static_cast<float>(v);
+    }
+}
+
 // ---- elementwise ------------------------------------------------------------------------------
 __global__ __launch_bounds__(FB) void k_average(double const* a, double const* b, double* o, long n)
 {
@@ -699,11 +756,6 @@ __global__ __launch_bounds__(FB) void k_fill3(Geom G0, int q0, VecPtr F)
 }
 
 // average of E and B sets (algorithm.hpp:68-76) + fillElectricGhosts of Eavg
-struct Mirror
-{
-    float* c[6];
-};
-
 template<int DIM, bool M32>
 __global__ __launch_bounds__(FB) void k_average_fill(Geom G0, CVecPtr E, CVecPtr Ep, CVecPtr B,
                                                      CVecPtr Bp, VecPtr Ea, VecPtr Ba, Mirror m32)
@@ -891,6 +943,58 @@ __global__ __launch_bounds__(FB) void k_max5(Geom G0, Five F)
         F.dst[q][n] = v[q];
 }
 
+// fillIonBorders and Electrons::update in one launch (periodic level): the node's five totals
+// are the max over its periodic images of the values before the max (F.src), taken as k_max5
+// takes them and stored in F.dst (rho, massDensity, Vi); the closure then uses them as
+// k_electrons does (Pe everywhere, Ve on the physical box)
+template<int DIM>
+__global__ __launch_bounds__(FB) void k_electrons_max(Geom G0, double Te, CVecPtr J, Five F,
+                                                      VecPtr Ve, double* Pe)
+{
+    GeomD<DIM> const G(G0);
+    long const t = static_cast<long>(blockIdx.x) * FB + threadIdx.x;
+    int i, j, k;
+    if (!nodeOf(G, QMOM, t, i, j, k))
+        return;
+    int const c[3] = {i, j, k};
+    int const n    = G.idx(QMOM, i, j, k);
+    Images<DIM> I;
+    imagesOf(G, c, I);
+    double v[5];
+#pragma unroll
+    for (int q = 0; q < 5; ++q)
+    {
+        v[q] = F.src[q][n];
+#pragma unroll
+        for (int x = 0; x < Images<DIM>::N; ++x)
+        {
+            double const o = F.src[q][I.valid[x] ? n - I.off[x] : n];
+            v[q]           = I.valid[x] && v[q] < o ? o : v[q];
+        }
+    }
+    int lo[3], hi[3];
+    physBox(G, QMOM, lo, hi);
+    bool const phys = inRange(G, i, j, k, lo, hi);
+    double jx = 0., jy = 0., jz = 0.;
+    if (phys)
+    {
+        // JxToMoments / JyToMoments / JzToMoments: dual->primal along the component's direction
+        jx = project(G, J.c[0], QJX, i, j, k, D2P, PN, PN);
+        jy = project(G, J.c[1], QJY, i, j, k, PN, D2P, PN);
+        jz = project(G, J.c[2], QJZ, i, j, k, PN, PN, D2P);
+    }
+#pragma unroll
+    for (int q = 0; q < 5; ++q)
+        F.dst[q][n] = v[q];
+    Pe[n] = v[0] * Te;
+    if (phys)
+    {
+        Ve.c[0][n] = v[2] - jx / v[0];
+        Ve.c[1][n] = v[3] - jy / v[0];
+        Ve.c[2][n] = v[4] - jz / v[0];
+    }
+}
+
 unsigned int gridFor(long n) { return static_cast<unsigned int>((n + FB - 1) / FB); }
 
 } // namespace
@@ -937,6 +1041,29 @@ void launchOhm(Geom const& G, double eta, double nu, int hyper, int level, doubl
         case 3: k_ohm<3><<<dim3(gridFor(G.size(QMOM)), 3), FB, 0, s>>>(G, eta, nu, hyper, lvlCoef, n, Ve, Pe, B, J, E, fill); break;
     }
 }
+void launchOhmAverage(Geom const& G, double eta, double nu, int hyper, int level, double const* n,
+                      CVecPtr Ve, double const* Pe, CVecPtr Bp, CVecPtr J, VecPtr Ep, CVecPtr E,
+                      CVecPtr B, VecPtr Ea, VecPtr Ba, hipStream_t s, float* const* mirror)
+{
+    double lvlCoef = 1.;
+    for (int l = 0; l < level; ++l)
+        lvlCoef *= 4.;
+    lvlCoef = 1. / lvlCoef;
+    Mirror m{};
+    for (int k = 0; k < 6 && mirror; ++k)
+        m.c[k] = mirror[k];
+    dim3 const grid(gridFor(G.size(QMOM)), 3);
+    switch (G.dim * 2 + (mirror ? 1 : 0))
+    {
+        case 2: k_ohm_avg<1, false><<<grid, FB, 0, s>>>(G, eta, nu, hyper, lvlCoef, n, Ve, Pe, Bp, J, Ep, E, B, Ea, Ba, m); break;
+        case 3: k_ohm_avg<1, true><<<grid, FB, 0, s>>>(G, eta, nu, hyper, lvlCoef, n, Ve, Pe, Bp, J, Ep, E, B, Ea, Ba, m); break;
+        case 4: k_ohm_avg<2, false><<<grid, FB, 0, s>>>(G, eta, nu, hyper, lvlCoef, n, Ve, Pe, Bp, J, Ep, E, B, Ea, Ba, m); break;
+        case 5: k_ohm_avg<2, true><<<grid, FB, 0, s>>>(G, eta, nu, hyper, lvlCoef, n, Ve, Pe, Bp, J, Ep, E, B, Ea, Ba, m); break;
+        case 6: k_ohm_avg<3, false><<<grid, FB, 0, s>>>(G, eta, nu, hyper, lvlCoef, n, Ve, Pe, Bp, J, Ep, E, B, Ea, Ba, m); break;
+        case 7: k_ohm_avg<3, true><<<grid, FB, 0, s>>>(G, eta, nu, hyper, lvlCoef, n, Ve, Pe, Bp, J, Ep, E, B, Ea, Ba, m); break;
+    }
+}
+
 void launchAverage(double const* a, double const* b, double* out, long count, hipStream_t s)
 {
     k_average<<<gridFor(count), FB, 0, s>>>(a, b, out, count);
@@ -1074,6 +1201,23 @@ void launchFoldTotals(Geom const& G, MomSetArgs const& A, double* trho, double* 
         case 1: k_fold_totals<1><<<gridFor(G.size(QMOM)), FB, 0, s>>>(G, M, trho, tmd, tvi); break;
         case 2: k_fold_totals<2><<<gridFor(G.size(QMOM)), FB, 0, s>>>(G, M, trho, tmd, tvi); break;
         case 3: k_fold_totals<3><<<gridFor(G.size(QMOM)), FB, 0, s>>>(G, M, trho, tmd, tvi); break;
+    }
+}
+
+void launchElectronsMax(Geom const& G, double Te, CVecPtr J, double const* const* src,
+                       double* const* dst, VecPtr Ve, double* Pe, hipStream_t s)
+{
+    Five F{};
+    for (int q = 0; q < 5; ++q)
+    {
+        F.src[q] = src[q];
+        F.dst[q] = dst[q];
+    }
+    switch (G.dim)
+    {
+        case 1: k_electrons_max<1><<<gridFor(G.size(QMOM)), FB, 0, s>>>(G, Te, J, F, Ve, Pe); break;
+        case 2: k_electrons_max<2><<<gridFor(G.size(QMOM)), FB, 0, s>>>(G, Te, J, F, Ve, Pe); break;
+        case 3: k_electrons_max<3><<<gridFor(G.size(QMOM)), FB, 0, s>>>(G, Te, J, F, Ve, Pe); break;
     }
 }
 

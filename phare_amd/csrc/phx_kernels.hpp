@@ -192,6 +192,11 @@ void launchOhm(Geom const& G, double eta, double nu, int hyper, int level, doubl
                CVecPtr Ve, double const* Pe, CVecPtr B, CVecPtr J, VecPtr E, hipStream_t s,
                int fill = 0);
 void launchAverage(double const* a, double const* b, double* out, long count, hipStream_t s);
+// predictor Ohm (Ep on the physical nodes from Bp, J) and the average (Ea, Ba from E, Ep, B, Bp)
+// of a periodic level in one launch; mirror as launchAverageFill
+void launchOhmAverage(Geom const& G, double eta, double nu, int hyper, int level, double const* n,
+                      CVecPtr Ve, double const* Pe, CVecPtr Bp, CVecPtr J, VecPtr Ep, CVecPtr E,
+                      CVecPtr B, VecPtr Ea, VecPtr Ba, hipStream_t s, float* const* mirror = nullptr);
 void launchFillGhosts(Geom const& G, int q, double* f, hipStream_t s);
 void launchBorderSum(Geom const& G, double const* orig, double* out, hipStream_t s);
 void launchBorderMax(Geom const& G, double const* orig, double* inout, hipStream_t s);
@@ -229,5 +234,9 @@ void launchEmToF32(Geom const& G, double const* const* src, float* const* dst, h
 void launchFoldTotals(Geom const& G, MomSetArgs const& A, double* trho, double* tmd, VecPtr tvi,
                       hipStream_t s);
 void launchMax5(Geom const& G, double const* const* src, double* const* dst, hipStream_t s);
+// the border max of the totals (src: before the max; dst: rho, massDensity, Vi) and the electron
+// closure in one launch
+void launchElectronsMax(Geom const& G, double Te, CVecPtr J, double const* const* src,
+                       double* const* dst, VecPtr Ve, double* Pe, hipStream_t s);
 
 } // namespace phx
