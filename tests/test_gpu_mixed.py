@@ -109,6 +109,24 @@ def test_mixed_twenty_steps(name):
     g.close()
 
 
+def test_mixed_two_populations():
+    """two populations (mass 1 and 4, charge 1 and 2): moments of each and the fields after 5
+    PPC steps against the FP64 oracle"""
+    _say("[gpu] mixed two populations")
+    wl = W.harris2d(ncells=(32, 24), ppc=8, order=2)
+    wl.masses, wl.charges = (1.0, 4.0), (1.0, 2.0)
+    g, o = _pair(wl)
+    g.advance(wl.dt, 5)
+    o.advance(wl.dt, 5)
+    e = {f"pop{p}.moment{k}": U.max_rel(g.get(phx.f_pop(p, k)), o.get(O.f_pop(p, k)))
+         for p in (0, 1) for k in range(5)}
+    e.update(_field_errors(g, o))
+    _say("  errors", {k: f"{v:.2e}" for k, v in e.items()})
+    for k, v in e.items():
+        assert v <= STATE_TOL_20, f"{k}: {v:.3e}"
+    g.close()
+
+
 def test_mixed_energy_drift_matches_fp64():
     """1000 PPC steps of a 2D Harris sheet (64x32 cells, 50 ppc, order 2) in both families: the
     total-energy drifts agree within 1e-4 and the total charge is conserved"""

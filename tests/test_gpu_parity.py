@@ -268,6 +268,53 @@ def test_advance_harris2d_small(order):
     g.close()
 
 
+def _two_pop_harris(order=2):
+    """the small Harris sheet with a second population (mass 4, charge 2) on the same profiles"""
+    wl = W.harris2d(ncells=(32, 24), ppc=8, order=order)
+    wl.masses, wl.charges = (1.0, 4.0), (1.0, 2.0)
+    return wl
+
+
+def test_advance_two_populations_2d():
+    """two populations of different mass and charge (each of one charge, so the segmented push
+    derives each charge density from n with its own charge), 12 PPC steps against the oracle"""
+    _say("[gpu] advance two populations 2d")
+    wl = _two_pop_harris()
+    g, o = _run_pair(wl, 12)
+    for fid in range(0, 6):
+        U.assert_close(g.get(fid), o.get(fid), what=f"field {fid}")
+    for pop in (0, 1):
+        for k in range(5):
+            U.assert_close(g.get(phx.f_pop(pop, k)), o.get(O.f_pop(pop, k)),
+                           what=f"pop {pop} moment {k}")
+        U.assert_same_particles(g.get_particles(pop), o.get_particles(pop), U.RTOL,
+                                what=f"pop {pop} particles")
+    U.assert_close(g.get(phx.F_RHO), o.get(O.F_RHO), what="rho")
+    g.close()
+
+
+def test_advance_mixed_charges_2d():
+    """a population whose particles carry different charges: the segmented push deposits the
+    charge density particle by particle; 10 PPC steps against the oracle"""
+    _say("[gpu] advance mixed charges 2d")
+    wl = W.harris2d(ncells=(24, 20), ppc=8, order=2)
+    rng = np.random.default_rng(77)
+    g = phx.Level(2, 2, wl.ncells, wl.dx, masses=wl.masses, Te=wl.Te, capacity=20000)
+    o = O.Sim(2, 2, wl.ncells, wl.dx, masses=wl.masses, Te=wl.Te)
+    p = U.random_particles(rng, 2, 4000, wl.ncells, [0, 0], 0, vscale=0.5)
+    p["charge"] = np.where(rng.random(4000) < 0.5, 1.0, 2.0)
+    for sim, fb in ((g, phx.F_B), (o, O.F_B)):
+        sim.set_particles(0, 0, p)
+        for c, b in enumerate(wl.b_arrays(lambda c: sim.shape(fb + c))):
+            sim.set(fb + c, b)
+        sim.initialize()
+    g.advance(wl.dt, 10)
+    o.advance(wl.dt, 10)
+    g.sync()
+    _check_state(g, o, wl)
+    g.close()
+
+
 @pytest.mark.skip(reason="one GPU run faulted the GPU in the 3D order-1 case of this test (first 3D "
                          "advance on the fused periodic path); the cause is not yet shown (an "
                          "AddressSanitizer CPU emulation of the same scenario is clean), so the 3D "
