@@ -906,6 +906,21 @@ phx_status sortDomain(phx_ctx* c)
     return PHX_OK;
 }
 
+// the mixed-precision push runs on segmented storage only: sort what an upload left linear
+phx_status sortIfLinear(phx_ctx* c)
+{
+    if (!mixed(c))
+        return PHX_OK;
+    for (int p = 0; p < c->cfg.n_pop; ++p)
+    {
+        Pop const& P = c->pops[p];
+        Store const& s = P.arr[PHX_DOMAIN];
+        if (!s.seg || segItemsCap(c->sg, s.cap) > P.itemsAlloc)
+            return sortDomain(c);
+    }
+    return PHX_OK;
+}
+
 // the particle storage a captured step addresses: pointers and counts of every array
 std::vector<unsigned long long> storageKey(phx_ctx const* c)
 {
@@ -1620,6 +1635,7 @@ phx_status phx_move_ions(phx_ctx* c, double dt, int mode)
     if (fastPath(c))
     {
         // the averaged set may have been set or computed outside the average kernel
+        PHX_TRY(sortIfLinear(c));
         if (mixed(c))
             launchEmToF32(c->G, c->em[PHX_EM_AVG], c->em32, c->stream);
         PHX_TRY(enqueueMoveIonsFast(c, dt, mode));
@@ -1702,6 +1718,7 @@ phx_status phx_advance(phx_ctx* c, double dt, int nsteps)
         }
         return checkDeviceError(c);
     }
+    PHX_TRY(sortIfLinear(c));
     for (int s = 0; s < nsteps; ++s)
     {
         phx_ctx::GraphEntry* g = nullptr;
@@ -1810,6 +1827,7 @@ phx_status phx_profile_push(phx_ctx* c, double dt, int mode, int reps, double* a
     // the push kernel alone, captured in a graph and replayed back to back between two events
     // on the context stream (the mean over the replays is the kernel time; per-launch event
     // pairs around direct launches read high on this runtime)
+    PHX_TRY(sortIfLinear(c));
     PushArgs A     = pushArgs(c, 0, KIND_DOMAIN, mode, PHX_EM_AVG, dt, true);
     A.periodicWrap = 1;
     Store const& dom = c->pops[0].arr[PHX_DOMAIN];

@@ -152,6 +152,20 @@ def test_mixed_energy_drift_matches_fp64():
     assert abs(charges[phx.PRECISION_FP32]) <= 1e-6
 
 
+def test_mixed_two_cell_move_is_an_error():
+    """the FP32 push raises the reference's two-cell error (boris.hpp:164-165) like the FP64 one;
+    particles set after the level's sort are sorted again before the push"""
+    _say("[gpu] mixed two-cell move error")
+    g = phx.Level(1, 1, [20], [0.3], precision=phx.PRECISION_FP32)
+    p = dict(icell=np.array([[5]], np.int32), delta=np.array([[0.5]]),
+             v=np.array([[1000.0, 0, 0]]), weight=np.ones(1), charge=np.ones(1))
+    g.set_particles(0, phx.DOMAIN, p)
+    with pytest.raises(phx.PhxError) as e:
+        g.move_ions(0.002, phx.MODE_ALL)
+    assert "moved 2 cells" in str(e.value)
+    g.close()
+
+
 def test_mixed_rejects_drop_in_calls():
     """the mixed-precision family is the resident level path: updatePopulations on host-bound
     arrays, ghost arrays and non-periodic patches are refused with an error, not run in FP64"""
