@@ -113,7 +113,7 @@ def step_bytes(wl, precision="fp64"):
     return wl.n_particles * (2 * R + M) + wl.n_cells * 8 * (189 + 20 * len(wl.masses))
 
 
-def run_family(wl, precision, steps, warmup):
+def run_family(wl, precision, steps, warmup, drift_steps=0):
     """steps/s of one precision family on the workload, and its push kernel's time measured
     inside the timed steps"""
     prec = phx.PRECISION_FP32 if precision == "fp32" else phx.PRECISION_FP64
@@ -138,14 +138,23 @@ def run_family(wl, precision, steps, warmup):
     ms_dom, n_dom = lvl.push_timing(phx.MODE_DOMAIN_ONLY)
     assert n_all == steps and n_dom == steps, (n_all, n_dom, steps)
     diag1 = lvl.diagnostics()
+    # untimed: continue to the workload's full run length for its energy drift
+    done = warmup + steps
+    if drift_steps > done:
+        lvl.advance(wl.dt, drift_steps - done)
+        done = drift_steps
+    diagN = lvl.diagnostics()
     lvl.close()
     e0 = diag0["kinetic"] + diag0["magnetic"]
     e1 = diag1["kinetic"] + diag1["magnetic"]
+    eN = diagN["kinetic"] + diagN["magnetic"]
     return dict(value=wl.n_particles * steps / elapsed, ms_step=1e3 * elapsed / steps,
                 ms_all=ms_all, ms_dom=ms_dom,
                 energy_drift=(e1 - e0) / e0,
                 charge_drift=(diag1["charge"] - diag0["charge"]) / diag0["charge"],
-                steps_done=warmup + steps)
+                steps_done=warmup + steps,
+                energy_drift_run=(eN - e0) / e0, run_steps=done,
+                charge_drift_run=(diagN["charge"] - diag0["charge"]) / diag0["charge"])
 
 
 def cpu_baseline(wl, steps):
@@ -175,6 +184,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision family")
+    ap.add_argument("--no-drift-run", action="store_true",
+                    help="skip the untimed continuation to the workload's run length (energy drift)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES (counter calibration)")
     args = ap.parse_args()
@@ -187,7 +198,8 @@ def main():
         sys.exit(2)
 
     wl = workload(args.workload)
-    f64 = run_family(wl, "fp64", args.steps, args.warmup)
+    drift_steps = 0 if args.no_drift_run else wl.nsteps
+    f64 = run_family(wl, "fp64", args.steps, args.warmup, drift_steps)
     bytes_all = push_bytes(wl, phx.MODE_ALL)
     achieved = bytes_all / (f64["ms_all"] * 1e-3) / 1e9
     out = {
@@ -217,10 +229,12 @@ def main():
         "step_hbm_fraction": step_bytes(wl) / (f64["ms_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
         "energy_drift": f64["energy_drift"],
         "charge_drift": f64["charge_drift"],
+        "run": {"steps": f64["run_steps"], "energy_drift": f64["energy_drift_run"],
+                "charge_drift": f64["charge_drift_run"]},
     }
     if not args.no_mixed:
         # the mixed-precision family on the same workload (FP32 particles, FP64 fields)
-        f32 = run_family(wl, "fp32", args.steps, args.warmup)
+        f32 = run_family(wl, "fp32", args.steps, args.warmup, drift_steps)
         b32 = push_bytes(wl, phx.MODE_ALL, "fp32")
         a32 = b32 / (f32["ms_all"] * 1e-3) / 1e9
         out["mixed_fp32"] = {
@@ -237,6 +251,9 @@ def main():
             "energy_drift_minus_fp64": f32["energy_drift"] - f64["energy_drift"],
             "charge_drift": f32["charge_drift"],
             "drift_steps": f32["steps_done"],
+            "run": {"steps": f32["run_steps"], "energy_drift": f32["energy_drift_run"],
+                    "charge_drift": f32["charge_drift_run"],
+                    "energy_drift_minus_fp64": f32["energy_drift_run"] - f64["energy_drift_run"]},
         }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(wl, args.cpu_steps)
