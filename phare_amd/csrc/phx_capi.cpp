@@ -1727,7 +1727,9 @@ phx_status phx_advance(phx_ctx* c, double dt, int nsteps)
         if (c->sortInterval > 0 && ++c->stepsSinceSort >= c->sortInterval)
             PHX_TRY(sortDomain(c));
     }
-    return PHX_OK;
+    // a two-cell move or a stencil out of the ghost box inside the steps is reported here (the
+    // steps run asynchronously; the first error is kept with its particle and stage)
+    return checkDeviceError(c);
 }
 
 phx_status phx_diagnostics(phx_ctx* c, double out[4])
