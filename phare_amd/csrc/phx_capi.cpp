@@ -487,9 +487,20 @@ bool fastPath(phx_ctx* c)
 // enqueue the particle stage of SolverPPC::moveIons_ on the fast path (no host sync): the push
 // deposits into the current raw set, then one kernel border-sums every population moment,
 // clears the other raw set and computes the totals, and one kernel applies their border max
+// a Faraday to run in the fold's launch: B of set bset, E of eset -> B of bnset (inPlace: the
+// corrector's update of the state B on its physical nodes)
+struct FaradaySpec
+{
+    int bset, eset, bnset;
+    double dt;
+    bool inPlace;
+};
+
 // borderMax false: the border max of the totals is left to the next electron closure
-// (launchElectronsMax), which a captured step runs before anything reads the totals
-phx_status enqueueMoveIonsFast(phx_ctx* c, double dt, int mode, bool borderMax = true)
+// (launchElectronsMax), which a captured step runs before anything reads the totals; far: the
+// Faraday that follows in the step, launched with the fold
+phx_status enqueueMoveIonsFast(phx_ctx* c, double dt, int mode, bool borderMax = true,
+                               FaradaySpec const* far = nullptr)
 {
     int const cur = c->rawCur;
     MomSetArgs M{};
@@ -519,7 +530,12 @@ phx_status enqueueMoveIonsFast(phx_ctx* c, double dt, int mode, bool borderMax =
         }
     }
     PHX_TRY(checkLaunch(c, "push"));
-    launchFoldTotals(c->G, M, c->tot5[0], c->tot5[1], vec(c->tot5 + 2), c->stream);
+    if (far)
+        launchFoldFaraday(c->G, M, c->tot5[0], c->tot5[1], vec(c->tot5 + 2), cvec(c->em[far->bset] + 3),
+                          cvec(c->em[far->eset]), vec(c->em[far->bnset] + 3), far->dt, far->inPlace,
+                          c->stream);
+    else
+        launchFoldTotals(c->G, M, c->tot5[0], c->tot5[1], vec(c->tot5 + 2), c->stream);
     if (borderMax)
     {
         double const* src[5] = {c->tot5[0], c->tot5[1], c->tot5[2], c->tot5[3], c->tot5[4]};
@@ -600,12 +616,14 @@ void electronsAfterFold(phx_ctx* c, bool borderMax)
     launchElectronsMax(c->G, c->cfg.Te, cvec(c->J), src, dst, vec(c->Ve), c->Pe, c->stream);
 }
 
-// predictor + average of a periodic level: Faraday and Ampere with their ghost layers, the
-// electron closure (with the pending border max), then Ohm and the average in one kernel
-void enqueuePredictorAverageFast(phx_ctx* c, double dt, int eset, bool borderMax)
+// predictor + average of a periodic level: Faraday (unless the previous fold's launch ran it)
+// and Ampere with their ghost layers, the electron closure (with the pending border max), then
+// Ohm and the average in one kernel
+void enqueuePredictorAverageFast(phx_ctx* c, double dt, int eset, bool borderMax, bool faraday)
 {
-    launchFaradayFill(c->G, cvec(c->em[PHX_EM] + 3), cvec(c->em[eset]), vec(c->em[PHX_EM_PRED] + 3), dt,
-                      c->stream);
+    if (faraday)
+        launchFaradayFill(c->G, cvec(c->em[PHX_EM] + 3), cvec(c->em[eset]), vec(c->em[PHX_EM_PRED] + 3),
+                          dt, c->stream);
     launchAmpereFill(c->G, cvec(c->em[PHX_EM_PRED] + 3), vec(c->J), c->stream);
     electronsAfterFold(c, borderMax);
     launchOhmAverage(c->G, c->cfg.eta, c->cfg.nu, c->cfg.hyper_mode, c->patch.level, c->rho,
@@ -615,12 +633,11 @@ void enqueuePredictorAverageFast(phx_ctx* c, double dt, int eset, bool borderMax
                      mixed(c) ? c->em32 : nullptr);
 }
 
-// SolverPPC::corrector_ of a periodic level: in-place Faraday and its B fill, Ampere, the
-// electron closure with the pending border max, Ohm with its ghost layers
-void enqueueCorrectorFast(phx_ctx* c, double dt)
+// SolverPPC::corrector_ of a periodic level after the in-place Faraday (run with the last
+// fold): the B fill, Ampere, the electron closure with the pending border max, Ohm with its
+// ghost layers
+void enqueueCorrectorFast(phx_ctx* c)
 {
-    launchFaraday(c->G, cvec(c->em[PHX_EM] + 3), cvec(c->em[PHX_EM_AVG]), vec(c->em[PHX_EM] + 3), dt,
-                  c->stream);
     launchFill3(c->G, QBX, vec(c->em[PHX_EM] + 3), c->stream);
     launchAmpereFill(c->G, cvec(c->em[PHX_EM] + 3), vec(c->J), c->stream);
     electronsAfterFold(c, true);
@@ -629,14 +646,17 @@ void enqueueCorrectorFast(phx_ctx* c, double dt)
 }
 
 // one PPC step of a periodic level, as captured: each moveIons leaves the border max of the
-// totals to the electron closure that follows it (the first reader of the totals)
+// totals to the electron closure that follows it (the first reader of the totals), and its fold
+// shares a launch with the next Faraday (predictor 2's, then the corrector's in-place one)
 phx_status enqueueAdvanceFast(phx_ctx* c, double dt)
 {
-    enqueuePredictorAverageFast(c, dt, PHX_EM, false);
-    PHX_TRY(enqueueMoveIonsFast(c, dt, PHX_MODE_DOMAIN_ONLY, false));
-    enqueuePredictorAverageFast(c, dt, PHX_EM_AVG, true);
-    PHX_TRY(enqueueMoveIonsFast(c, dt, PHX_MODE_ALL, false));
-    enqueueCorrectorFast(c, dt);
+    enqueuePredictorAverageFast(c, dt, PHX_EM, false, true);
+    FaradaySpec const pred2{PHX_EM, PHX_EM_AVG, PHX_EM_PRED, dt, false};
+    PHX_TRY(enqueueMoveIonsFast(c, dt, PHX_MODE_DOMAIN_ONLY, false, &pred2));
+    enqueuePredictorAverageFast(c, dt, PHX_EM_AVG, true, false);
+    FaradaySpec const corr{PHX_EM, PHX_EM_AVG, PHX_EM, dt, true};
+    PHX_TRY(enqueueMoveIonsFast(c, dt, PHX_MODE_ALL, false, &corr));
+    enqueueCorrectorFast(c);
     return checkLaunch(c, "advance");
 }
 

@@ -853,15 +853,11 @@ __device__ __forceinline__ void imagesOf(GT const& G, int const* c, Images<GT::d
 // then its overlapping periodic images in lexicographic order. Loads are unconditional (an
 // image that does not overlap reads the node itself and is not added) and precede the
 // population's stores, so that they are in flight together.
-template<int DIM>
-__global__ __launch_bounds__(FB) void k_fold_totals(Geom G0, MomSet M, double* trho, double* tmd,
-                                                    VecPtr tvi)
+template<class GT>
+__device__ __forceinline__ void foldTotalsAt(GT const& G, MomSet const& M, int i, int j, int k,
+                                             double* trho, double* tmd, VecPtr tvi)
 {
-    GeomD<DIM> const G(G0);
-    long const t = static_cast<long>(blockIdx.x) * FB + threadIdx.x;
-    int i, j, k;
-    if (!nodeOf(G, QMOM, t, i, j, k))
-        return;
+    constexpr int DIM = GT::dim;
     int const c[3] = {i, j, k};
     int const n    = G.idx(QMOM, i, j, k);
     Images<DIM> I;
@@ -904,6 +900,53 @@ __global__ __launch_bounds__(FB) void k_fold_totals(Geom G0, MomSet M, double* t
     tvi.c[0][n] = vx / m;
     tvi.c[1][n] = vy / m;
     tvi.c[2][n] = vz / m;
+}
+
+template<int DIM>
+__global__ __launch_bounds__(FB) void k_fold_totals(Geom G0, MomSet M, double* trho, double* tmd,
+                                                    VecPtr tvi)
+{
+    GeomD<DIM> const G(G0);
+    long const t = static_cast<long>(blockIdx.x) * FB + threadIdx.x;
+    int i, j, k;
+    if (!nodeOf(G, QMOM, t, i, j, k))
+        return;
+    foldTotalsAt(G, M, i, j, k, trho, tmd, tvi);
+}
+
+// the fold of a moveIons and the Faraday that follows it in a captured step, in one launch (the
+// two are independent): grid rows 0-2 compute component c of Faraday, as k_faraday_fill does
+// (INPLACE false: into Bn with its ghost layers) or as k_faraday does (INPLACE true: Bn is B, the
+// physical nodes only), row 3 computes the fold as k_fold_totals does
+template<int DIM, bool INPLACE>
+__global__ __launch_bounds__(FB) void k_fold_faraday(Geom G0, MomSet M, double* trho, double* tmd,
+                                                     VecPtr tvi, CVecPtr B, CVecPtr E, VecPtr Bn,
+                                                     double dt)
+{
+    GeomD<DIM> const G(G0);
+    long const t = static_cast<long>(blockIdx.x) * FB + threadIdx.x;
+    int i, j, k;
+    if (!nodeOf(G, QMOM, t, i, j, k))
+        return;
+    int const c = static_cast<int>(blockIdx.y);
+    if (c == 3)
+    {
+        foldTotalsAt(G, M, i, j, k, trho, tmd, tvi);
+        return;
+    }
+    if (!inShape(G, QBX + c, i, j, k))
+        return;
+    int s[3] = {i, j, k};
+    if constexpr (INPLACE)
+    {
+        int lo[3], hi[3];
+        physBox(G, QBX + c, lo, hi);
+        if (!inRange(G, i, j, k, lo, hi))
+            return;
+    }
+    else
+        wrapNode(G, QBX + c, s);
+    Bn.c[c][G.idx(QBX + c, i, j, k)] = faradayAt(G, B, E, dt, c, s[0], s[1], s[2]);
 }
 
 struct Five
@@ -1201,6 +1244,33 @@ void launchFoldTotals(Geom const& G, MomSetArgs const& A, double* trho, double* 
         case 1: k_fold_totals<1><<<gridFor(G.size(QMOM)), FB, 0, s>>>(G, M, trho, tmd, tvi); break;
         case 2: k_fold_totals<2><<<gridFor(G.size(QMOM)), FB, 0, s>>>(G, M, trho, tmd, tvi); break;
         case 3: k_fold_totals<3><<<gridFor(G.size(QMOM)), FB, 0, s>>>(G, M, trho, tmd, tvi); break;
+    }
+}
+
+void launchFoldFaraday(Geom const& G, MomSetArgs const& A, double* trho, double* tmd, VecPtr tvi,
+                       CVecPtr B, CVecPtr E, VecPtr Bn, double dt, bool inPlace, hipStream_t s)
+{
+    MomSet M{};
+    M.npop = A.npop;
+    for (int p = 0; p < A.npop; ++p)
+    {
+        M.mass[p] = A.mass[p];
+        for (int q = 0; q < 5; ++q)
+        {
+            M.raw[p][q]  = A.raw[p][q];
+            M.mom[p][q]  = A.mom[p][q];
+            M.zero[p][q] = A.zero[p][q];
+        }
+    }
+    dim3 const grid(gridFor(G.size(QMOM)), 4);
+    switch (G.dim * 2 + (inPlace ? 1 : 0))
+    {
+        case 2: k_fold_faraday<1, false><<<grid, FB, 0, s>>>(G, M, trho, tmd, tvi, B, E, Bn, dt); break;
+        case 3: k_fold_faraday<1, true><<<grid, FB, 0, s>>>(G, M, trho, tmd, tvi, B, E, Bn, dt); break;
+        case 4: k_fold_faraday<2, false><<<grid, FB, 0, s>>>(G, M, trho, tmd, tvi, B, E, Bn, dt); break;
+        case 5: k_fold_faraday<2, true><<<grid, FB, 0, s>>>(G, M, trho, tmd, tvi, B, E, Bn, dt); break;
+        case 6: k_fold_faraday<3, false><<<grid, FB, 0, s>>>(G, M, trho, tmd, tvi, B, E, Bn, dt); break;
+        case 7: k_fold_faraday<3, true><<<grid, FB, 0, s>>>(G, M, trho, tmd, tvi, B, E, Bn, dt); break;
     }
 }
 

@@ -234,6 +234,10 @@ void launchEmToF32(Geom const& G, double const* const* src, float* const* dst, h
 void launchFoldTotals(Geom const& G, MomSetArgs const& A, double* trho, double* tmd, VecPtr tvi,
                       hipStream_t s);
 void launchMax5(Geom const& G, double const* const* src, double* const* dst, hipStream_t s);
+// launchFoldTotals and the Faraday that follows it in a captured step, in one launch: B, E -> Bn
+// with Bn's ghost layers, or (inPlace: Bn is B) on B's physical nodes
+void launchFoldFaraday(Geom const& G, MomSetArgs const& A, double* trho, double* tmd, VecPtr tvi,
+                       CVecPtr B, CVecPtr E, VecPtr Bn, double dt, bool inPlace, hipStream_t s);
 // the border max of the totals (src: before the max; dst: rho, massDensity, Vi) and the electron
 // closure in one launch
 void launchElectronsMax(Geom const& G, double Te, CVecPtr J, double const* const* src,
