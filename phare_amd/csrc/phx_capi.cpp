@@ -123,6 +123,8 @@ struct phx_ctx
     Store sortBuf[PHX_MAX_POP];
     SegGeom sg{};
     unsigned int* segWork = nullptr; // per-cell counts and per-segment work items of a sort
+    unsigned int* sortRank = nullptr; // rank of each particle in its cell
+    size_t rankCap         = 0;
     int sortInterval      = 50;
     int stepsSinceSort    = 0;
 
@@ -634,12 +636,11 @@ void enqueuePredictorAverageFast(phx_ctx* c, double dt, int eset, bool borderMax
 }
 
 // SolverPPC::corrector_ of a periodic level after the in-place Faraday (run with the last
-// fold): the B fill, Ampere, the electron closure with the pending border max, Ohm with its
-// ghost layers
+// fold): the B fill and Ampere in one launch, the electron closure with the pending border max,
+// Ohm with its ghost layers
 void enqueueCorrectorFast(phx_ctx* c)
 {
-    launchFill3(c->G, QBX, vec(c->em[PHX_EM] + 3), c->stream);
-    launchAmpereFill(c->G, cvec(c->em[PHX_EM] + 3), vec(c->J), c->stream);
+    launchAmpereFillB(c->G, vec(c->em[PHX_EM] + 3), vec(c->J), c->stream);
     electronsAfterFold(c, true);
     launchOhm(c->G, c->cfg.eta, c->cfg.nu, c->cfg.hyper_mode, c->patch.level, c->rho, cvec(c->Ve),
               c->Pe, cvec(c->em[PHX_EM] + 3), cvec(c->J), vec(c->em[PHX_EM]), c->stream, 1);
@@ -868,8 +869,10 @@ size_t recordSize(int dim) { return dim == 1 ? 56 : (dim == 2 ? 64 : 80); }
 // =============================================================================================
 namespace
 {
-// counting sort of every population's domain array into cell-segmented storage in its sort
-// buffer; the two swap. Lost particles are dropped.
+// counting sort of every population's domain array into cell-segmented storage: scattered into
+// the population's sort buffer, then copied back, so that the domain array keeps its address and
+// the captured step its graph. The two swap when the array is linear (the first sort after an
+// upload) or too small for the sorted slots. Lost particles are dropped.
 phx_status sortDomain(phx_ctx* c)
 {
     for (int p = 0; p < c->cfg.n_pop; ++p)
@@ -877,18 +880,29 @@ phx_status sortDomain(phx_ctx* c)
         Pop& P     = c->pops[p];
         Store& dom = P.arr[PHX_DOMAIN];
         Store& buf = c->sortBuf[p];
+        if (c->rankCap < dom.n)
+        {
+            devFree(c, c->sortRank);
+            c->sortRank = nullptr;
+            c->rankCap  = 0;
+            size_t const want = dom.n + dom.n / 2 + 4096;
+            PHX_TRY(devAlloc(c, (void**)&c->sortRank, want * 4));
+            c->rankCap = want;
+        }
         if (mixed(c))
-            launchSegCount(c->sg, dom.view32(), dom.n, c->segWork, P.seg, c->stream);
+            launchSegCount(c->sg, dom.view32(), dom.n, c->segWork, c->sortRank, P.seg, c->stream);
         else
-            launchSegCount(c->sg, dom.view(), dom.n, c->segWork, P.seg, c->stream);
+            launchSegCount(c->sg, dom.view(), dom.n, c->segWork, c->sortRank, P.seg, c->stream);
         PHX_TRY(checkLaunch(c, "particle sort (count)"));
         unsigned long long slots = 0;
         PHX_HIP(c, hipStreamSynchronize(c->stream));
         PHX_HIP(c, hipMemcpy(&slots, P.seg.off + c->sg.nsegs + 1, 8, hipMemcpyDeviceToHost));
         if (buf.cap < slots || buf.cap < dom.cap)
         {
-            // room for the empty slots of growing segments, so that later sorts reuse it
-            size_t const want = slots + slots / 8 + 4096;
+            // room for the empty slots of growing segments, so that later sorts reuse it: a
+            // segment has as many rows as its most populated cell has particles, which grows as
+            // a loaded plasma mixes (by a quarter at a hundred particles per cell)
+            size_t const want = slots + slots / 2 + 4096;
             storeFree(c, buf);
             PHX_TRY(storeAlloc(c, buf, std::max(want, dom.cap)));
         }
@@ -902,12 +916,24 @@ phx_status sortDomain(phx_ctx* c)
             P.itemsAlloc = need;
         }
         if (mixed(c))
-            launchSegScatter(c->sg, dom.view32(), buf.view32(), dom.n, c->segWork, P.seg, c->cfg.dim,
+            launchSegScatter(c->sg, dom.view32(), buf.view32(), dom.n, c->segWork, c->sortRank, P.seg,
                              c->stream);
         else
-            launchSegScatter(c->sg, dom.view(), buf.view(), dom.n, c->segWork, P.seg, c->cfg.dim,
+            launchSegScatter(c->sg, dom.view(), buf.view(), dom.n, c->segWork, c->sortRank, P.seg,
                              c->stream);
         PHX_TRY(checkLaunch(c, "particle sort (scatter)"));
+        // a linear array (an upload, sized for it) moves into the buffer with its headroom
+        if (dom.seg && slots <= dom.cap)
+        {
+            if (mixed(c))
+                launchSegCopy(buf.view32(), dom.view32(), slots, c->cfg.dim, c->stream);
+            else
+                launchSegCopy(buf.view(), dom.view(), slots, c->cfg.dim, c->stream);
+            PHX_TRY(checkLaunch(c, "particle sort (copy)"));
+            dom.n   = slots;
+            dom.seg = true;
+            continue;
+        }
         buf.n   = slots;
         buf.seg = true;
         std::swap(dom, buf);

@@ -693,27 +693,36 @@ __global__ __launch_bounds__(FB) void k_faraday_fill(Geom G0, CVecPtr B, CVecPtr
     }
 }
 
-template<class GT>
-__device__ __forceinline__ double ampereAt(GT const& G, CVecPtr B, int c, int i, int j, int k)
+// component c of curl B at a J node; bAt(d, i, j, k) gives B component d at a node
+template<class GT, class BF>
+__device__ __forceinline__ double ampereAtFn(GT const& G, BF const& bAt, int c, int i, int j, int k)
 {
     int const D = G.dim;
+    auto comp   = [&](int d) { return [&, d](int a, int b, int cc) { return bAt(d, a, b, cc); }; };
     if (c == 0)
     {
         if (D == 1)
             return 0.0;
         if (D == 2)
-            return deriv(G, B.c[2], QBZ, 1, i, j, k);
-        return deriv(G, B.c[2], QBZ, 1, i, j, k) - deriv(G, B.c[1], QBY, 2, i, j, k);
+            return derivFn(G, comp(2), QBZ, 1, i, j, k);
+        return derivFn(G, comp(2), QBZ, 1, i, j, k) - derivFn(G, comp(1), QBY, 2, i, j, k);
     }
     if (c == 1)
     {
         if (D < 3)
-            return -deriv(G, B.c[2], QBZ, 0, i, j, k);
-        return deriv(G, B.c[0], QBX, 2, i, j, k) - deriv(G, B.c[2], QBZ, 0, i, j, k);
+            return -derivFn(G, comp(2), QBZ, 0, i, j, k);
+        return derivFn(G, comp(0), QBX, 2, i, j, k) - derivFn(G, comp(2), QBZ, 0, i, j, k);
     }
     if (D == 1)
-        return deriv(G, B.c[1], QBY, 0, i, j, k);
-    return deriv(G, B.c[1], QBY, 0, i, j, k) - deriv(G, B.c[0], QBX, 1, i, j, k);
+        return derivFn(G, comp(1), QBY, 0, i, j, k);
+    return derivFn(G, comp(1), QBY, 0, i, j, k) - derivFn(G, comp(0), QBX, 1, i, j, k);
+}
+
+template<class GT>
+__device__ __forceinline__ double ampereAt(GT const& G, CVecPtr B, int c, int i, int j, int k)
+{
+    return ampereAtFn(G, [&](int d, int a, int b, int cc) { return B.c[d][G.idx(QBX + d, a, b, cc)]; },
+                      c, i, j, k);
 }
 
 // Ampere + fillCurrentGhosts
@@ -733,6 +742,43 @@ __global__ __launch_bounds__(FB) void k_ampere_fill(Geom G0, CVecPtr B, VecPtr J
         wrapNode(G, QJX + c, s);
         J.c[c][G.idx(QJX + c, i, j, k)] = ampereAt(G, B, c, s[0], s[1], s[2]);
     }
+}
+
+// Ampere + fillCurrentGhosts on a B whose ghost layers are not filled yet, and that fill
+// (fillMagneticGhosts) in the same launch: Ampere reads every B node at its periodic source,
+// which is what the filled ghost node holds; grid row c also writes B component c's ghost nodes
+// from their sources. Used after the corrector's in-place Faraday.
+template<int DIM>
+__global__ __launch_bounds__(FB) void k_ampere_fill_b(Geom G0, VecPtr B, VecPtr J)
+{
+    GeomD<DIM> const G(G0);
+    long const t = static_cast<long>(blockIdx.x) * FB + threadIdx.x;
+    int i, j, k;
+    if (!nodeOf(G, QMOM, t, i, j, k))
+        return;
+    int const c = static_cast<int>(blockIdx.y); // one component per grid row
+    // loads first: the B source of this node (if a ghost), then curl B from sources
+    bool const onB = inShape(G, QBX + c, i, j, k);
+    int sb[3]      = {i, j, k};
+    bool const gB  = onB && wrapNode(G, QBX + c, sb);
+    double const bsrc = gB ? B.c[c][G.idx(QBX + c, sb[0], sb[1], sb[2])] : 0.;
+    bool const onJ = inShape(G, QJX + c, i, j, k);
+    double jv      = 0.;
+    if (onJ)
+    {
+        int s[3] = {i, j, k};
+        wrapNode(G, QJX + c, s);
+        auto bAt = [&](int d, int a, int b, int cc) {
+            int w[3] = {a, b, cc};
+            wrapNode(G, QBX + d, w);
+            return B.c[d][G.idx(QBX + d, w[0], w[1], w[2])];
+        };
+        jv = ampereAtFn(G, bAt, c, s[0], s[1], s[2]);
+    }
+    if (gB)
+        B.c[c][G.idx(QBX + c, i, j, k)] = bsrc;
+    if (onJ)
+        J.c[c][G.idx(QJX + c, i, j, k)] = jv;
 }
 
 // ghost fill of three components in one launch
@@ -1165,6 +1211,15 @@ void launchAmpereFill(Geom const& G, CVecPtr B, VecPtr J, hipStream_t s)
         case 1: k_ampere_fill<1><<<dim3(gridFor(G.size(QMOM)), 3), FB, 0, s>>>(G, B, J); break;
         case 2: k_ampere_fill<2><<<dim3(gridFor(G.size(QMOM)), 3), FB, 0, s>>>(G, B, J); break;
         case 3: k_ampere_fill<3><<<dim3(gridFor(G.size(QMOM)), 3), FB, 0, s>>>(G, B, J); break;
+    }
+}
+void launchAmpereFillB(Geom const& G, VecPtr B, VecPtr J, hipStream_t s)
+{
+    switch (G.dim)
+    {
+        case 1: k_ampere_fill_b<1><<<dim3(gridFor(G.size(QMOM)), 3), FB, 0, s>>>(G, B, J); break;
+        case 2: k_ampere_fill_b<2><<<dim3(gridFor(G.size(QMOM)), 3), FB, 0, s>>>(G, B, J); break;
+        case 3: k_ampere_fill_b<3><<<dim3(gridFor(G.size(QMOM)), 3), FB, 0, s>>>(G, B, J); break;
     }
 }
 void launchFill3(Geom const& G, int q0, VecPtr F, hipStream_t s)

@@ -153,19 +153,24 @@ SegGeom segGeom(int dim, int const* lo, int const* hi);
 TileSpec segTileShape(int dim, int order, SegGeom const& sg, int copies);
 int segRowsPerItem();
 unsigned int segItemsCap(SegGeom const& sg, unsigned long long capacity);
-// first half of a sort of src (n slots) into cell-segmented storage: counts per cell and the
-// segment tables up to the total slot count (tables.off[nsegs + 1]); `work` holds
-// ncells + 1 + nsegs + 1 words
+// first half of a sort of src (n slots) into cell-segmented storage: counts per cell, the rank
+// of every particle in its cell (rank: n words) and the segment tables up to the total slot count
+// (tables.off[nsegs + 1]); `work` holds ncells + 1 + nsegs + 1 words
 void launchSegCount(SegGeom const& sg, PView const& src, unsigned long long n, unsigned int* work,
-                    SegTables const& tables, hipStream_t s);
+                    unsigned int* rank, SegTables const& tables, hipStream_t s);
 // second half, once dst can hold the slot count: work items, scatter and empty-slot marking
 void launchSegScatter(SegGeom const& sg, PView const& src, PView const& dst, unsigned long long n,
-                      unsigned int* work, SegTables const& tables, int dim, hipStream_t s);
+                      unsigned int* work, unsigned int const* rank, SegTables const& tables,
+                      hipStream_t s);
 // mixed-precision family: the same sort over FP32 arrays
 void launchSegCount(SegGeom const& sg, PView32 const& src, unsigned long long n, unsigned int* work,
-                    SegTables const& tables, hipStream_t s);
+                    unsigned int* rank, SegTables const& tables, hipStream_t s);
 void launchSegScatter(SegGeom const& sg, PView32 const& src, PView32 const& dst, unsigned long long n,
-                      unsigned int* work, SegTables const& tables, int dim, hipStream_t s);
+                      unsigned int* work, unsigned int const* rank, SegTables const& tables,
+                      hipStream_t s);
+// copy of the first n slots of every array (dim position components)
+void launchSegCopy(PView const& src, PView const& dst, unsigned long long n, int dim, hipStream_t s);
+void launchSegCopy(PView32 const& src, PView32 const& dst, unsigned long long n, int dim, hipStream_t s);
 // mixed-precision push + gather + deposit over cell-segmented FP32 storage, in place (A.pushOn 0:
 // deposit only); the LDS tile of segTileShape32
 void launchPush32(PushArgs const& A, PView32 const& v, hipStream_t s);
@@ -225,6 +230,9 @@ struct MomSetArgs
 void launchFaradayFill(Geom const& G, CVecPtr B, CVecPtr E, VecPtr Bnew, double dt, hipStream_t s);
 void launchAmpereFill(Geom const& G, CVecPtr B, VecPtr J, hipStream_t s);
 void launchFill3(Geom const& G, int q0, VecPtr F, hipStream_t s);
+// the B ghost fill and Ampere with its ghost layers in one launch, for a B whose ghost layers
+// are stale (after an in-place Faraday): Ampere reads B at periodic sources
+void launchAmpereFillB(Geom const& G, VecPtr B, VecPtr J, hipStream_t s);
 // mirror (nullable): 6 FP32 arrays (Ex..Bz) that also receive the averaged set, for the
 // mixed-precision push's gather
 void launchAverageFill(Geom const& G, CVecPtr E, CVecPtr Ep, CVecPtr B, CVecPtr Bp, VecPtr Ea,

@@ -921,9 +921,12 @@ __global__ __launch_bounds__(BLOCK) void k_seg_zero(unsigned int* __restrict__ p
         p[i] = 0u;
 }
 
+// particles per cell key, and the rank of each particle among those of its key (the order in
+// which the atomics reached the count): the scatter places particle i at row rank[i] of its cell
 template<class V>
 __global__ __launch_bounds__(BLOCK) void k_seg_count(SegGeom S, V v, unsigned long long n,
-                                                     unsigned int* __restrict__ counts)
+                                                     unsigned int* __restrict__ counts,
+                                                     unsigned int* __restrict__ rank)
 {
     unsigned long long const i = static_cast<unsigned long long>(blockIdx.x) * BLOCK + threadIdx.x;
     if (i >= n)
@@ -931,32 +934,33 @@ __global__ __launch_bounds__(BLOCK) void k_seg_count(SegGeom S, V v, unsigned lo
     int c[3];
     unsigned int const key = cellKey(S, v, i, c);
     if (key != DEAD_KEY)
-        atomicAdd(counts + key, 1u);
+        rank[i] = atomicAdd(counts + key, 1u);
 }
 
 // rows of each segment (the most particles one of its cells holds; the overflow segment packs
-// its particles 64 to a row) and its work items
+// its particles 64 to a row) and its work items: one wave per segment, lane l reading the count
+// of the segment's l-th cell
 __global__ __launch_bounds__(BLOCK) void k_seg_rows(SegGeom S, unsigned int const* __restrict__ counts,
                                                     unsigned int* __restrict__ rows,
                                                     unsigned int* __restrict__ itemCount)
 {
-    unsigned int const s = blockIdx.x * BLOCK + threadIdx.x;
-    if (s > static_cast<unsigned int>(S.nsegs))
-        return;
-    unsigned int R = 0;
+    unsigned int const s = blockIdx.x * (BLOCK / LANES) + threadIdx.x / LANES;
+    int const lane       = static_cast<int>(threadIdx.x % LANES);
+    unsigned int R       = 0;
     if (s < static_cast<unsigned int>(S.nsegs))
     {
-        for (int l = 0; l < LANES; ++l)
-        {
-            unsigned int const key = laneKey(S, s, l);
-            if (key != DEAD_KEY)
-                R = max(R, counts[key]);
-        }
+        unsigned int const key = laneKey(S, s, lane);
+        R                      = key != DEAD_KEY ? counts[key] : 0u;
     }
-    else
+    else if (s == static_cast<unsigned int>(S.nsegs))
         R = (counts[S.ncells] + LANES - 1) / LANES;
-    rows[s]      = R;
-    itemCount[s] = (R + SEG_ROWS - 1) / SEG_ROWS;
+    for (int o = LANES / 2; o > 0; o >>= 1)
+        R = max(R, static_cast<unsigned int>(__shfl_xor(static_cast<int>(R), o, LANES)));
+    if (lane == 0 && s <= static_cast<unsigned int>(S.nsegs))
+    {
+        rows[s]      = R;
+        itemCount[s] = (R + SEG_ROWS - 1) / SEG_ROWS;
+    }
 }
 
 // exclusive scans over the m = nsegs + 1 segments in one workgroup: first slots (64 per row;
@@ -1026,7 +1030,7 @@ __global__ __launch_bounds__(BLOCK) void k_seg_items(unsigned int const* __restr
 template<class V>
 __global__ __launch_bounds__(BLOCK) void k_seg_scatter(SegGeom S, V src, V dst,
                                                        unsigned long long n,
-                                                       unsigned int* __restrict__ cursor,
+                                                       unsigned int const* __restrict__ rank,
                                                        unsigned long long const* __restrict__ off)
 {
     unsigned long long const i = static_cast<unsigned long long>(blockIdx.x) * BLOCK + threadIdx.x;
@@ -1042,11 +1046,10 @@ __global__ __launch_bounds__(BLOCK) void k_seg_scatter(SegGeom S, V src, V dst,
         unsigned int seg;
         int lane;
         segLane(S, c, seg, lane);
-        unsigned int const r = atomicAdd(cursor + key, 1u);
-        j = off[seg] + static_cast<unsigned long long>(r) * LANES + lane;
+        j = off[seg] + static_cast<unsigned long long>(rank[i]) * LANES + lane;
     }
     else
-        j = off[S.nsegs] + atomicAdd(cursor + key, 1u);
+        j = off[S.nsegs] + rank[i];
     for (int d = 0; d < S.dim; ++d)
     {
         dst.ic[d][j] = src.ic[d][i];
@@ -1084,57 +1087,95 @@ __global__ __launch_bounds__(BLOCK) void k_seg_pad(SegGeom S, V dst,
     }
 }
 
+// the first n slots of every array of src into dst (a sorted storage back into the array the
+// captured steps address)
+template<class V>
+__global__ __launch_bounds__(BLOCK) void k_seg_copy(V src, V dst, unsigned long long n, int dim)
+{
+    unsigned long long const i = static_cast<unsigned long long>(blockIdx.x) * BLOCK + threadIdx.x;
+    if (i >= n)
+        return;
+    for (int d = 0; d < dim; ++d)
+    {
+        dst.ic[d][i] = src.ic[d][i];
+        dst.de[d][i] = src.de[d][i];
+    }
+    dst.v[0][i] = src.v[0][i];
+    dst.v[1][i] = src.v[1][i];
+    dst.v[2][i] = src.v[2][i];
+    dst.w[i]    = src.w[i];
+    if (src.q)
+        dst.q[i] = src.q[i];
+}
+
 template<class V>
 static void segCountT(SegGeom const& sg, V const& src, unsigned long long n, unsigned int* work,
-                      SegTables const& tables, hipStream_t s)
+                      unsigned int* rank, SegTables const& tables, hipStream_t s)
 {
     unsigned int const nc = static_cast<unsigned int>(sg.ncells) + 1;
     unsigned int const m  = static_cast<unsigned int>(sg.nsegs) + 1;
     unsigned int* counts    = work;
     unsigned int* itemCount = work + nc;
+    constexpr unsigned int SEGS_PER_BLOCK = BLOCK / LANES;
     k_seg_zero<<<(nc + BLOCK - 1) / BLOCK, BLOCK, 0, s>>>(counts, nc);
     if (n)
         k_seg_count<V><<<static_cast<unsigned int>((n + BLOCK - 1) / BLOCK), BLOCK, 0, s>>>(sg, src, n,
-                                                                                          counts);
-    k_seg_rows<<<(m + BLOCK - 1) / BLOCK, BLOCK, 0, s>>>(sg, counts, tables.rows, itemCount);
+                                                                                          counts, rank);
+    k_seg_rows<<<(m + SEGS_PER_BLOCK - 1) / SEGS_PER_BLOCK, BLOCK, 0, s>>>(sg, counts, tables.rows,
+                                                                           itemCount);
     k_seg_scan<<<1, 1024, 0, s>>>(tables.rows, tables.off, itemCount, tables.nItems, m);
 }
 
 template<class V>
 static void segScatterT(SegGeom const& sg, V const& src, V const& dst, unsigned long long n,
-                        unsigned int* work, SegTables const& tables, hipStream_t s)
+                        unsigned int* work, unsigned int const* rank, SegTables const& tables,
+                        hipStream_t s)
 {
     unsigned int const nc = static_cast<unsigned int>(sg.ncells) + 1;
     unsigned int const m  = static_cast<unsigned int>(sg.nsegs) + 1;
     unsigned int* counts    = work;
     unsigned int* itemCount = work + nc;
     k_seg_items<<<(m + BLOCK - 1) / BLOCK, BLOCK, 0, s>>>(tables.rows, itemCount, tables.items, m);
-    k_seg_zero<<<(nc + BLOCK - 1) / BLOCK, BLOCK, 0, s>>>(counts, nc);
     if (n)
         k_seg_scatter<V><<<static_cast<unsigned int>((n + BLOCK - 1) / BLOCK), BLOCK, 0, s>>>(
-            sg, src, dst, n, counts, tables.off);
+            sg, src, dst, n, rank, tables.off);
     k_seg_pad<V><<<m, BLOCK, 0, s>>>(sg, dst, tables.rows, tables.off, counts);
 }
 
 void launchSegCount(SegGeom const& sg, PView const& src, unsigned long long n, unsigned int* work,
-                    SegTables const& tables, hipStream_t s)
+                    unsigned int* rank, SegTables const& tables, hipStream_t s)
 {
-    segCountT(sg, src, n, work, tables, s);
+    segCountT(sg, src, n, work, rank, tables, s);
 }
 void launchSegCount(SegGeom const& sg, PView32 const& src, unsigned long long n, unsigned int* work,
-                    SegTables const& tables, hipStream_t s)
+                    unsigned int* rank, SegTables const& tables, hipStream_t s)
 {
-    segCountT(sg, src, n, work, tables, s);
+    segCountT(sg, src, n, work, rank, tables, s);
 }
 void launchSegScatter(SegGeom const& sg, PView const& src, PView const& dst, unsigned long long n,
-                      unsigned int* work, SegTables const& tables, int, hipStream_t s)
+                      unsigned int* work, unsigned int const* rank, SegTables const& tables,
+                      hipStream_t s)
 {
-    segScatterT(sg, src, dst, n, work, tables, s);
+    segScatterT(sg, src, dst, n, work, rank, tables, s);
 }
 void launchSegScatter(SegGeom const& sg, PView32 const& src, PView32 const& dst, unsigned long long n,
-                      unsigned int* work, SegTables const& tables, int, hipStream_t s)
+                      unsigned int* work, unsigned int const* rank, SegTables const& tables,
+                      hipStream_t s)
 {
-    segScatterT(sg, src, dst, n, work, tables, s);
+    segScatterT(sg, src, dst, n, work, rank, tables, s);
+}
+
+void launchSegCopy(PView const& src, PView const& dst, unsigned long long n, int dim, hipStream_t s)
+{
+    if (n)
+        k_seg_copy<PView><<<static_cast<unsigned int>((n + BLOCK - 1) / BLOCK), BLOCK, 0, s>>>(src, dst, n,
+                                                                                              dim);
+}
+void launchSegCopy(PView32 const& src, PView32 const& dst, unsigned long long n, int dim, hipStream_t s)
+{
+    if (n)
+        k_seg_copy<PView32><<<static_cast<unsigned int>((n + BLOCK - 1) / BLOCK), BLOCK, 0, s>>>(src, dst,
+                                                                                                n, dim);
 }
 
 SegGeom segGeom(int dim, int const* lo, int const* hi)

@@ -388,13 +388,18 @@ def test_sort_groups_particles_by_segment(dim):
     q = g.get_particles()
     U.assert_same_particles(q, p, exact=True)
     assert np.all(np.diff(_segment_of(q["icell"], ncells)) >= 0)
+    # a second sort copies the sorted slots back into the same storage
+    g.sort_particles()
+    q = g.get_particles()
+    U.assert_same_particles(q, p, exact=True)
+    assert np.all(np.diff(_segment_of(q["icell"], ncells)) >= 0)
     g.close()
 
 
 @pytest.mark.parametrize("interval", [1, 3])
 def test_advance_with_frequent_sorts(interval):
-    """sorting every `interval` PPC steps (the captured step alternates between two storages)
-    leaves the run equal to the oracle's"""
+    """sorting every `interval` PPC steps (the first sort moves the linear upload into segmented
+    storage, later ones copy the sorted slots back into it) leaves the run equal to the oracle's"""
     _say(f"[gpu] advance harris2d sorted every {interval} steps")
     wl = W.harris2d(ncells=(32, 24), ppc=12, order=2)
     g = phx.Level(wl.dim, wl.order, wl.ncells, wl.dx, masses=wl.masses, Te=wl.Te,
