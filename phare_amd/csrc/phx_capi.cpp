@@ -1198,7 +1198,7 @@ phx_status phx_create(phx_config const* cfg, phx_patch const* patch, size_t part
         // segmented storage over the domain box
         c->sg = segGeom(cfg->dim, c->boxes.dom_lo, c->boxes.dom_hi);
         size_t const nsegs = static_cast<size_t>(c->sg.nsegs);
-        s = devAlloc(c, (void**)&c->segWork, (static_cast<size_t>(c->sg.ncells) + 1 + nsegs + 1) * 4);
+        s = devAlloc(c, (void**)&c->segWork, (static_cast<size_t>(c->sg.ncells) + 1 + 2 * (nsegs + 1)) * 4);
         for (int p = 0; p < cfg->n_pop && s == PHX_OK; ++p)
         {
             SegTables& T = c->pops[p].seg;

@@ -73,8 +73,9 @@ struct TileSpec
 };
 
 // Cell-segmented storage of a domain array: the domain box is cut into segments, boxes of at
-// most 64 cells (shape `seg`); a segment holds rows of 64 slots, slot l of row r being the r-th
-// particle of the segment's l-th cell (row-major inside the segment), empty slots holding dead
+// most 64 cells (shape `seg`); a segment holding P particles has R = ceil(P / 64) rows of 64
+// slots. Its particles, in the order of their cells (row-major inside the segment), take the
+// slots column by column: position p is slot p / R of row p % R. The slots past P hold dead
 // particles. One more segment at the end holds, unordered, the particles outside the domain box.
 struct SegGeom
 {
@@ -155,7 +156,7 @@ int segRowsPerItem();
 unsigned int segItemsCap(SegGeom const& sg, unsigned long long capacity);
 // first half of a sort of src (n slots) into cell-segmented storage: counts per cell, the rank
 // of every particle in its cell (rank: n words) and the segment tables up to the total slot count
-// (tables.off[nsegs + 1]); `work` holds ncells + 1 + nsegs + 1 words
+// (tables.off[nsegs + 1]); `work` holds ncells + 1 + 2 (nsegs + 1) words
 void launchSegCount(SegGeom const& sg, PView const& src, unsigned long long n, unsigned int* work,
                     unsigned int* rank, SegTables const& tables, hipStream_t s);
 // second half, once dst can hold the slot count: work items, scatter and empty-slot marking

@@ -8,9 +8,11 @@
 // f64 charge), each component a contiguous stream, in one of two storage orders:
 //  - linear: a workgroup owns a chunk of consecutive particles, lanes taking them with a stride
 //    across the chunk (any order is correct; cell order is faster);
-//  - cell-segmented (built by the sort): the 64 lanes of a wavefront read one row of a segment,
-//    so that each lane owns one cell and consecutive lanes own neighbouring cells. Reads stay
-//    coalesced and the lanes of one LDS deposit instruction add into distinct words.
+//  - cell-segmented (built by the sort): the 64 lanes of a wavefront read one row of a segment
+//    (a box of up to 64 cells whose particles, in cell order, fill its rows column by column), so
+//    that each lane walks a run of cell-ordered particles and the lanes of one row hold cells
+//    spread over the segment. Reads stay coalesced and the lanes of one LDS deposit instruction
+//    add into distinct words, except where a crowded cell spans two lanes.
 // Either way a workgroup accumulates its moments in an LDS tile over its cells and flushes the
 // tile with one HBM atomic per touched node and moment; the gather reads E/B through the L1/L2
 // caches, leaving the LDS to the deposit. Particles whose stencil leaves the tile deposit
@@ -937,29 +939,46 @@ __global__ __launch_bounds__(BLOCK) void k_seg_count(SegGeom S, V v, unsigned lo
         rank[i] = atomicAdd(counts + key, 1u);
 }
 
-// rows of each segment (the most particles one of its cells holds; the overflow segment packs
-// its particles 64 to a row) and its work items: one wave per segment, lane l reading the count
-// of the segment's l-th cell
-__global__ __launch_bounds__(BLOCK) void k_seg_rows(SegGeom S, unsigned int const* __restrict__ counts,
+// rows of each segment, ceil(particles / 64), and its work items; the count of each cell becomes
+// the cell's first position in the segment's cell order (its particles take the positions after,
+// in rank order) and total[s] the segment's particle count. One wave per segment, lane l taking
+// the segment's l-th cell; the overflow segment packs its particles 64 to a row.
+__global__ __launch_bounds__(BLOCK) void k_seg_rows(SegGeom S, unsigned int* __restrict__ counts,
                                                     unsigned int* __restrict__ rows,
-                                                    unsigned int* __restrict__ itemCount)
+                                                    unsigned int* __restrict__ itemCount,
+                                                    unsigned int* __restrict__ total)
 {
+    __shared__ unsigned int sc[BLOCK];
     unsigned int const s = blockIdx.x * (BLOCK / LANES) + threadIdx.x / LANES;
     int const lane       = static_cast<int>(threadIdx.x % LANES);
-    unsigned int R       = 0;
+    unsigned int key     = DEAD_KEY;
+    unsigned int cnt     = 0;
     if (s < static_cast<unsigned int>(S.nsegs))
     {
-        unsigned int const key = laneKey(S, s, lane);
-        R                      = key != DEAD_KEY ? counts[key] : 0u;
+        key = laneKey(S, s, lane);
+        cnt = key != DEAD_KEY ? counts[key] : 0u;
     }
-    else if (s == static_cast<unsigned int>(S.nsegs))
-        R = (counts[S.ncells] + LANES - 1) / LANES;
-    for (int o = LANES / 2; o > 0; o >>= 1)
-        R = max(R, static_cast<unsigned int>(__shfl_xor(static_cast<int>(R), o, LANES)));
+    // inclusive scan of the counts over each wave's lanes
+    sc[threadIdx.x] = cnt;
+    __syncthreads();
+    for (int o = 1; o < LANES; o <<= 1)
+    {
+        unsigned int const a = lane >= o ? sc[threadIdx.x - o] : 0u;
+        __syncthreads();
+        sc[threadIdx.x] += a;
+        __syncthreads();
+    }
+    unsigned int T = sc[(threadIdx.x & ~static_cast<unsigned int>(LANES - 1)) + LANES - 1];
+    if (key != DEAD_KEY)
+        counts[key] = sc[threadIdx.x] - cnt;
+    if (s == static_cast<unsigned int>(S.nsegs))
+        T = counts[S.ncells];
     if (lane == 0 && s <= static_cast<unsigned int>(S.nsegs))
     {
-        rows[s]      = R;
-        itemCount[s] = (R + SEG_ROWS - 1) / SEG_ROWS;
+        unsigned int const R = (T + LANES - 1) / LANES;
+        rows[s]              = R;
+        itemCount[s]         = (R + SEG_ROWS - 1) / SEG_ROWS;
+        total[s]             = T;
     }
 }
 
@@ -1027,10 +1046,14 @@ __global__ __launch_bounds__(BLOCK) void k_seg_items(unsigned int const* __restr
     }
 }
 
+// particle i to its slot: position p of segment s (R rows) is row p % R, lane p / R, so that
+// the lanes of a row are R positions apart in the segment's cell order
 template<class V>
 __global__ __launch_bounds__(BLOCK) void k_seg_scatter(SegGeom S, V src, V dst,
                                                        unsigned long long n,
                                                        unsigned int const* __restrict__ rank,
+                                                       unsigned int const* __restrict__ first,
+                                                       unsigned int const* __restrict__ rows,
                                                        unsigned long long const* __restrict__ off)
 {
     unsigned long long const i = static_cast<unsigned long long>(blockIdx.x) * BLOCK + threadIdx.x;
@@ -1046,7 +1069,9 @@ __global__ __launch_bounds__(BLOCK) void k_seg_scatter(SegGeom S, V src, V dst,
         unsigned int seg;
         int lane;
         segLane(S, c, seg, lane);
-        j = off[seg] + static_cast<unsigned long long>(rank[i]) * LANES + lane;
+        unsigned int const p = first[key] + rank[i];
+        unsigned int const R = rows[seg];
+        j = off[seg] + static_cast<unsigned long long>(p % R) * LANES + p / R;
     }
     else
         j = off[S.nsegs] + rank[i];
@@ -1063,26 +1088,23 @@ __global__ __launch_bounds__(BLOCK) void k_seg_scatter(SegGeom S, V src, V dst,
         dst.q[j] = src.q[i];
 }
 
-// marks the empty slots of every segment dead (one workgroup per segment)
+// marks the empty slots of every segment dead (one workgroup per segment): those past the
+// segment's particle count in its position order (the overflow segment: in slot order)
 template<class V>
 __global__ __launch_bounds__(BLOCK) void k_seg_pad(SegGeom S, V dst,
                                                    unsigned int const* __restrict__ rows,
                                                    unsigned long long const* __restrict__ off,
-                                                   unsigned int const* __restrict__ counts)
+                                                   unsigned int const* __restrict__ total)
 {
-    unsigned int const s = blockIdx.x;
-    unsigned int const slots = rows[s] * LANES;
+    unsigned int const s     = blockIdx.x;
+    unsigned int const R     = rows[s];
+    unsigned int const slots = R * LANES;
+    unsigned int const T     = total[s];
+    bool const packed        = s < static_cast<unsigned int>(S.nsegs);
     for (unsigned int x = threadIdx.x; x < slots; x += BLOCK)
     {
-        bool real;
-        if (s < static_cast<unsigned int>(S.nsegs))
-        {
-            unsigned int const key = laneKey(S, s, static_cast<int>(x % LANES));
-            real = key != DEAD_KEY && x / LANES < counts[key];
-        }
-        else
-            real = x < counts[S.ncells];
-        if (!real)
+        unsigned int const p = packed ? (x % LANES) * R + x / LANES : x;
+        if (p >= T)
             dst.ic[0][off[s] + x] = DEAD_CELL;
     }
 }
@@ -1116,13 +1138,14 @@ static void segCountT(SegGeom const& sg, V const& src, unsigned long long n, uns
     unsigned int const m  = static_cast<unsigned int>(sg.nsegs) + 1;
     unsigned int* counts    = work;
     unsigned int* itemCount = work + nc;
+    unsigned int* total     = work + nc + m;
     constexpr unsigned int SEGS_PER_BLOCK = BLOCK / LANES;
     k_seg_zero<<<(nc + BLOCK - 1) / BLOCK, BLOCK, 0, s>>>(counts, nc);
     if (n)
         k_seg_count<V><<<static_cast<unsigned int>((n + BLOCK - 1) / BLOCK), BLOCK, 0, s>>>(sg, src, n,
                                                                                           counts, rank);
     k_seg_rows<<<(m + SEGS_PER_BLOCK - 1) / SEGS_PER_BLOCK, BLOCK, 0, s>>>(sg, counts, tables.rows,
-                                                                           itemCount);
+                                                                           itemCount, total);
     k_seg_scan<<<1, 1024, 0, s>>>(tables.rows, tables.off, itemCount, tables.nItems, m);
 }
 
@@ -1133,13 +1156,14 @@ static void segScatterT(SegGeom const& sg, V const& src, V const& dst, unsigned 
 {
     unsigned int const nc = static_cast<unsigned int>(sg.ncells) + 1;
     unsigned int const m  = static_cast<unsigned int>(sg.nsegs) + 1;
-    unsigned int* counts    = work;
+    unsigned int* first     = work; // the counts, turned into first positions by k_seg_rows
     unsigned int* itemCount = work + nc;
+    unsigned int* total     = work + nc + m;
     k_seg_items<<<(m + BLOCK - 1) / BLOCK, BLOCK, 0, s>>>(tables.rows, itemCount, tables.items, m);
     if (n)
         k_seg_scatter<V><<<static_cast<unsigned int>((n + BLOCK - 1) / BLOCK), BLOCK, 0, s>>>(
-            sg, src, dst, n, rank, tables.off);
-    k_seg_pad<V><<<m, BLOCK, 0, s>>>(sg, dst, tables.rows, tables.off, counts);
+            sg, src, dst, n, rank, first, tables.rows, tables.off);
+    k_seg_pad<V><<<m, BLOCK, 0, s>>>(sg, dst, tables.rows, tables.off, total);
 }
 
 void launchSegCount(SegGeom const& sg, PView const& src, unsigned long long n, unsigned int* work,
