@@ -499,7 +499,7 @@ struct FaradaySpec
 };
 
 // borderMax false: the border max of the totals is left to the next electron closure
-// (launchElectronsMax), which a captured step runs before anything reads the totals; far: the
+// (launchAmpereElectrons), which a captured step runs before anything reads the totals; far: the
 // Faraday that follows in the step, launched with the fold
 phx_status enqueueMoveIonsFast(phx_ctx* c, double dt, int mode, bool borderMax = true,
                                FaradaySpec const* far = nullptr)
@@ -605,29 +605,25 @@ void enqueueCorrector(phx_ctx* c, double dt)
               c->Pe, cvec(c->em[PHX_EM] + 3), cvec(c->J), vec(c->em[PHX_EM]), c->stream, 1);
 }
 
-// the electron closure, with the border max of the totals first when the last moveIons left it
-void electronsAfterFold(phx_ctx* c, bool borderMax)
+// Ampere on B (em set bset) with J's ghost layers and the electron closure, in one launch;
+// fillB: B's ghost layers are filled by the same launch
+void ampereElectronsAfterFold(phx_ctx* c, int bset, bool borderMax, bool fillB)
 {
-    if (!borderMax)
-    {
-        electrons(c);
-        return;
-    }
     double const* src[5] = {c->tot5[0], c->tot5[1], c->tot5[2], c->tot5[3], c->tot5[4]};
     double* dst[5]       = {c->rho, c->massD, c->Vi[0], c->Vi[1], c->Vi[2]};
-    launchElectronsMax(c->G, c->cfg.Te, cvec(c->J), src, dst, vec(c->Ve), c->Pe, c->stream);
+    launchAmpereElectrons(c->G, c->cfg.Te, vec(c->em[bset] + 3), vec(c->J), src, dst, cvec(c->Vi),
+                          c->rho, vec(c->Ve), c->Pe, borderMax, fillB, c->stream);
 }
 
-// predictor + average of a periodic level: Faraday (unless the previous fold's launch ran it)
-// and Ampere with their ghost layers, the electron closure (with the pending border max), then
-// Ohm and the average in one kernel
+// predictor + average of a periodic level: Faraday with its ghost layers (unless the previous
+// fold's launch ran it), Ampere with its ghost layers and the electron closure (with the pending
+// border max) in one launch, then Ohm and the average in one kernel
 void enqueuePredictorAverageFast(phx_ctx* c, double dt, int eset, bool borderMax, bool faraday)
 {
     if (faraday)
         launchFaradayFill(c->G, cvec(c->em[PHX_EM] + 3), cvec(c->em[eset]), vec(c->em[PHX_EM_PRED] + 3),
                           dt, c->stream);
-    launchAmpereFill(c->G, cvec(c->em[PHX_EM_PRED] + 3), vec(c->J), c->stream);
-    electronsAfterFold(c, borderMax);
+    ampereElectronsAfterFold(c, PHX_EM_PRED, borderMax, false);
     launchOhmAverage(c->G, c->cfg.eta, c->cfg.nu, c->cfg.hyper_mode, c->patch.level, c->rho,
                      cvec(c->Ve), c->Pe, cvec(c->em[PHX_EM_PRED] + 3), cvec(c->J),
                      vec(c->em[PHX_EM_PRED]), cvec(c->em[PHX_EM]), cvec(c->em[PHX_EM] + 3),
@@ -636,12 +632,11 @@ void enqueuePredictorAverageFast(phx_ctx* c, double dt, int eset, bool borderMax
 }
 
 // SolverPPC::corrector_ of a periodic level after the in-place Faraday (run with the last
-// fold): the B fill and Ampere in one launch, the electron closure with the pending border max,
-// Ohm with its ghost layers
+// fold): the B fill, Ampere and the electron closure with the pending border max in one launch,
+// then Ohm with its ghost layers
 void enqueueCorrectorFast(phx_ctx* c)
 {
-    launchAmpereFillB(c->G, vec(c->em[PHX_EM] + 3), vec(c->J), c->stream);
-    electronsAfterFold(c, true);
+    ampereElectronsAfterFold(c, PHX_EM, true, true);
     launchOhm(c->G, c->cfg.eta, c->cfg.nu, c->cfg.hyper_mode, c->patch.level, c->rho, cvec(c->Ve),
               c->Pe, cvec(c->em[PHX_EM] + 3), cvec(c->J), vec(c->em[PHX_EM]), c->stream, 1);
 }

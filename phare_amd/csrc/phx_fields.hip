@@ -744,43 +744,6 @@ __global__ __launch_bounds__(FB) void k_ampere_fill(Geom G0, CVecPtr B, VecPtr J
     }
 }
 
-// Ampere + fillCurrentGhosts on a B whose ghost layers are not filled yet, and that fill
-// (fillMagneticGhosts) in the same launch: Ampere reads every B node at its periodic source,
-// which is what the filled ghost node holds; grid row c also writes B component c's ghost nodes
-// from their sources. Used after the corrector's in-place Faraday.
-template<int DIM>
-__global__ __launch_bounds__(FB) void k_ampere_fill_b(Geom G0, VecPtr B, VecPtr J)
-{
-    GeomD<DIM> const G(G0);
-    long const t = static_cast<long>(blockIdx.x) * FB + threadIdx.x;
-    int i, j, k;
-    if (!nodeOf(G, QMOM, t, i, j, k))
-        return;
-    int const c = static_cast<int>(blockIdx.y); // one component per grid row
-    // loads first: the B source of this node (if a ghost), then curl B from sources
-    bool const onB = inShape(G, QBX + c, i, j, k);
-    int sb[3]      = {i, j, k};
-    bool const gB  = onB && wrapNode(G, QBX + c, sb);
-    double const bsrc = gB ? B.c[c][G.idx(QBX + c, sb[0], sb[1], sb[2])] : 0.;
-    bool const onJ = inShape(G, QJX + c, i, j, k);
-    double jv      = 0.;
-    if (onJ)
-    {
-        int s[3] = {i, j, k};
-        wrapNode(G, QJX + c, s);
-        auto bAt = [&](int d, int a, int b, int cc) {
-            int w[3] = {a, b, cc};
-            wrapNode(G, QBX + d, w);
-            return B.c[d][G.idx(QBX + d, w[0], w[1], w[2])];
-        };
-        jv = ampereAtFn(G, bAt, c, s[0], s[1], s[2]);
-    }
-    if (gB)
-        B.c[c][G.idx(QBX + c, i, j, k)] = bsrc;
-    if (onJ)
-        J.c[c][G.idx(QJX + c, i, j, k)] = jv;
-}
-
 // ghost fill of three components in one launch
 template<int DIM>
 __global__ __launch_bounds__(FB) void k_fill3(Geom G0, int q0, VecPtr F)
@@ -1001,6 +964,30 @@ struct Five
     double* dst[5];
 };
 
+// the five totals at primal node (i, j, k) (index n): the max over its periodic images of the
+// values in F.src. A non-overlapping image reads the node itself, which leaves the max as it is,
+// so every load is unconditional and its value always used (the loads are issued together rather
+// than each behind its own branch).
+template<int DIM>
+__device__ __forceinline__ void maxOverImages(GeomD<DIM> const& G, Five const& F, int i, int j, int k,
+                                              int n, double* v)
+{
+    int const c[3] = {i, j, k};
+    Images<DIM> I;
+    imagesOf(G, c, I);
+#pragma unroll
+    for (int q = 0; q < 5; ++q)
+    {
+        v[q] = F.src[q][n];
+#pragma unroll
+        for (int x = 0; x < Images<DIM>::N; ++x)
+        {
+            double const o = F.src[q][I.valid[x] ? n - I.off[x] : n];
+            v[q]           = v[q] < o ? o : v[q];
+        }
+    }
+}
+
 // fillIonBorders: max over the periodic images (snapshot in src)
 template<int DIM>
 __global__ __launch_bounds__(FB) void k_max5(Geom G0, Five F)
@@ -1010,56 +997,76 @@ __global__ __launch_bounds__(FB) void k_max5(Geom G0, Five F)
     int i, j, k;
     if (!nodeOf(G, QMOM, t, i, j, k))
         return;
-    int const c[3] = {i, j, k};
-    int const n    = G.idx(QMOM, i, j, k);
-    Images<DIM> I;
-    imagesOf(G, c, I);
-    // unconditional loads (a non-overlapping image reads the node itself), stores after all loads
+    int const n = G.idx(QMOM, i, j, k);
     double v[5];
-#pragma unroll
-    for (int q = 0; q < 5; ++q)
-    {
-        v[q] = F.src[q][n];
-#pragma unroll
-        for (int x = 0; x < Images<DIM>::N; ++x)
-        {
-            double const o = F.src[q][I.valid[x] ? n - I.off[x] : n];
-            v[q]           = I.valid[x] && v[q] < o ? o : v[q];
-        }
-    }
+    maxOverImages(G, F, i, j, k, n, v); // every load before the stores
 #pragma unroll
     for (int q = 0; q < 5; ++q)
         F.dst[q][n] = v[q];
 }
 
-// fillIonBorders and Electrons::update in one launch (periodic level): the node's five totals
-// are the max over its periodic images of the values before the max (F.src), taken as k_max5
-// takes them and stored in F.dst (rho, massDensity, Vi); the closure then uses them as
-// k_electrons does (Pe everywhere, Ve on the physical box)
-template<int DIM>
-__global__ __launch_bounds__(FB) void k_electrons_max(Geom G0, double Te, CVecPtr J, Five F,
-                                                      VecPtr Ve, double* Pe)
+// Ampere with J's ghost layers and the electron closure in one launch (periodic level). A thread
+// stores J at its node (every component whose shape holds the node) and runs the closure at the
+// same primal node; the J values the closure projects are recomputed from B, each one the value
+// k_ampere_fill stores at that node: curl B at the node's periodic source. MAX: the closure takes
+// the totals from the border max of F.src over the node's periodic images (stored in F.dst), as
+// k_max5 takes it, else from Ne, Vi as k_electrons does. BFILL: B's ghost layers are stale
+// (after the corrector's in-place Faraday); curl B reads B at periodic sources, which is what
+// the filled ghost nodes hold, and the thread also writes the node's B ghost values from their
+// sources (fillMagneticGhosts).
+template<int DIM, bool MAX, bool BFILL>
+__global__ __launch_bounds__(FB) void k_ampere_electrons(Geom G0, double Te, VecPtr B, VecPtr J,
+                                                         Five F, CVecPtr Vi, double const* Ne,
+                                                         VecPtr Ve, double* Pe)
 {
     GeomD<DIM> const G(G0);
     long const t = static_cast<long>(blockIdx.x) * FB + threadIdx.x;
     int i, j, k;
     if (!nodeOf(G, QMOM, t, i, j, k))
         return;
-    int const c[3] = {i, j, k};
-    int const n    = G.idx(QMOM, i, j, k);
-    Images<DIM> I;
-    imagesOf(G, c, I);
-    double v[5];
+    auto bAt = [&](int d, int a, int b, int cc) {
+        int w[3] = {a, b, cc};
+        if (BFILL)
+            wrapNode(G, QBX + d, w);
+        return B.c[d][G.idx(QBX + d, w[0], w[1], w[2])];
+    };
+    auto jAt = [&](int c, int a, int b, int cc) {
+        int s[3] = {a, b, cc};
+        wrapNode(G, QJX + c, s);
+        return ampereAtFn(G, bAt, c, s[0], s[1], s[2]);
+    };
+    // every load before the first store
+    bool onJ[3];
+    double jv[3];
 #pragma unroll
-    for (int q = 0; q < 5; ++q)
+    for (int c = 0; c < 3; ++c)
     {
-        v[q] = F.src[q][n];
+        onJ[c] = inShape(G, QJX + c, i, j, k);
+        jv[c]  = onJ[c] ? jAt(c, i, j, k) : 0.;
+    }
+    bool gB[3] = {false, false, false};
+    double bsrc[3] = {0., 0., 0.};
+    if (BFILL)
+    {
 #pragma unroll
-        for (int x = 0; x < Images<DIM>::N; ++x)
+        for (int c = 0; c < 3; ++c)
         {
-            double const o = F.src[q][I.valid[x] ? n - I.off[x] : n];
-            v[q]           = I.valid[x] && v[q] < o ? o : v[q];
+            int sb[3] = {i, j, k};
+            gB[c]     = inShape(G, QBX + c, i, j, k) && wrapNode(G, QBX + c, sb);
+            bsrc[c]   = gB[c] ? B.c[c][G.idx(QBX + c, sb[0], sb[1], sb[2])] : 0.;
         }
+    }
+    int const n = G.idx(QMOM, i, j, k);
+    double v[5];
+    if (MAX)
+        maxOverImages(G, F, i, j, k, n, v);
+    else
+    {
+        v[0] = Ne[n];
+        v[1] = 0.;
+        v[2] = Vi.c[0][n];
+        v[3] = Vi.c[1][n];
+        v[4] = Vi.c[2][n];
     }
     int lo[3], hi[3];
     physBox(G, QMOM, lo, hi);
@@ -1068,13 +1075,24 @@ __global__ __launch_bounds__(FB) void k_electrons_max(Geom G0, double Te, CVecPt
     if (phys)
     {
         // JxToMoments / JyToMoments / JzToMoments: dual->primal along the component's direction
-        jx = project(G, J.c[0], QJX, i, j, k, D2P, PN, PN);
-        jy = project(G, J.c[1], QJY, i, j, k, PN, D2P, PN);
-        jz = project(G, J.c[2], QJZ, i, j, k, PN, PN, D2P);
+        jx = projectFn(G, [&](int a, int b, int cc) { return jAt(0, a, b, cc); }, i, j, k, D2P, PN, PN);
+        jy = projectFn(G, [&](int a, int b, int cc) { return jAt(1, a, b, cc); }, i, j, k, PN, D2P, PN);
+        jz = projectFn(G, [&](int a, int b, int cc) { return jAt(2, a, b, cc); }, i, j, k, PN, PN, D2P);
     }
 #pragma unroll
-    for (int q = 0; q < 5; ++q)
-        F.dst[q][n] = v[q];
+    for (int c = 0; c < 3; ++c)
+    {
+        if (onJ[c])
+            J.c[c][G.idx(QJX + c, i, j, k)] = jv[c];
+        if (BFILL && gB[c])
+            B.c[c][G.idx(QBX + c, i, j, k)] = bsrc[c];
+    }
+    if (MAX)
+    {
+#pragma unroll
+        for (int q = 0; q < 5; ++q)
+            F.dst[q][n] = v[q];
+    }
     Pe[n] = v[0] * Te;
     if (phys)
     {
@@ -1086,7 +1104,39 @@ __global__ __launch_bounds__(FB) void k_electrons_max(Geom G0, double Te, CVecPt
 
 unsigned int gridFor(long n) { return static_cast<unsigned int>((n + FB - 1) / FB); }
 
+template<bool MAX, bool BFILL>
+void ampereElectronsT(Geom const& G, double Te, VecPtr B, VecPtr J, Five const& F, CVecPtr Vi,
+                      double const* Ne, VecPtr Ve, double* Pe, hipStream_t s)
+{
+    unsigned int const g = gridFor(G.size(QMOM));
+    switch (G.dim)
+    {
+        case 1: k_ampere_electrons<1, MAX, BFILL><<<g, FB, 0, s>>>(G, Te, B, J, F, Vi, Ne, Ve, Pe); break;
+        case 2: k_ampere_electrons<2, MAX, BFILL><<<g, FB, 0, s>>>(G, Te, B, J, F, Vi, Ne, Ve, Pe); break;
+        case 3: k_ampere_electrons<3, MAX, BFILL><<<g, FB, 0, s>>>(G, Te, B, J, F, Vi, Ne, Ve, Pe); break;
+    }
+}
+
 } // namespace
+
+void launchAmpereElectrons(Geom const& G, double Te, VecPtr B, VecPtr J, double const* const* src,
+                           double* const* dst, CVecPtr Vi, double const* Ne, VecPtr Ve, double* Pe,
+                           bool borderMax, bool fillB, hipStream_t s)
+{
+    Five F{};
+    if (borderMax)
+        for (int q = 0; q < 5; ++q)
+        {
+            F.src[q] = src[q];
+            F.dst[q] = dst[q];
+        }
+    if (!borderMax)
+        ampereElectronsT<false, false>(G, Te, B, J, F, Vi, Ne, Ve, Pe, s);
+    else if (!fillB)
+        ampereElectronsT<true, false>(G, Te, B, J, F, Vi, Ne, Ve, Pe, s);
+    else
+        ampereElectronsT<true, true>(G, Te, B, J, F, Vi, Ne, Ve, Pe, s);
+}
 
 void launchFaraday(Geom const& G, CVecPtr B, CVecPtr E, VecPtr Bnew, double dt, hipStream_t s)
 {
@@ -1213,15 +1263,6 @@ void launchAmpereFill(Geom const& G, CVecPtr B, VecPtr J, hipStream_t s)
         case 3: k_ampere_fill<3><<<dim3(gridFor(G.size(QMOM)), 3), FB, 0, s>>>(G, B, J); break;
     }
 }
-void launchAmpereFillB(Geom const& G, VecPtr B, VecPtr J, hipStream_t s)
-{
-    switch (G.dim)
-    {
-        case 1: k_ampere_fill_b<1><<<dim3(gridFor(G.size(QMOM)), 3), FB, 0, s>>>(G, B, J); break;
-        case 2: k_ampere_fill_b<2><<<dim3(gridFor(G.size(QMOM)), 3), FB, 0, s>>>(G, B, J); break;
-        case 3: k_ampere_fill_b<3><<<dim3(gridFor(G.size(QMOM)), 3), FB, 0, s>>>(G, B, J); break;
-    }
-}
 void launchFill3(Geom const& G, int q0, VecPtr F, hipStream_t s)
 {
     switch (G.dim)
@@ -1326,23 +1367,6 @@ void launchFoldFaraday(Geom const& G, MomSetArgs const& A, double* trho, double*
         case 5: k_fold_faraday<2, true><<<grid, FB, 0, s>>>(G, M, trho, tmd, tvi, B, E, Bn, dt); break;
         case 6: k_fold_faraday<3, false><<<grid, FB, 0, s>>>(G, M, trho, tmd, tvi, B, E, Bn, dt); break;
         case 7: k_fold_faraday<3, true><<<grid, FB, 0, s>>>(G, M, trho, tmd, tvi, B, E, Bn, dt); break;
-    }
-}
-
-void launchElectronsMax(Geom const& G, double Te, CVecPtr J, double const* const* src,
-                       double* const* dst, VecPtr Ve, double* Pe, hipStream_t s)
-{
-    Five F{};
-    for (int q = 0; q < 5; ++q)
-    {
-        F.src[q] = src[q];
-        F.dst[q] = dst[q];
-    }
-    switch (G.dim)
-    {
-        case 1: k_electrons_max<1><<<gridFor(G.size(QMOM)), FB, 0, s>>>(G, Te, J, F, Ve, Pe); break;
-        case 2: k_electrons_max<2><<<gridFor(G.size(QMOM)), FB, 0, s>>>(G, Te, J, F, Ve, Pe); break;
-        case 3: k_electrons_max<3><<<gridFor(G.size(QMOM)), FB, 0, s>>>(G, Te, J, F, Ve, Pe); break;
     }
 }
 

@@ -231,9 +231,6 @@ struct MomSetArgs
 void launchFaradayFill(Geom const& G, CVecPtr B, CVecPtr E, VecPtr Bnew, double dt, hipStream_t s);
 void launchAmpereFill(Geom const& G, CVecPtr B, VecPtr J, hipStream_t s);
 void launchFill3(Geom const& G, int q0, VecPtr F, hipStream_t s);
-// the B ghost fill and Ampere with its ghost layers in one launch, for a B whose ghost layers
-// are stale (after an in-place Faraday): Ampere reads B at periodic sources
-void launchAmpereFillB(Geom const& G, VecPtr B, VecPtr J, hipStream_t s);
 // mirror (nullable): 6 FP32 arrays (Ex..Bz) that also receive the averaged set, for the
 // mixed-precision push's gather
 void launchAverageFill(Geom const& G, CVecPtr E, CVecPtr Ep, CVecPtr B, CVecPtr Bp, VecPtr Ea,
@@ -247,9 +244,12 @@ void launchMax5(Geom const& G, double const* const* src, double* const* dst, hip
 // with Bn's ghost layers, or (inPlace: Bn is B) on B's physical nodes
 void launchFoldFaraday(Geom const& G, MomSetArgs const& A, double* trho, double* tmd, VecPtr tvi,
                        CVecPtr B, CVecPtr E, VecPtr Bn, double dt, bool inPlace, hipStream_t s);
-// the border max of the totals (src: before the max; dst: rho, massDensity, Vi) and the electron
-// closure in one launch
-void launchElectronsMax(Geom const& G, double Te, CVecPtr J, double const* const* src,
-                       double* const* dst, VecPtr Ve, double* Pe, hipStream_t s);
+// Ampere with J's ghost layers and the electron closure in one launch (periodic level):
+// borderMax: the closure takes the totals from the border max of src (stored in dst: rho,
+// massDensity, Vi), else from Ne, Vi; fillB (with borderMax): B's ghost layers are stale and are
+// filled by the same launch
+void launchAmpereElectrons(Geom const& G, double Te, VecPtr B, VecPtr J, double const* const* src,
+                           double* const* dst, CVecPtr Vi, double const* Ne, VecPtr Ve, double* Pe,
+                           bool borderMax, bool fillB, hipStream_t s);
 
 } // namespace phx

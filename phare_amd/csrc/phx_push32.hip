@@ -495,7 +495,7 @@ __device__ __forceinline__ void pushSlot32(PushArgs const& A, Push32 const& K, P
 // and SEG_ROWS of its rows), in place
 // ---------------------------------------------------------------------------------------------
 #ifndef PHX_SEG_WAVES32
-#define PHX_SEG_WAVES32 5 // waves per SIMD of the mixed-precision push (fewer registers than FP64)
+#define PHX_SEG_WAVES32 4 // waves per SIMD the register allocation of the mixed-precision push targets
 #endif
 template<int D, int O>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(PHX_SEG_WAVES32)))
