@@ -38,7 +38,7 @@ HBM_PEAK_GBS = 8000.0
 # this script, FETCH_SIZE scaled x2 and WRITE_SIZE x1 by a 1 GiB device copy's readings; keyed by
 # workload and precision family (k_push_seg / k_push_seg32), measured for the default Harris
 # workload only
-MEASURED_TRAFFIC = {("harris2d", "fp64"): 426.2e6, ("harris2d", "fp32"): 246.5e6}
+MEASURED_TRAFFIC = {("harris2d", "fp64"): 426.4e6, ("harris2d", "fp32"): 246.6e6}
 
 
 def _hip():
@@ -181,7 +181,7 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="harris2d")
-    ap.add_argument("--cpu-steps", type=int, default=5)
+    ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision family")
     ap.add_argument("--no-drift-run", action="store_true",
