@@ -396,6 +396,51 @@ def test_sort_groups_particles_by_segment(dim):
     g.close()
 
 
+def _crowded_particles(rng, ncells):
+    """two crowded cells in one segment, a crowded cell in the periodic corner, a sparse band of
+    columns, every other segment empty"""
+    cells = [((3, 70), 3000), ((3, 71), 700), ((ncells[0] - 1, ncells[1] - 1), 800)]
+    ic = [np.tile(np.array(c, np.int32), (n, 1)) for c, n in cells]
+    band = np.stack([rng.integers(10, 13, 500), rng.integers(0, ncells[1], 500)], 1)
+    ic = np.concatenate(ic + [band.astype(np.int32)])
+    n = len(ic)
+    return dict(icell=ic, delta=rng.random((n, 2)), v=rng.standard_normal((n, 3)),
+                weight=0.5 + rng.random(n), charge=np.ones(n))
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_sorted_push_crowded_cells_and_empty_segments(precision):
+    """cell-segmented storage at its extremes: a crowded cell fills most lanes of its segment's
+    rows (same-cell LDS deposits within a row), segments with no particle, a crowded cell on the
+    periodic corner; both moveIons modes under uniform fields against the oracle (FP64 1e-12,
+    mixed precision 1e-5 on the moments)"""
+    _say(f"[gpu] sorted push, crowded cells and empty segments, {precision}")
+    ncells, dx, dt = (20, 150), (0.4, 0.4), 0.05
+    prec = phx.PRECISION_FP32 if precision == "fp32" else phx.PRECISION_FP64
+    rng = np.random.default_rng(31)
+    p = _crowded_particles(rng, ncells)
+    g = phx.Level(2, 2, ncells, dx, capacity=len(p["weight"]), precision=prec)
+    o = O.Sim(2, 2, ncells, dx)
+    E, B = (0.01, -0.02, 0.005), (0.5, 0.1, 1.0)
+    for sim, fe, fb in ((g, phx.F_EAVG, phx.F_BAVG), (o, O.F_EAVG, O.F_BAVG)):
+        sim.set_particles(0, phx.DOMAIN, p)
+        for c in range(3):
+            sim.set(fe + c, np.full(sim.shape(fe + c), E[c]))
+            sim.set(fb + c, np.full(sim.shape(fb + c), B[c]))
+    g.sort_particles()
+    rtol = U.RTOL if precision == "fp64" else 1e-5
+    for mode in (phx.MODE_DOMAIN_ONLY, phx.MODE_ALL):
+        g.move_ions(dt, mode)
+        o.move_ions(dt, mode)
+        for k in range(5):
+            U.assert_close(g.get(phx.f_pop(0, k)), o.get(O.f_pop(0, k)), rtol,
+                           what=f"mode {mode} moment {k}")
+    assert g.count() == o.count() == len(p["weight"])
+    if precision == "fp64":
+        U.assert_same_particles(g.get_particles(), o.get_particles(), U.RTOL)
+    g.close()
+
+
 @pytest.mark.parametrize("interval", [1, 3])
 def test_advance_with_frequent_sorts(interval):
     """sorting every `interval` PPC steps (the first sort moves the linear upload into segmented
