@@ -676,26 +676,17 @@ __global__ __launch_bounds__(BLOCK) void k_push(PushArgs A, PView in, PView out)
 }
 
 // ---------------------------------------------------------------------------------------------
-// fused push + deposit over cell-segmented storage: one workgroup per work item (a segment and
-// SEG_ROWS of its rows); wavefront w takes rows first + w, first + w + NW, ...
+// fused push + deposit over cell-segmented storage
 // ---------------------------------------------------------------------------------------------
+// rows [rA, rB) of segment seg through the LDS tile: zero the tile, push the rows (wavefront w
+// takes rows rA + w, rA + w + NW, ...), flush the tile. Every work-item of the workgroup calls it.
 template<int D, int O, bool RN>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(PHX_SEG_WAVES)))
-void k_push_seg(PushArgs A, PView in)
+__device__ __forceinline__ void pushPiece(PushArgs const& A, PView const& in, double* lds,
+                                          unsigned int seg, unsigned int rA, unsigned int rB)
 {
     PView const& out = in; // segmented arrays are pushed in place
     using C = Cfg<D, O>;
-    extern __shared__ double lds[];
-    unsigned int const item = blockIdx.x;
-    unsigned int const nItems = *A.st.nItems;
-    if (item >= nItems)
-        return;
-    pc::ktBegin(A.kt);
     Geom const& G          = A.G;
-    SegGeom const& S       = A.sg;
-    unsigned int const seg = A.st.items[2 * item];
-    unsigned int const r0  = A.st.items[2 * item + 1];
-    unsigned int const R   = A.st.rows[seg];
     unsigned long long const off = A.st.off[seg];
     int const t    = threadIdx.x;
     int const lane = t & 63;
@@ -706,36 +697,37 @@ void k_push_seg(PushArgs A, PView in)
     // motion, drift since the sort); the overflow segment has no cell range
     TileView T;
     tileView(T, lds, A.tile);
-    pc::segTileOrigin<D>(G, S, A.tile, seg, T.org);
+    pc::segTileOrigin<D>(G, A.sg, A.tile, seg, T.org);
     __syncthreads();
 
     Acc<D, O> acc;
     if constexpr (C::ACC)
         accZero(acc);
-    unsigned int const rEnd = min(r0 + static_cast<unsigned int>(SEG_ROWS), R);
-#ifdef PHX_PREFETCH
-    // the next row's particle is read while this row's is pushed
-    unsigned int r = r0 + (t >> 6);
-    Part<D> next;
-    if (r < rEnd)
-        loadPart<D>(in, off + static_cast<unsigned long long>(r) * LANES + lane, next);
-    for (; r < rEnd; r += NW)
-    {
-        Part<D> const cur = next;
-        if (r + NW < rEnd)
-            loadPart<D>(in, off + static_cast<unsigned long long>(r + NW) * LANES + lane, next);
-        pushLoaded<D, O, RN>(A, out, false, off + static_cast<unsigned long long>(r) * LANES + lane,
-                             cur, T, copy, acc);
-    }
-#else
-    for (unsigned int r = r0 + (t >> 6); r < rEnd; r += NW)
+    for (unsigned int r = rA + (t >> 6); r < rB; r += NW)
         pushParticle<D, O, RN>(A, in, out, off + static_cast<unsigned long long>(r) * LANES + lane, T,
                                copy, acc);
-#endif
     if constexpr (C::ACC)
         accFlush<D, O>(acc, G, T, copy, A.mom);
     __syncthreads();
     flushTile<D, RN>(G, T, A.tile.nodes, A.mom, A.charge);
+}
+
+// one workgroup per work item (a segment and its share of the segment's rows, at most SEG_ROWS)
+template<int D, int O, bool RN>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(PHX_SEG_WAVES)))
+void k_push_seg(PushArgs A, PView in)
+{
+    extern __shared__ double lds[];
+    unsigned int const item = blockIdx.x;
+    unsigned int const nItems = *A.st.nItems;
+    if (item >= nItems)
+        return;
+    pc::ktBegin(A.kt);
+    unsigned int const seg = A.st.items[2 * item];
+    unsigned int const R   = A.st.rows[seg];
+    unsigned int r0, rEnd;
+    pc::itemRows(A.st.items[2 * item + 1], R, r0, rEnd);
+    pushPiece<D, O, RN>(A, in, lds, seg, r0, rEnd);
     if (A.kt)
     {
         __syncthreads();
@@ -1042,7 +1034,7 @@ __global__ __launch_bounds__(BLOCK) void k_seg_items(unsigned int const* __restr
     for (unsigned int j = 0; j < k; ++j)
     {
         items[2 * (itemOff[s] + j)]     = s;
-        items[2 * (itemOff[s] + j) + 1] = j * SEG_ROWS;
+        items[2 * (itemOff[s] + j) + 1] = j; // rows of item j: itemRows
     }
 }
 

@@ -491,26 +491,20 @@ __device__ __forceinline__ void pushSlot32(PushArgs const& A, Push32 const& K, P
 } // namespace
 
 // ---------------------------------------------------------------------------------------------
-// fused push + deposit over cell-segmented FP32 storage: one workgroup per work item (a segment
-// and SEG_ROWS of its rows), in place
+// fused push + deposit over cell-segmented FP32 storage, in place
 // ---------------------------------------------------------------------------------------------
 #ifndef PHX_SEG_WAVES32
 #define PHX_SEG_WAVES32 4 // waves per SIMD the register allocation of the mixed-precision push targets
 #endif
-template<int D, int O>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(PHX_SEG_WAVES32)))
-void k_push_seg32(PushArgs A, PView32 v)
+namespace
 {
-    extern __shared__ double lds32[];
-    unsigned int const item = blockIdx.x;
-    unsigned int const nItems = *A.st.nItems;
-    if (item >= nItems)
-        return;
-    pc::ktBegin(A.kt);
+// rows [rA, rB) of segment seg through the FP64 LDS tile: zero the tile, push the rows (wavefront
+// w takes rows rA + w, rA + w + NW, ...), flush the tile. Every work-item of the workgroup calls it.
+template<int D, int O>
+__device__ __forceinline__ void pushPiece32(PushArgs const& A, PView32 const& v, double* lds32,
+                                            unsigned int seg, unsigned int rA, unsigned int rB)
+{
     Geom const& G          = A.G;
-    unsigned int const seg = A.st.items[2 * item];
-    unsigned int const r0  = A.st.items[2 * item + 1];
-    unsigned int const R   = A.st.rows[seg];
     unsigned long long const off = A.st.off[seg];
     int const t    = threadIdx.x;
     int const lane = t & 63;
@@ -537,8 +531,7 @@ void k_push_seg32(PushArgs A, PView32 v)
     K.coef  = static_cast<float>(A.coef);
     __syncthreads();
 
-    unsigned int const rEnd = min(r0 + static_cast<unsigned int>(SEG_ROWS), R);
-    for (unsigned int r = r0 + (t >> 6); r < rEnd; r += NW)
+    for (unsigned int r = rA + (t >> 6); r < rB; r += NW)
         pushSlot32<D, O>(A, K, v, off + static_cast<unsigned long long>(r) * LANES + lane, T);
     __syncthreads();
 
@@ -572,6 +565,25 @@ void k_push_seg32(PushArgs A, PView32 v)
             }
         }
     }
+}
+} // namespace
+
+// one workgroup per work item (a segment and its share of the segment's rows, at most SEG_ROWS)
+template<int D, int O>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(PHX_SEG_WAVES32)))
+void k_push_seg32(PushArgs A, PView32 v)
+{
+    extern __shared__ double lds32[];
+    unsigned int const item = blockIdx.x;
+    unsigned int const nItems = *A.st.nItems;
+    if (item >= nItems)
+        return;
+    pc::ktBegin(A.kt);
+    unsigned int const seg = A.st.items[2 * item];
+    unsigned int const R   = A.st.rows[seg];
+    unsigned int r0, rEnd;
+    pc::itemRows(A.st.items[2 * item + 1], R, r0, rEnd);
+    pushPiece32<D, O>(A, v, lds32, seg, r0, rEnd);
     if (A.kt)
     {
         __syncthreads();

@@ -5,7 +5,7 @@
 #include "phx_kernels.hpp"
 
 #ifndef PHX_SEG_ROWS
-#define PHX_SEG_ROWS 16 // segment rows per workgroup of the segmented push
+#define PHX_SEG_ROWS 32 // most segment rows per workgroup of the segmented push
 #endif
 #ifndef PHX_TILE_SPREAD
 #define PHX_TILE_SPREAD 1
@@ -63,6 +63,16 @@ __device__ __forceinline__ void ktEnd(KTime* kt, unsigned int working)
     atomicAdd(&kt->total, e - s);
     atomicAdd(&kt->count, 1u);
     atomicExch(&kt->done, 0u);
+}
+
+// rows [r0, r1) of work item j of a segment of R rows: the segment's ceil(R / SEG_ROWS) items
+// share its rows evenly (at most SEG_ROWS each)
+__device__ __forceinline__ void itemRows(unsigned int j, unsigned int R, unsigned int& r0,
+                                         unsigned int& r1)
+{
+    unsigned int const k = (R + SEG_ROWS - 1) / SEG_ROWS;
+    r0 = static_cast<unsigned int>(static_cast<unsigned long long>(j) * R / k);
+    r1 = static_cast<unsigned int>(static_cast<unsigned long long>(j + 1) * R / k);
 }
 
 // local node index of the LDS tile origin of segment `seg` (the segment's first cell less the
