@@ -605,14 +605,14 @@ void enqueueCorrector(phx_ctx* c, double dt)
               c->Pe, cvec(c->em[PHX_EM] + 3), cvec(c->J), vec(c->em[PHX_EM]), c->stream, 1);
 }
 
-// Ampere on B (em set bset) with J's ghost layers and the electron closure, in one launch;
-// fillB: B's ghost layers are filled by the same launch
-void ampereElectronsAfterFold(phx_ctx* c, int bset, bool borderMax, bool fillB)
+// Ampere on B (em set bset, ghost layers filled) with J's ghost layers and the electron closure,
+// in one launch
+void ampereElectronsAfterFold(phx_ctx* c, int bset, bool borderMax)
 {
     double const* src[5] = {c->tot5[0], c->tot5[1], c->tot5[2], c->tot5[3], c->tot5[4]};
     double* dst[5]       = {c->rho, c->massD, c->Vi[0], c->Vi[1], c->Vi[2]};
-    launchAmpereElectrons(c->G, c->cfg.Te, vec(c->em[bset] + 3), vec(c->J), src, dst, cvec(c->Vi),
-                          c->rho, vec(c->Ve), c->Pe, borderMax, fillB, c->stream);
+    launchAmpereElectrons(c->G, c->cfg.Te, cvec(c->em[bset] + 3), vec(c->J), src, dst, cvec(c->Vi),
+                          c->rho, vec(c->Ve), c->Pe, borderMax, c->stream);
 }
 
 // predictor + average of a periodic level: Faraday with its ghost layers (unless the previous
@@ -623,7 +623,7 @@ void enqueuePredictorAverageFast(phx_ctx* c, double dt, int eset, bool borderMax
     if (faraday)
         launchFaradayFill(c->G, cvec(c->em[PHX_EM] + 3), cvec(c->em[eset]), vec(c->em[PHX_EM_PRED] + 3),
                           dt, c->stream);
-    ampereElectronsAfterFold(c, PHX_EM_PRED, borderMax, false);
+    ampereElectronsAfterFold(c, PHX_EM_PRED, borderMax);
     launchOhmAverage(c->G, c->cfg.eta, c->cfg.nu, c->cfg.hyper_mode, c->patch.level, c->rho,
                      cvec(c->Ve), c->Pe, cvec(c->em[PHX_EM_PRED] + 3), cvec(c->J),
                      vec(c->em[PHX_EM_PRED]), cvec(c->em[PHX_EM]), cvec(c->em[PHX_EM] + 3),
@@ -631,12 +631,12 @@ void enqueuePredictorAverageFast(phx_ctx* c, double dt, int eset, bool borderMax
                      mixed(c) ? c->em32 : nullptr);
 }
 
-// SolverPPC::corrector_ of a periodic level after the in-place Faraday (run with the last
-// fold): the B fill, Ampere and the electron closure with the pending border max in one launch,
-// then Ohm with its ghost layers
+// SolverPPC::corrector_ of a periodic level after the in-place Faraday (run with the last fold,
+// which also fills B's ghost layers): Ampere and the electron closure with the pending border max
+// in one launch, then Ohm with its ghost layers
 void enqueueCorrectorFast(phx_ctx* c)
 {
-    ampereElectronsAfterFold(c, PHX_EM, true, true);
+    ampereElectronsAfterFold(c, PHX_EM, true);
     launchOhm(c->G, c->cfg.eta, c->cfg.nu, c->cfg.hyper_mode, c->patch.level, c->rho, cvec(c->Ve),
               c->Pe, cvec(c->em[PHX_EM] + 3), cvec(c->J), vec(c->em[PHX_EM]), c->stream, 1);
 }

@@ -955,7 +955,35 @@ __global__ __launch_bounds__(FB) void k_fold_faraday(Geom G0, MomSet M, double* 
     }
     else
         wrapNode(G, QBX + c, s);
-    Bn.c[c][G.idx(QBX + c, i, j, k)] = faradayAt(G, B, E, dt, c, s[0], s[1], s[2]);
+    double const val = faradayAt(G, B, E, dt, c, s[0], s[1], s[2]);
+    Bn.c[c][G.idx(QBX + c, i, j, k)] = val;
+    if constexpr (INPLACE)
+    {
+        // fillMagneticGhosts: every ghost node whose periodic source is this node (the nodes
+        // q = (i, j, k) + sum_d k_d n_d e_d, k_d in {-1, 0, 1}, that wrapNode maps back here)
+        // takes the new value. Faraday reads B only at its own node, so no thread reads a
+        // ghost value this launch writes.
+        constexpr int NI = DIM == 1 ? 3 : (DIM == 2 ? 9 : 27);
+        for (int m = 0; m < NI; ++m)
+        {
+            int q[3]  = {i, j, k};
+            int mm    = m;
+            bool self = true;
+            for (int d = 0; d < DIM; ++d)
+            {
+                int const kd = mm % 3 - 1;
+                mm /= 3;
+                q[d] += kd * G.n[d];
+                self = self && kd == 0;
+            }
+            if (self || q[0] < 0 || q[1] < 0 || q[2] < 0 || !inShape(G, QBX + c, q[0], q[1], q[2]))
+                continue;
+            int w[3] = {q[0], q[1], q[2]};
+            if (!wrapNode(G, QBX + c, w) || w[0] != i || w[1] != j || w[2] != k)
+                continue;
+            Bn.c[c][G.idx(QBX + c, q[0], q[1], q[2])] = val;
+        }
+    }
 }
 
 struct Five
@@ -1010,12 +1038,9 @@ __global__ __launch_bounds__(FB) void k_max5(Geom G0, Five F)
 // same primal node; the J values the closure projects are recomputed from B, each one the value
 // k_ampere_fill stores at that node: curl B at the node's periodic source. MAX: the closure takes
 // the totals from the border max of F.src over the node's periodic images (stored in F.dst), as
-// k_max5 takes it, else from Ne, Vi as k_electrons does. BFILL: B's ghost layers are stale
-// (after the corrector's in-place Faraday); curl B reads B at periodic sources, which is what
-// the filled ghost nodes hold, and the thread also writes the node's B ghost values from their
-// sources (fillMagneticGhosts).
-template<int DIM, bool MAX, bool BFILL>
-__global__ __launch_bounds__(FB) void k_ampere_electrons(Geom G0, double Te, VecPtr B, VecPtr J,
+// k_max5 takes it, else from Ne, Vi as k_electrons does.
+template<int DIM, bool MAX>
+__global__ __launch_bounds__(FB) void k_ampere_electrons(Geom G0, double Te, CVecPtr B, VecPtr J,
                                                          Five F, CVecPtr Vi, double const* Ne,
                                                          VecPtr Ve, double* Pe)
 {
@@ -1024,12 +1049,7 @@ __global__ __launch_bounds__(FB) void k_ampere_electrons(Geom G0, double Te, Vec
     int i, j, k;
     if (!nodeOf(G, QMOM, t, i, j, k))
         return;
-    auto bAt = [&](int d, int a, int b, int cc) {
-        int w[3] = {a, b, cc};
-        if (BFILL)
-            wrapNode(G, QBX + d, w);
-        return B.c[d][G.idx(QBX + d, w[0], w[1], w[2])];
-    };
+    auto bAt = [&](int d, int a, int b, int cc) { return B.c[d][G.idx(QBX + d, a, b, cc)]; };
     auto jAt = [&](int c, int a, int b, int cc) {
         int s[3] = {a, b, cc};
         wrapNode(G, QJX + c, s);
@@ -1043,18 +1063,6 @@ __global__ __launch_bounds__(FB) void k_ampere_electrons(Geom G0, double Te, Vec
     {
         onJ[c] = inShape(G, QJX + c, i, j, k);
         jv[c]  = onJ[c] ? jAt(c, i, j, k) : 0.;
-    }
-    bool gB[3] = {false, false, false};
-    double bsrc[3] = {0., 0., 0.};
-    if (BFILL)
-    {
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-        {
-            int sb[3] = {i, j, k};
-            gB[c]     = inShape(G, QBX + c, i, j, k) && wrapNode(G, QBX + c, sb);
-            bsrc[c]   = gB[c] ? B.c[c][G.idx(QBX + c, sb[0], sb[1], sb[2])] : 0.;
-        }
     }
     int const n = G.idx(QMOM, i, j, k);
     double v[5];
@@ -1081,12 +1089,8 @@ __global__ __launch_bounds__(FB) void k_ampere_electrons(Geom G0, double Te, Vec
     }
 #pragma unroll
     for (int c = 0; c < 3; ++c)
-    {
         if (onJ[c])
             J.c[c][G.idx(QJX + c, i, j, k)] = jv[c];
-        if (BFILL && gB[c])
-            B.c[c][G.idx(QBX + c, i, j, k)] = bsrc[c];
-    }
     if (MAX)
     {
 #pragma unroll
@@ -1104,24 +1108,24 @@ __global__ __launch_bounds__(FB) void k_ampere_electrons(Geom G0, double Te, Vec
 
 unsigned int gridFor(long n) { return static_cast<unsigned int>((n + FB - 1) / FB); }
 
-template<bool MAX, bool BFILL>
-void ampereElectronsT(Geom const& G, double Te, VecPtr B, VecPtr J, Five const& F, CVecPtr Vi,
+template<bool MAX>
+void ampereElectronsT(Geom const& G, double Te, CVecPtr B, VecPtr J, Five const& F, CVecPtr Vi,
                       double const* Ne, VecPtr Ve, double* Pe, hipStream_t s)
 {
     unsigned int const g = gridFor(G.size(QMOM));
     switch (G.dim)
     {
-        case 1: k_ampere_electrons<1, MAX, BFILL><<<g, FB, 0, s>>>(G, Te, B, J, F, Vi, Ne, Ve, Pe); break;
-        case 2: k_ampere_electrons<2, MAX, BFILL><<<g, FB, 0, s>>>(G, Te, B, J, F, Vi, Ne, Ve, Pe); break;
-        case 3: k_ampere_electrons<3, MAX, BFILL><<<g, FB, 0, s>>>(G, Te, B, J, F, Vi, Ne, Ve, Pe); break;
+        case 1: k_ampere_electrons<1, MAX><<<g, FB, 0, s>>>(G, Te, B, J, F, Vi, Ne, Ve, Pe); break;
+        case 2: k_ampere_electrons<2, MAX><<<g, FB, 0, s>>>(G, Te, B, J, F, Vi, Ne, Ve, Pe); break;
+        case 3: k_ampere_electrons<3, MAX><<<g, FB, 0, s>>>(G, Te, B, J, F, Vi, Ne, Ve, Pe); break;
     }
 }
 
 } // namespace
 
-void launchAmpereElectrons(Geom const& G, double Te, VecPtr B, VecPtr J, double const* const* src,
+void launchAmpereElectrons(Geom const& G, double Te, CVecPtr B, VecPtr J, double const* const* src,
                            double* const* dst, CVecPtr Vi, double const* Ne, VecPtr Ve, double* Pe,
-                           bool borderMax, bool fillB, hipStream_t s)
+                           bool borderMax, hipStream_t s)
 {
     Five F{};
     if (borderMax)
@@ -1130,12 +1134,10 @@ void launchAmpereElectrons(Geom const& G, double Te, VecPtr B, VecPtr J, double 
             F.src[q] = src[q];
             F.dst[q] = dst[q];
         }
-    if (!borderMax)
-        ampereElectronsT<false, false>(G, Te, B, J, F, Vi, Ne, Ve, Pe, s);
-    else if (!fillB)
-        ampereElectronsT<true, false>(G, Te, B, J, F, Vi, Ne, Ve, Pe, s);
+    if (borderMax)
+        ampereElectronsT<true>(G, Te, B, J, F, Vi, Ne, Ve, Pe, s);
     else
-        ampereElectronsT<true, true>(G, Te, B, J, F, Vi, Ne, Ve, Pe, s);
+        ampereElectronsT<false>(G, Te, B, J, F, Vi, Ne, Ve, Pe, s);
 }
 
 void launchFaraday(Geom const& G, CVecPtr B, CVecPtr E, VecPtr Bnew, double dt, hipStream_t s)

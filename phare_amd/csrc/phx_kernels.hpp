@@ -241,15 +241,15 @@ void launchFoldTotals(Geom const& G, MomSetArgs const& A, double* trho, double* 
                       hipStream_t s);
 void launchMax5(Geom const& G, double const* const* src, double* const* dst, hipStream_t s);
 // launchFoldTotals and the Faraday that follows it in a captured step, in one launch: B, E -> Bn
-// with Bn's ghost layers, or (inPlace: Bn is B) on B's physical nodes
+// with Bn's ghost layers, or (inPlace: Bn is B) on B's physical nodes, each node's new value also
+// written to its periodic images in the ghost layers
 void launchFoldFaraday(Geom const& G, MomSetArgs const& A, double* trho, double* tmd, VecPtr tvi,
                        CVecPtr B, CVecPtr E, VecPtr Bn, double dt, bool inPlace, hipStream_t s);
-// Ampere with J's ghost layers and the electron closure in one launch (periodic level):
-// borderMax: the closure takes the totals from the border max of src (stored in dst: rho,
-// massDensity, Vi), else from Ne, Vi; fillB (with borderMax): B's ghost layers are stale and are
-// filled by the same launch
-void launchAmpereElectrons(Geom const& G, double Te, VecPtr B, VecPtr J, double const* const* src,
+// Ampere with J's ghost layers and the electron closure in one launch (periodic level; B with
+// its ghost layers): borderMax: the closure takes the totals from the border max of src (stored
+// in dst: rho, massDensity, Vi), else from Ne, Vi
+void launchAmpereElectrons(Geom const& G, double Te, CVecPtr B, VecPtr J, double const* const* src,
                            double* const* dst, CVecPtr Vi, double const* Ne, VecPtr Ve, double* Pe,
-                           bool borderMax, bool fillB, hipStream_t s);
+                           bool borderMax, hipStream_t s);
 
 } // namespace phx
