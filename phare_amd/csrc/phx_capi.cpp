@@ -125,6 +125,8 @@ struct phx_ctx
     unsigned int* segWork = nullptr; // per-cell counts and per-segment work items of a sort
     unsigned int* sortRank = nullptr; // rank of each particle in its cell
     size_t rankCap         = 0;
+    unsigned int* sortPerm = nullptr; // source particle of each sorted slot
+    size_t permCap         = 0;
     int sortInterval      = 50;
     int stepsSinceSort    = 0;
 
@@ -875,6 +877,8 @@ phx_status sortDomain(phx_ctx* c)
         Pop& P     = c->pops[p];
         Store& dom = P.arr[PHX_DOMAIN];
         Store& buf = c->sortBuf[p];
+        if (dom.n >= 0xFFFFFFFFull)
+            return fail(c, PHX_ERR_ARG, "particle sort: more than 2^32 - 2 particles in one array");
         if (c->rankCap < dom.n)
         {
             devFree(c, c->sortRank);
@@ -894,12 +898,19 @@ phx_status sortDomain(phx_ctx* c)
         PHX_HIP(c, hipMemcpy(&slots, P.seg.off + c->sg.nsegs + 1, 8, hipMemcpyDeviceToHost));
         if (buf.cap < slots || buf.cap < dom.cap)
         {
-            // room for the empty slots of growing segments, so that later sorts reuse it: a
-            // segment has as many rows as its most populated cell has particles, which grows as
-            // a loaded plasma mixes (by a quarter at a hundred particles per cell)
+            // room for segments that gain particles between sorts (each rounds its particle
+            // count up to whole rows), so that later sorts reuse it
             size_t const want = slots + slots / 2 + 4096;
             storeFree(c, buf);
             PHX_TRY(storeAlloc(c, buf, std::max(want, dom.cap)));
+        }
+        if (c->permCap < buf.cap)
+        {
+            devFree(c, c->sortPerm);
+            c->sortPerm = nullptr;
+            c->permCap  = 0;
+            PHX_TRY(devAlloc(c, (void**)&c->sortPerm, buf.cap * 4));
+            c->permCap = buf.cap;
         }
         // work items for the larger of the two storages the sort alternates between
         unsigned int const need = segItemsCap(c->sg, std::max(buf.cap, dom.cap));
@@ -911,11 +922,11 @@ phx_status sortDomain(phx_ctx* c)
             P.itemsAlloc = need;
         }
         if (mixed(c))
-            launchSegScatter(c->sg, dom.view32(), buf.view32(), dom.n, c->segWork, c->sortRank, P.seg,
-                             c->stream);
+            launchSegScatter(c->sg, dom.view32(), buf.view32(), dom.n, slots, c->segWork, c->sortRank,
+                             c->sortPerm, P.seg, c->cfg.dim, c->stream);
         else
-            launchSegScatter(c->sg, dom.view(), buf.view(), dom.n, c->segWork, c->sortRank, P.seg,
-                             c->stream);
+            launchSegScatter(c->sg, dom.view(), buf.view(), dom.n, slots, c->segWork, c->sortRank,
+                             c->sortPerm, P.seg, c->cfg.dim, c->stream);
         PHX_TRY(checkLaunch(c, "particle sort (scatter)"));
         // a linear array (an upload, sized for it) moves into the buffer with its headroom
         if (dom.seg && slots <= dom.cap)

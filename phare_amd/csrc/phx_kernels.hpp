@@ -159,16 +159,17 @@ unsigned int segItemsCap(SegGeom const& sg, unsigned long long capacity);
 // (tables.off[nsegs + 1]); `work` holds ncells + 1 + 2 (nsegs + 1) words
 void launchSegCount(SegGeom const& sg, PView const& src, unsigned long long n, unsigned int* work,
                     unsigned int* rank, SegTables const& tables, hipStream_t s);
-// second half, once dst can hold the slot count: work items, scatter and empty-slot marking
+// second half, once dst can hold the slot count: work items, the source particle of every slot
+// (perm: slots words) with the empty ones marked, then dst filled slot by slot
 void launchSegScatter(SegGeom const& sg, PView const& src, PView const& dst, unsigned long long n,
-                      unsigned int* work, unsigned int const* rank, SegTables const& tables,
-                      hipStream_t s);
+                      unsigned long long slots, unsigned int* work, unsigned int const* rank,
+                      unsigned int* perm, SegTables const& tables, int dim, hipStream_t s);
 // mixed-precision family: the same sort over FP32 arrays
 void launchSegCount(SegGeom const& sg, PView32 const& src, unsigned long long n, unsigned int* work,
                     unsigned int* rank, SegTables const& tables, hipStream_t s);
 void launchSegScatter(SegGeom const& sg, PView32 const& src, PView32 const& dst, unsigned long long n,
-                      unsigned int* work, unsigned int const* rank, SegTables const& tables,
-                      hipStream_t s);
+                      unsigned long long slots, unsigned int* work, unsigned int const* rank,
+                      unsigned int* perm, SegTables const& tables, int dim, hipStream_t s);
 // copy of the first n slots of every array (dim position components)
 void launchSegCopy(PView const& src, PView const& dst, unsigned long long n, int dim, hipStream_t s);
 void launchSegCopy(PView32 const& src, PView32 const& dst, unsigned long long n, int dim, hipStream_t s);

@@ -1038,15 +1038,18 @@ __global__ __launch_bounds__(BLOCK) void k_seg_items(unsigned int const* __restr
     }
 }
 
-// particle i to its slot: position p of segment s (R rows) is row p % R, lane p / R, so that
-// the lanes of a row are R positions apart in the segment's cell order
+constexpr unsigned int NO_PARTICLE = 0xFFFFFFFFu;
+
+// the slot of source particle i, recorded as perm[slot] = i: position p of segment s (R rows) is
+// row p % R, lane p / R, so that the lanes of a row are R positions apart in the segment's cell
+// order; the overflow segment packs its particles in rank order
 template<class V>
-__global__ __launch_bounds__(BLOCK) void k_seg_scatter(SegGeom S, V src, V dst,
-                                                       unsigned long long n,
-                                                       unsigned int const* __restrict__ rank,
-                                                       unsigned int const* __restrict__ first,
-                                                       unsigned int const* __restrict__ rows,
-                                                       unsigned long long const* __restrict__ off)
+__global__ __launch_bounds__(BLOCK) void k_seg_perm(SegGeom S, V src, unsigned long long n,
+                                                    unsigned int const* __restrict__ rank,
+                                                    unsigned int const* __restrict__ first,
+                                                    unsigned int const* __restrict__ rows,
+                                                    unsigned long long const* __restrict__ off,
+                                                    unsigned int* __restrict__ perm)
 {
     unsigned long long const i = static_cast<unsigned long long>(blockIdx.x) * BLOCK + threadIdx.x;
     if (i >= n)
@@ -1067,26 +1070,15 @@ __global__ __launch_bounds__(BLOCK) void k_seg_scatter(SegGeom S, V src, V dst,
     }
     else
         j = off[S.nsegs] + rank[i];
-    for (int d = 0; d < S.dim; ++d)
-    {
-        dst.ic[d][j] = src.ic[d][i];
-        dst.de[d][j] = src.de[d][i];
-    }
-    dst.v[0][j] = src.v[0][i];
-    dst.v[1][j] = src.v[1][i];
-    dst.v[2][j] = src.v[2][i];
-    dst.w[j]    = src.w[i];
-    if (src.q)
-        dst.q[j] = src.q[i];
+    perm[j] = static_cast<unsigned int>(i);
 }
 
-// marks the empty slots of every segment dead (one workgroup per segment): those past the
-// segment's particle count in its position order (the overflow segment: in slot order)
-template<class V>
-__global__ __launch_bounds__(BLOCK) void k_seg_pad(SegGeom S, V dst,
-                                                   unsigned int const* __restrict__ rows,
+// the empty slots of every segment (one workgroup per segment): those past the segment's
+// particle count in its position order (the overflow segment: in slot order)
+__global__ __launch_bounds__(BLOCK) void k_seg_pad(SegGeom S, unsigned int const* __restrict__ rows,
                                                    unsigned long long const* __restrict__ off,
-                                                   unsigned int const* __restrict__ total)
+                                                   unsigned int const* __restrict__ total,
+                                                   unsigned int* __restrict__ perm)
 {
     unsigned int const s     = blockIdx.x;
     unsigned int const R     = rows[s];
@@ -1097,8 +1089,36 @@ __global__ __launch_bounds__(BLOCK) void k_seg_pad(SegGeom S, V dst,
     {
         unsigned int const p = packed ? (x % LANES) * R + x / LANES : x;
         if (p >= T)
-            dst.ic[0][off[s] + x] = DEAD_CELL;
+            perm[off[s] + x] = NO_PARTICLE;
     }
+}
+
+// slot j of the sorted storage takes source particle perm[j] (empty slots: a dead particle);
+// consecutive work-items write consecutive slots
+template<class V>
+__global__ __launch_bounds__(BLOCK) void k_seg_gather(V src, V dst, unsigned int const* __restrict__ perm,
+                                                      unsigned long long slots, int dim)
+{
+    unsigned long long const j = static_cast<unsigned long long>(blockIdx.x) * BLOCK + threadIdx.x;
+    if (j >= slots)
+        return;
+    unsigned int const i = perm[j];
+    if (i == NO_PARTICLE)
+    {
+        dst.ic[0][j] = DEAD_CELL;
+        return;
+    }
+    for (int d = 0; d < dim; ++d)
+    {
+        dst.ic[d][j] = src.ic[d][i];
+        dst.de[d][j] = src.de[d][i];
+    }
+    dst.v[0][j] = src.v[0][i];
+    dst.v[1][j] = src.v[1][i];
+    dst.v[2][j] = src.v[2][i];
+    dst.w[j]    = src.w[i];
+    if (src.q)
+        dst.q[j] = src.q[i];
 }
 
 // the first n slots of every array of src into dst (a sorted storage back into the array the
@@ -1143,8 +1163,8 @@ static void segCountT(SegGeom const& sg, V const& src, unsigned long long n, uns
 
 template<class V>
 static void segScatterT(SegGeom const& sg, V const& src, V const& dst, unsigned long long n,
-                        unsigned int* work, unsigned int const* rank, SegTables const& tables,
-                        hipStream_t s)
+                        unsigned long long slots, unsigned int* work, unsigned int const* rank,
+                        unsigned int* perm, SegTables const& tables, int dim, hipStream_t s)
 {
     unsigned int const nc = static_cast<unsigned int>(sg.ncells) + 1;
     unsigned int const m  = static_cast<unsigned int>(sg.nsegs) + 1;
@@ -1153,9 +1173,12 @@ static void segScatterT(SegGeom const& sg, V const& src, V const& dst, unsigned 
     unsigned int* total     = work + nc + m;
     k_seg_items<<<(m + BLOCK - 1) / BLOCK, BLOCK, 0, s>>>(tables.rows, itemCount, tables.items, m);
     if (n)
-        k_seg_scatter<V><<<static_cast<unsigned int>((n + BLOCK - 1) / BLOCK), BLOCK, 0, s>>>(
-            sg, src, dst, n, rank, first, tables.rows, tables.off);
-    k_seg_pad<V><<<m, BLOCK, 0, s>>>(sg, dst, tables.rows, tables.off, total);
+        k_seg_perm<V><<<static_cast<unsigned int>((n + BLOCK - 1) / BLOCK), BLOCK, 0, s>>>(
+            sg, src, n, rank, first, tables.rows, tables.off, perm);
+    k_seg_pad<<<m, BLOCK, 0, s>>>(sg, tables.rows, tables.off, total, perm);
+    if (slots)
+        k_seg_gather<V><<<static_cast<unsigned int>((slots + BLOCK - 1) / BLOCK), BLOCK, 0, s>>>(
+            src, dst, perm, slots, dim);
 }
 
 void launchSegCount(SegGeom const& sg, PView const& src, unsigned long long n, unsigned int* work,
@@ -1169,16 +1192,16 @@ void launchSegCount(SegGeom const& sg, PView32 const& src, unsigned long long n,
     segCountT(sg, src, n, work, rank, tables, s);
 }
 void launchSegScatter(SegGeom const& sg, PView const& src, PView const& dst, unsigned long long n,
-                      unsigned int* work, unsigned int const* rank, SegTables const& tables,
-                      hipStream_t s)
+                      unsigned long long slots, unsigned int* work, unsigned int const* rank,
+                      unsigned int* perm, SegTables const& tables, int dim, hipStream_t s)
 {
-    segScatterT(sg, src, dst, n, work, rank, tables, s);
+    segScatterT(sg, src, dst, n, slots, work, rank, perm, tables, dim, s);
 }
 void launchSegScatter(SegGeom const& sg, PView32 const& src, PView32 const& dst, unsigned long long n,
-                      unsigned int* work, unsigned int const* rank, SegTables const& tables,
-                      hipStream_t s)
+                      unsigned long long slots, unsigned int* work, unsigned int const* rank,
+                      unsigned int* perm, SegTables const& tables, int dim, hipStream_t s)
 {
-    segScatterT(sg, src, dst, n, work, rank, tables, s);
+    segScatterT(sg, src, dst, n, slots, work, rank, perm, tables, dim, s);
 }
 
 void launchSegCopy(PView const& src, PView const& dst, unsigned long long n, int dim, hipStream_t s)
