@@ -149,10 +149,10 @@ int chunkSize(int dim, int order);
 int blockThreads();
 unsigned int compactBlocks(unsigned long long n);
 
-// segmented storage: geometry of a domain box, LDS tile of the segmented push, work rows per item
+// segmented storage: geometry of a domain box, LDS tile of the segmented push, the most work
+// items a storage of `capacity` slots can need
 SegGeom segGeom(int dim, int const* lo, int const* hi);
 TileSpec segTileShape(int dim, int order, SegGeom const& sg, int copies);
-int segRowsPerItem();
 unsigned int segItemsCap(SegGeom const& sg, unsigned long long capacity);
 // first half of a sort of src (n slots) into cell-segmented storage: counts per cell, the rank
 // of every particle in its cell (rank: n words) and the segment tables up to the total slot count

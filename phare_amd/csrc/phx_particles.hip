@@ -1281,7 +1281,6 @@ TileSpec segTileShape(int dim, int order, SegGeom const& sg, int copies)
     return T;
 }
 
-int segRowsPerItem() { return SEG_ROWS; }
 
 unsigned int segItemsCap(SegGeom const& sg, unsigned long long capacity)
 {
