@@ -1219,10 +1219,12 @@ void launchSegCopy(PView32 const& src, PView32 const& dst, unsigned long long n,
 
 SegGeom segGeom(int dim, int const* lo, int const* hi)
 {
-    // preferred segment shapes: a run of 64 cells along the fastest dimension in 1D/2D; a
-    // 2x4x8 block in 3D, whose LDS tile with its margins stays small. Clipped to the box, the
-    // cells left over go to the slower dimensions.
-    static int const pref[3][3] = {{64, 1, 1}, {1, 64, 1}, {2, 4, 8}};
+    // preferred segment shapes: a run of 64 cells in 1D; a 2x32 block in 2D (32 cells along the
+    // fastest dimension, so that a row's gather loads stay in runs of consecutive nodes); a
+    // 2x4x8 block in 3D. Compact blocks keep the LDS tile with its margins small (a 2x32 tile is
+    // 9 x 39 nodes in 2D at order 2 against 8 x 71 for a 1x64 run: fewer nodes to flush).
+    // Clipped to the box, the cells left over go to the slower dimensions.
+    static int const pref[3][3] = {{64, 1, 1}, {1, 32, 1}, {2, 4, 8}};
     SegGeom S{};
     S.dim    = dim;
     S.nsegs  = 1;

@@ -357,10 +357,10 @@ def test_move_ions_general_path_matches_fast_path():
 
 def _segment_of(icell, ncells):
     """segment index of each particle's cell: the domain box is cut into segments of at most 64
-    cells (a run of 64 cells along the fastest dimension in 1D/2D, a 2x4x8 block in 3D, clipped
-    to the box with the cells left over given to the slower dimensions), numbered row-major"""
+    cells (a run of 64 cells in 1D, a 2x32 block in 2D, a 2x4x8 block in 3D, clipped to the box
+    with the cells left over given to the slower dimensions), numbered row-major"""
     dim = len(ncells)
-    pref = [[64], [1, 64], [2, 4, 8]][dim - 1]
+    pref = [[64], [1, 32], [2, 4, 8]][dim - 1]
     seg, room = [1] * dim, 64
     for d in range(dim - 1, -1, -1):
         want = room if d == 0 else min(pref[d], room)
