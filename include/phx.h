@@ -173,7 +173,7 @@ phx_status phx_advance(phx_ctx* ctx, double dt, int nsteps);
 /* Storage order (no counterpart in the reference, whose ParticleArray keeps insertion order and
    a CellMap): reorders every population's domain particles by cell, row-major over the AMR box,
    the order the push kernel's LDS deposit tile relies on. phx_initialize sorts once and
-   phx_advance every `steps` PPC steps (0: never; default 50). Results do not depend on the order
+   phx_advance every `steps` PPC steps (0: never; default 30). Results do not depend on the order
    beyond the summation order of the deposit. */
 phx_status phx_sort_particles(phx_ctx* ctx);
 phx_status phx_set_sort_interval(phx_ctx* ctx, int steps);

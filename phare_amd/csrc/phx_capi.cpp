@@ -127,7 +127,7 @@ struct phx_ctx
     size_t rankCap         = 0;
     unsigned int* sortPerm = nullptr; // source particle of each sorted slot
     size_t permCap         = 0;
-    int sortInterval      = 50;
+    int sortInterval      = 30;
     int stepsSinceSort    = 0;
 
     // captured PPC steps, keyed on dt, the raw deposit set and the particle storage they address
