@@ -38,7 +38,7 @@ HBM_PEAK_GBS = 8000.0
 # this script, FETCH_SIZE scaled x2 and WRITE_SIZE x1 by a 1 GiB device copy's readings; keyed by
 # workload and precision family (k_push_seg / k_push_seg32), measured for the default Harris
 # workload only
-MEASURED_TRAFFIC = {("harris2d", "fp64"): 404.2e6, ("harris2d", "fp32"): 226.8e6}
+MEASURED_TRAFFIC = {("harris2d", "fp64"): 401.1e6, ("harris2d", "fp32"): 220.9e6}
 
 
 def _hip():
