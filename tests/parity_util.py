@@ -1,7 +1,7 @@
 """Comparison helpers shared by the parity tests."""
 import numpy as np
 
-# north_star tolerance of the plain FP64 path against the oracle
+# tolerance of the plain FP64 path against the oracle: max-norm relative error of a whole array
 RTOL = 1e-12
 
 
