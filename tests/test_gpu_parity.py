@@ -441,6 +441,31 @@ def test_sorted_push_crowded_cells_and_empty_segments(precision):
     g.close()
 
 
+@pytest.mark.parametrize("dim", [1, 2, 3])
+def test_sort_puts_particles_outside_the_domain_last(dim):
+    """particles in the ghost layer (outside the domain box) go to the overflow segment after
+    every domain segment; the particle set is kept bit for bit, through a second sort as well
+    (which copies the sorted slots back into the same storage)"""
+    _say(f"[gpu] sort with particles outside the domain dim={dim}")
+    rng = np.random.default_rng(41 + dim)
+    ncells = [[300], [40, 100], [12, 10, 9]][dim - 1]
+    g = phx.Level(dim, 2, ncells, [0.3] * dim)
+    p = U.random_particles(rng, dim, 9000, ncells, [0] * dim, 1)
+    outside = np.any((p["icell"] < 0) | (p["icell"] >= np.array(ncells)), axis=1)
+    assert outside.any() and not outside.all()
+    g.set_particles(0, phx.DOMAIN, p)
+    for _ in range(2):
+        g.sort_particles()
+        q = g.get_particles()
+        U.assert_same_particles(q, p, exact=True)
+        out_q = np.any((q["icell"] < 0) | (q["icell"] >= np.array(ncells)), axis=1)
+        n_in = int((~out_q).sum())
+        assert n_in == int((~outside).sum())
+        assert not out_q[:n_in].any() and out_q[n_in:].all()
+        assert np.all(np.diff(_segment_of(q["icell"][:n_in], ncells)) >= 0)
+    g.close()
+
+
 @pytest.mark.parametrize("interval", [1, 3])
 def test_advance_with_frequent_sorts(interval):
     """sorting every `interval` PPC steps (the first sort moves the linear upload into segmented
