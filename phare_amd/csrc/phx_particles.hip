@@ -569,20 +569,21 @@ __device__ __forceinline__ void pushParticle(PushArgs const& A, PView const& in,
     pushLoaded<D, O, RN>(A, out, out.w != in.w, i, P, T, copy, acc);
 }
 
+template<int TB = BLOCK>
 __device__ __forceinline__ void zeroTile(double* lds, TileSpec const& S)
 {
-    for (int x = threadIdx.x; x < 5 * S.copies * S.stride; x += BLOCK)
+    for (int x = threadIdx.x; x < 5 * S.copies * S.stride; x += TB)
         lds[x] = 0.;
 }
 
 // flush the tile: the copies of each node summed, one HBM atomic per touched node and moment;
 // RN: the charge density is the population's charge times n
-template<int D, bool RN = false>
+template<int D, bool RN = false, int TB = BLOCK>
 __device__ __forceinline__ void flushTile(Geom const& G, TileView const& T, int nodes,
                                           double* const* mom, double charge = 0.)
 {
     int const e12 = T.pitch[1] * T.pitch[2];
-    for (int x = threadIdx.x; x < nodes; x += BLOCK)
+    for (int x = threadIdx.x; x < nodes; x += TB)
     {
         int node = -1;
 #pragma unroll
@@ -692,7 +693,7 @@ __device__ __forceinline__ void pushPiece(PushArgs const& A, PView const& in, do
     int const lane = t & 63;
     int const copy = lane & (A.tile.copies - 1);
 
-    zeroTile(lds, A.tile);
+    zeroTile<pc::SEG_BLOCK>(lds, A.tile);
     // the tile covers the segment's cells with host-chosen margins (stencil, one cell of
     // motion, drift since the sort); the overflow segment has no cell range
     TileView T;
@@ -703,18 +704,18 @@ __device__ __forceinline__ void pushPiece(PushArgs const& A, PView const& in, do
     Acc<D, O> acc;
     if constexpr (C::ACC)
         accZero(acc);
-    for (unsigned int r = rA + (t >> 6); r < rB; r += NW)
+    for (unsigned int r = rA + (t >> 6); r < rB; r += pc::SEG_NW)
         pushParticle<D, O, RN>(A, in, out, off + static_cast<unsigned long long>(r) * LANES + lane, T,
                                copy, acc);
     if constexpr (C::ACC)
         accFlush<D, O>(acc, G, T, copy, A.mom);
     __syncthreads();
-    flushTile<D, RN>(G, T, A.tile.nodes, A.mom, A.charge);
+    flushTile<D, RN, pc::SEG_BLOCK>(G, T, A.tile.nodes, A.mom, A.charge);
 }
 
 // one workgroup per work item (a segment and its share of the segment's rows, at most SEG_ROWS)
 template<int D, int O, bool RN>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(PHX_SEG_WAVES)))
+__global__ __launch_bounds__(pc::SEG_BLOCK) __attribute__((amdgpu_waves_per_eu(PHX_SEG_WAVES)))
 void k_push_seg(PushArgs A, PView in)
 {
     extern __shared__ double lds[];
@@ -1300,9 +1301,9 @@ static void launchPushT(PushArgs const& A, PView const& in, PView const& out, hi
     {
         // segmented arrays are pushed in place (out is in)
         if (A.rhoFromN)
-            k_push_seg<D, O, true><<<A.st.itemsCap, BLOCK, A.tile.bytes, s>>>(A, in);
+            k_push_seg<D, O, true><<<A.st.itemsCap, pc::SEG_BLOCK, A.tile.bytes, s>>>(A, in);
         else
-            k_push_seg<D, O, false><<<A.st.itemsCap, BLOCK, A.tile.bytes, s>>>(A, in);
+            k_push_seg<D, O, false><<<A.st.itemsCap, pc::SEG_BLOCK, A.tile.bytes, s>>>(A, in);
         return;
     }
     if (A.n == 0)

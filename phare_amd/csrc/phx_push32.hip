@@ -510,7 +510,7 @@ __device__ __forceinline__ void pushPiece32(PushArgs const& A, PView32 const& v,
     int const lane = t & 63;
 
     int const slabs = 4 * A.tile.stride;
-    for (int x = t; x < slabs; x += BLOCK)
+    for (int x = t; x < slabs; x += pc::SEG_BLOCK)
         lds32[x] = 0.;
     Tile32 T;
     T.lds    = lds32;
@@ -531,13 +531,13 @@ __device__ __forceinline__ void pushPiece32(PushArgs const& A, PView32 const& v,
     K.coef  = static_cast<float>(A.coef);
     __syncthreads();
 
-    for (unsigned int r = rA + (t >> 6); r < rB; r += NW)
+    for (unsigned int r = rA + (t >> 6); r < rB; r += pc::SEG_NW)
         pushSlot32<D, O>(A, K, v, off + static_cast<unsigned long long>(r) * LANES + lane, T);
     __syncthreads();
 
     // flush: one HBM atomic per touched node and moment
     int const e12 = T.pitch[1] * T.pitch[2];
-    for (int x = t; x < A.tile.nodes; x += BLOCK)
+    for (int x = t; x < A.tile.nodes; x += pc::SEG_BLOCK)
     {
         int node = -1;
 #pragma unroll
@@ -570,7 +570,7 @@ __device__ __forceinline__ void pushPiece32(PushArgs const& A, PView32 const& v,
 
 // one workgroup per work item (a segment and its share of the segment's rows, at most SEG_ROWS)
 template<int D, int O>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(PHX_SEG_WAVES32)))
+__global__ __launch_bounds__(pc::SEG_BLOCK) __attribute__((amdgpu_waves_per_eu(PHX_SEG_WAVES32)))
 void k_push_seg32(PushArgs A, PView32 v)
 {
     extern __shared__ double lds32[];
@@ -594,7 +594,7 @@ void k_push_seg32(PushArgs A, PView32 v)
 template<int D, int O>
 static void launchPush32T(PushArgs const& A, PView32 const& v, hipStream_t s)
 {
-    k_push_seg32<D, O><<<A.st.itemsCap, BLOCK, A.tile.bytes, s>>>(A, v);
+    k_push_seg32<D, O><<<A.st.itemsCap, pc::SEG_BLOCK, A.tile.bytes, s>>>(A, v);
 }
 
 void launchPush32(PushArgs const& A, PView32 const& v, hipStream_t s)

@@ -20,6 +20,11 @@ namespace pc
 {
 constexpr int BLOCK    = 256;
 constexpr int NW       = BLOCK / 64; // wavefronts per workgroup
+#ifndef PHX_SEG_BLOCK
+#define PHX_SEG_BLOCK 256
+#endif
+constexpr int SEG_BLOCK = PHX_SEG_BLOCK;    // work-items per workgroup of the segmented pushes
+constexpr int SEG_NW    = SEG_BLOCK / 64;
 constexpr int LANES    = 64;
 constexpr int SEG_ROWS = PHX_SEG_ROWS;
 constexpr unsigned int TILE_BYTES_MAX = 65536;
