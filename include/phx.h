@@ -140,6 +140,30 @@ phx_status phx_load_maxwellian(phx_ctx* ctx, int pop, const double* density, con
    semantics of updateAndDepositDomain_ / updateAndDepositAll_ (:169-295) and deposits
    (Interpolator::operator() interpolator.hpp:468-510). em_set selects the E/B set. */
 phx_status phx_update_populations(phx_ctx* ctx, int em_set, double dt, int mode);
+/* BorisPusher::move(rangeIn, rangeOut, em, mass, interpolator, layout, firstSelector,
+   secondSelector) (src/core/numerics/pusher/boris.hpp:93-138; Pusher interface pusher.hpp:28-35,
+   factory name "modified_boris" pusher_factory.hpp:18-29), in place (rangeIn == rangeOut, as
+   IonUpdater::updateAndDepositAll_ calls it) over array `kind` of population `pop`, with the
+   population's mass, dt and the context's mesh (setMeshAndTimeStep, boris.hpp:143-148) and E/B of
+   em_set. Each selector is a list of AMR boxes (n x 3 lower, n x 3 upper, inclusive cells): a
+   particle is selected when its cell lies in one of them; n = 0 is the noop selector. Every
+   particle is prePushed; those the first selector keeps are gathered, accelerated and
+   postPushed. The array is left partitioned: [kept by both selectors | by the first only | by
+   neither]; counts[0] = end of the second selection (the returned ParticleRange), counts[1] = end
+   of the first. A move of more than 2 cells returns PHX_ERR_MOVE_TWO_CELLS (the reference's
+   DictionaryException). */
+phx_status phx_push(phx_ctx* ctx, int pop, int kind, int em_set, double dt, int n_first,
+                    const int* first_lower, const int* first_upper, int n_second,
+                    const int* second_lower, const int* second_upper, size_t counts[2]);
+/* Interpolator::operator()(particle, em, layout) (src/core/numerics/interpolator/
+   interpolator.hpp:420-456) over a whole array: E[3 n], B[3 n] at each particle, in the order
+   phx_get_particles returns the array */
+phx_status phx_gather(phx_ctx* ctx, int em_set, int pop, int kind, double* E, double* B);
+/* Interpolator::operator()(range, n, rho, F, layout, coef) (interpolator.hpp:468-510): the
+   particles of array `kind` added into the population's density, charge density and flux with
+   weight coef (the moments are not reset; phx_update_populations resets them) */
+phx_status phx_deposit(phx_ctx* ctx, int pop, int kind, double coef);
+
 /* IonUpdater::updateIons (ion_updater.hpp:111-116) */
 phx_status phx_update_ions(phx_ctx* ctx);
 /* same-level fills of HybridHybridMessengerStrategy (hybrid_hybrid_messenger_strategy.hpp):

@@ -10,15 +10,26 @@
 #include "core/data/field/field.hpp"
 #include "core/data/grid/gridlayout.hpp"
 #include "core/data/grid/gridlayoutimplyee.hpp"
+#include "core/data/particles/particle.hpp"
 #include "core/data/particles/particle_array.hpp"
 #include "core/data/vecfield/vecfield.hpp"
 #include "core/models/options/hybrid_options.hpp"
 #include "core/numerics/ampere/ampere.hpp"
 #include "core/numerics/faraday/faraday.hpp"
 #include "core/numerics/interpolator/interpolator.hpp"
+#include "core/utilities/types.hpp"
+
+// defined in the reference's maxwellian_particle_initializer.cpp, compiled beside this harness
+// (its header includes the un-vendored cppdict, so the declaration is restated here)
+namespace PHARE::core
+{
+void maxwellianVelocity(std::array<double, 3> const& V, std::array<double, 3> const& Vth,
+                        std::mt19937_64& generator, std::array<double, 3>& partVelocity);
+}
 
 #include <array>
 #include <cstdint>
+#include <random>
 #include <string>
 #include <vector>
 
@@ -297,6 +308,54 @@ long partitionF(int const* mapLo, int const* mapHi, long np, int const* ic0, int
     return static_cast<long>(sel.iend());
 }
 
+// the Maxwellian loader's draws (MaxwellianParticleInitializer::loadParticles,
+// maxwellian_particle_initializer.hpp:137-198) from the reference's own pieces: the cell order of
+// GridLayout::indices(AMRBox) (gridlayout.hpp:219), maxwellianVelocity (compiled from
+// maxwellian_particle_initializer.cpp:12-22), ParticleDeltaDistribution<double> (particle.hpp:18-27)
+// and an mt19937_64 seeded as getRNG seeds it (maxwellian_particle_initializer.hpp:71-80). The
+// loop is restated (the class itself includes the un-vendored cppdict); profiles are per cell in
+// that order, the magnetic basis is not used (cartesian). Writes at most cap particles.
+template<std::size_t D>
+long loadMaxwellianF(int const* n, int const* lo, double const* dens, double const* const* V,
+                     double const* const* Vth, double charge, unsigned ppc, unsigned long long seed,
+                     double cutoff, long cap, int* ic, double* de, double* v, double* w, double* q)
+{
+    double dx[3] = {1., 1., 1.};
+    auto const L = T<D, 1>::layout(n, dx, lo);
+    auto const cells = L.indices(L.AMRBox());
+    std::mt19937_64 gen(static_cast<std::size_t>(seed));
+    ParticleDeltaDistribution<double> deltaDistrib;
+    long np = 0;
+    for (std::size_t flat = 0; flat < cells.size(); ++flat)
+    {
+        if (dens[flat] < cutoff)
+            continue;
+        double const cellWeight = dens[flat] / ppc;
+        auto const cell = for_N_make_array<D>([&](auto i) { return std::get<i>(cells[flat]); });
+        for (unsigned ip = 0; ip < ppc; ++ip)
+        {
+            std::array<double, 3> vel;
+            maxwellianVelocity({V[0][flat], V[1][flat], V[2][flat]},
+                               {Vth[0][flat], Vth[1][flat], Vth[2][flat]}, gen, vel);
+            auto const delta = for_N_make_array<D>([&](auto) { return deltaDistrib(gen); });
+            if (np < cap)
+            {
+                for (std::size_t d = 0; d < D; ++d)
+                {
+                    ic[np * D + d] = cell[d];
+                    de[np * D + d] = delta[d];
+                }
+                for (int c = 0; c < 3; ++c)
+                    v[np * 3 + c] = vel[c];
+                w[np] = cellWeight;
+                q[np] = charge;
+            }
+            ++np;
+        }
+    }
+    return np;
+}
+
 template<template<std::size_t, std::size_t> class F>
 struct Dispatch;
 
@@ -384,6 +443,20 @@ int ref_project(int dim, int order, int const* n, double const* dx, int const* l
                 int srcq, int dstq, double* src, double* out)
 {
     return PHX_REF_DISPATCH(dim, order, projectF, n, dx, lo, which, srcq, dstq, src, out);
+}
+
+long ref_load_maxwellian(int dim, int const* n, int const* lo, double const* dens,
+                         double const* const* V, double const* const* Vth, double charge, unsigned ppc,
+                         unsigned long long seed, double cutoff, long cap, int* ic, double* de,
+                         double* v, double* w, double* q)
+{
+    if (dim == 1)
+        return loadMaxwellianF<1>(n, lo, dens, V, Vth, charge, ppc, seed, cutoff, cap, ic, de, v, w, q);
+    if (dim == 2)
+        return loadMaxwellianF<2>(n, lo, dens, V, Vth, charge, ppc, seed, cutoff, cap, ic, de, v, w, q);
+    if (dim == 3)
+        return loadMaxwellianF<3>(n, lo, dens, V, Vth, charge, ppc, seed, cutoff, cap, ic, de, v, w, q);
+    return -1;
 }
 
 long ref_partition(int dim, int const* mapLo, int const* mapHi, long np, int const* ic0,

@@ -825,9 +825,10 @@ struct BorisPusher
 
     // boris.hpp:93-138 with rangeIn == rangeOut (in place, as the IonUpdater calls it).
     // Returns the end of the range selected by `second` (begin is b).
+    // firstEnd (optional) receives the end of the range selected by `first`.
     std::size_t move(ParticleArray<D>& arr, std::size_t b, std::size_t e, EM const& em, double mass,
                      Layout<D> const& L, Selector<D> const& first, Selector<D> const& second,
-                     PushError& err) const
+                     PushError& err, std::size_t* firstEnd = nullptr) const
     {
         // prePushStep_ (boris.hpp:180-216)
         for (std::size_t i = b; i < e; ++i)
@@ -839,6 +840,8 @@ struct BorisPusher
                 arr.change_icell(nc, i);
         }
         std::size_t e1 = first.pred ? arr.partition(b, e, first.pred) : e;
+        if (firstEnd)
+            *firstEnd = e1;
         double const dto2m = 0.5 * dt / mass;
         for (std::size_t i = b; i < e1; ++i)
         {

@@ -55,6 +55,9 @@ struct Base
     virtual int fillGhosts(int fid)                                                      = 0;
     virtual int gather(int emset, int pop, int kind, double* E, double* B)               = 0;
     virtual int pushError(double* out)                                                   = 0;
+    virtual int push(int emset, int pop, int kind, double dt, int nf, int const* flo, int const* fhi,
+                     int ns, int const* slo, int const* shi, long* out)                  = 0;
+    virtual int deposit(int pop, int kind, double coef)                                  = 0;
     virtual int diagnostics(double* out)                                                 = 0;
 };
 
@@ -268,6 +271,55 @@ struct Impl : Base
         }
         return 0;
     }
+    // BorisPusher::move (boris.hpp:93-138) in place, with selectors given as box lists (an empty
+    // list is the noop selector); out = {end of the second selection, end of the first}
+    int push(int e, int pop, int kind, double dt, int nf, int const* flo, int const* fhi, int ns,
+             int const* slo, int const* shi, long* out) override
+    {
+        auto a = arr(pop, kind);
+        if (!a)
+            return 1;
+        auto sel = [](int nb, int const* lo, int const* hi) {
+            Selector<D> S;
+            if (nb <= 0)
+                return S;
+            std::vector<Box<D>> bx(static_cast<std::size_t>(nb));
+            for (int b = 0; b < nb; ++b)
+                for (int d = 0; d < D; ++d)
+                {
+                    bx[b].lo[d] = lo[3 * b + d];
+                    bx[b].hi[d] = hi[3 * b + d];
+                }
+            S.pred = [bx](std::array<int, D> const& c) {
+                for (auto const& x : bx)
+                    if (x.contains(c))
+                        return true;
+                return false;
+            };
+            return S;
+        };
+        BorisPusher<D, O> P;
+        P.setMeshAndTimeStep(s.L.dx, dt);
+        PushError err;
+        std::size_t e1 = 0;
+        std::size_t const e2 = P.move(*a, 0, a->size(), emset(e), s.ions.pops[pop].mass, s.L,
+                                      sel(nf, flo, fhi), sel(ns, slo, shi), err, &e1);
+        s.lastError = err;
+        out[0] = static_cast<long>(e2);
+        out[1] = static_cast<long>(e1);
+        return err.set ? 2 : 0;
+    }
+    // Interpolator::operator() deposit (interpolator.hpp:468-510) of a whole array, added to the
+    // population's moments
+    int deposit(int pop, int kind, double coef) override
+    {
+        auto a = arr(pop, kind);
+        if (!a)
+            return 1;
+        auto& P = s.ions.pops[pop];
+        refcpu::deposit<D, O>(s.L, *a, 0, a->size(), P.n, P.rho, P.F, coef);
+        return 0;
+    }
     int pushError(double* out) override
     {
         out[0] = s.lastError.set ? 1. : 0.;
@@ -455,6 +507,12 @@ int ora_gather(void* h, int emset, int pop, int kind, double* E, double* B)
     return B_(h)->gather(emset, pop, kind, E, B);
 }
 int ora_push_error(void* h, double* out) { return B_(h)->pushError(out); }
+int ora_push(void* h, int emset, int pop, int kind, double dt, int nf, int const* flo, int const* fhi,
+             int ns, int const* slo, int const* shi, long* out)
+{
+    return B_(h)->push(emset, pop, kind, dt, nf, flo, fhi, ns, slo, shi, out);
+}
+int ora_deposit(void* h, int pop, int kind, double coef) { return B_(h)->deposit(pop, kind, coef); }
 int ora_diagnostics(void* h, double* out) { return B_(h)->diagnostics(out); }
 
 // B-spline weights of one direction, as the Interpolator computes them for a local cell index

@@ -97,7 +97,6 @@ struct phx_ctx
     std::vector<void*> allocs;
     double* em[3][6] = {};    // E/B of each set, one block per set (em[set][0] is its base)
     float* em32[6]   = {};    // mixed-precision family: FP32 copy of the averaged set (gather)
-    unsigned int emBytes = 0; // bytes of one set's block
     double* J[3]     = {};
     double* Ve[3]    = {};
     double* Pe = nullptr, *rho = nullptr, *massD = nullptr, *tmp = nullptr;
@@ -363,11 +362,8 @@ PushArgs pushArgs(phx_ctx* c, int pop, int kind, int mode, int emSet, double dt,
         A.em.E[i] = c->em[emSet][i];
         A.em.B[i] = c->em[emSet][3 + i];
     }
-    A.em.bytes = c->emBytes;
     for (int k = 0; k < 6; ++k)
         A.em32[k] = c->em32[k];
-    for (int k = 0; k < 6; ++k)
-        A.em.off[k] = static_cast<unsigned int>((c->em[emSet][k] - c->em[emSet][0]) * 8);
     for (int m = 0; m < 5; ++m)
         A.mom[m] = c->pops[pop].mom[m];
     for (int d = 0; d < 3; ++d)
@@ -656,6 +652,17 @@ phx_status enqueueAdvanceFast(phx_ctx* c, double dt)
     PHX_TRY(enqueueMoveIonsFast(c, dt, PHX_MODE_ALL, false, &corr));
     enqueueCorrectorFast(c);
     return checkLaunch(c, "advance");
+}
+
+// the fused periodic path in 3D (segmented push, fused field kernels) stays off: one GPU run of it
+// faulted and the cause is not shown
+phx_status fused3dGuard(phx_ctx* c, char const* who)
+{
+    if (c->cfg.dim != 3)
+        return PHX_OK;
+    return fail(c, PHX_ERR_STATE,
+                std::string(who) + ": the fused 3D periodic path is disabled (a GPU run of it "
+                                   "faulted and the cause is not shown)");
 }
 
 phx_status validPop(phx_ctx* c, int pop, int kind)
@@ -1145,8 +1152,7 @@ phx_status phx_create(phx_config const* cfg, phx_patch const* patch, size_t part
     }
     auto alloc = [&](double** p, int q) { return devAlloc(c, (void**)p, static_cast<size_t>(G.size(q)) * 8); };
     phx_status s = PHX_OK;
-    // each E/B set is one block (Ex, Ey, Ez, Bx, By, Bz, each starting on a 256-byte boundary),
-    // so that the gather addresses all six components from one base with 32-bit offsets
+    // each E/B set is one block (Ex, Ey, Ez, Bx, By, Bz, each starting on a 256-byte boundary)
     size_t emWords = 0;
     size_t emOff[6];
     for (int k = 0; k < 6; ++k)
@@ -1161,7 +1167,6 @@ phx_status phx_create(phx_config const* cfg, phx_patch const* patch, size_t part
         for (int k = 0; k < 6 && s == PHX_OK; ++k)
             c->em[set][k] = block + emOff[k];
     }
-    c->emBytes = static_cast<unsigned int>(emWords * 8);
     if (s == PHX_OK && cfg->precision == PHX_PRECISION_FP32)
     {
         float* block = nullptr;
@@ -1217,6 +1222,8 @@ phx_status phx_create(phx_config const* cfg, phx_patch const* patch, size_t part
     }
     if (s == PHX_OK)
         s = devAlloc(c, (void**)&c->err, sizeof(DevError));
+    if (s == PHX_OK)
+        G.stop = &c->err->code;
     if (s == PHX_OK)
         s = devAlloc(c, (void**)&c->dropped, 64);
     if (s == PHX_OK)
@@ -1565,6 +1572,116 @@ phx_status phx_update_populations(phx_ctx* c, int em_set, double dt, int mode)
     return checkDeviceError(c);
 }
 
+phx_status phx_push(phx_ctx* c, int pop, int kind, int em_set, double dt, int n_first,
+                    int const* first_lower, int const* first_upper, int n_second,
+                    int const* second_lower, int const* second_upper, size_t counts[2])
+{
+    PHX_TRY(validPop(c, pop, kind));
+    if (!counts || em_set < 0 || em_set > 2 || n_first < 0 || n_first > MAX_NLGB || n_second < 0
+        || n_second > MAX_NLGB || (n_first && (!first_lower || !first_upper))
+        || (n_second && (!second_lower || !second_upper)))
+        return fail(c, PHX_ERR_ARG, "phx_push: bad em set, selector box list or counts pointer");
+    if (mixed(c))
+        return fail(c, PHX_ERR_STATE, "phx_push: the mixed-precision family runs the resident level path");
+    int const D = c->cfg.dim;
+    auto boxes = [D](int nb, int const* lo, int const* hi) {
+        SelBoxes S{};
+        S.n = nb;
+        for (int b = 0; b < nb; ++b)
+            for (int d = 0; d < 3; ++d)
+            {
+                S.lo[b][d] = d < D ? lo[3 * b + d] : 0;
+                S.hi[b][d] = d < D ? hi[3 * b + d] : 0;
+            }
+        return S;
+    };
+    Store& s       = c->pops[pop].arr[kind];
+    size_t const n = s.n;
+    counts[0] = counts[1] = 0;
+    if (n == 0)
+        return PHX_OK;
+    PHX_TRY(scratchReserve(c, n));
+    PushArgs A = pushArgs(c, pop, kind, PHX_MODE_ALL, em_set, dt, false);
+    launchMove(A, s.view(), boxes(n_first, first_lower, first_upper),
+               boxes(n_second, second_lower, second_upper), c->status, c->stream);
+    PHX_TRY(checkLaunch(c, "phx_push"));
+    PHX_TRY(checkDeviceError(c));
+    // the partitions of the two selectors: [both | first only | neither]; empty slots dropped
+    Wrap noWrap{};
+    unsigned long long got[3] = {0, 0, 0}, off = 0;
+    for (uint8_t want = 0; want < 3; ++want)
+    {
+        launchCompact(s.view(), c->scratch.view(), c->status, n, want, c->blockCnt, c->tot, off, D, noWrap,
+                      c->stream);
+        PHX_HIP(c, hipStreamSynchronize(c->stream));
+        PHX_HIP(c, hipMemcpy(&got[want], c->tot, 8, hipMemcpyDeviceToHost));
+        off += got[want];
+    }
+    std::swap(s, c->scratch);
+    s.n            = off;
+    s.seg          = false;
+    c->scratch.n   = 0;
+    c->scratch.seg = false;
+    counts[0]      = got[0];
+    counts[1]      = got[0] + got[1];
+    return PHX_OK;
+}
+
+phx_status phx_gather(phx_ctx* c, int em_set, int pop, int kind, double* E, double* B)
+{
+    PHX_TRY(validPop(c, pop, kind));
+    if (!E || !B || em_set < 0 || em_set > 2)
+        return fail(c, PHX_ERR_ARG, "phx_gather: bad em set or output buffer");
+    if (mixed(c))
+        return fail(c, PHX_ERR_STATE, "phx_gather: the mixed-precision family runs the resident level path");
+    Store const& s = c->pops[pop].arr[kind];
+    size_t const n = s.n;
+    if (n == 0)
+        return PHX_OK;
+    double* dev = nullptr;
+    PHX_TRY(devAlloc(c, (void**)&dev, n * 6 * 8));
+    PushArgs A = pushArgs(c, pop, kind, PHX_MODE_ALL, em_set, 0., false);
+    launchGather(A, s.view(), dev, dev + 3 * n, c->stream);
+    phx_status st = checkLaunch(c, "phx_gather");
+    if (st == PHX_OK)
+        st = checkDeviceError(c);
+    std::vector<double> h(n * 6);
+    std::vector<int32_t> ic0(n);
+    if (st == PHX_OK && hipMemcpy(h.data(), dev, n * 6 * 8, hipMemcpyDeviceToHost) != hipSuccess)
+        st = fail(c, PHX_ERR_HIP, "phx_gather: copy of the gathered fields");
+    if (st == PHX_OK && hipMemcpy(ic0.data(), s.ic[0], n * 4, hipMemcpyDeviceToHost) != hipSuccess)
+        st = fail(c, PHX_ERR_HIP, "phx_gather: copy of the cells");
+    devFree(c, dev);
+    if (st != PHX_OK)
+        return st;
+    // live slots in storage order, as phx_get_particles returns them
+    size_t o = 0;
+    for (size_t i = 0; i < n; ++i)
+    {
+        if (ic0[i] == DEAD_CELL)
+            continue;
+        for (int k = 0; k < 3; ++k)
+        {
+            E[3 * o + k] = h[3 * i + k];
+            B[3 * o + k] = h[3 * n + 3 * i + k];
+        }
+        ++o;
+    }
+    return PHX_OK;
+}
+
+phx_status phx_deposit(phx_ctx* c, int pop, int kind, double coef)
+{
+    PHX_TRY(validPop(c, pop, kind));
+    if (mixed(c))
+        return fail(c, PHX_ERR_STATE, "phx_deposit: the mixed-precision family runs the resident level path");
+    PushArgs A = pushArgs(c, pop, kind, PHX_MODE_ALL, PHX_EM, 0., true);
+    A.coef     = coef;
+    launchDeposit(A, c->pops[pop].arr[kind].view(), c->stream);
+    PHX_TRY(checkLaunch(c, "phx_deposit"));
+    return checkDeviceError(c);
+}
+
 phx_status phx_update_ions(phx_ctx* c)
 {
     if (!c)
@@ -1686,6 +1803,7 @@ phx_status phx_move_ions(phx_ctx* c, double dt, int mode)
         return fail(c, PHX_ERR_ARG, "bad mode");
     if (fastPath(c))
     {
+        PHX_TRY(fused3dGuard(c, "phx_move_ions"));
         // the averaged set may have been set or computed outside the average kernel
         PHX_TRY(sortIfLinear(c));
         if (mixed(c))
@@ -1770,6 +1888,7 @@ phx_status phx_advance(phx_ctx* c, double dt, int nsteps)
         }
         return checkDeviceError(c);
     }
+    PHX_TRY(fused3dGuard(c, "phx_advance"));
     PHX_TRY(sortIfLinear(c));
     for (int s = 0; s < nsteps; ++s)
     {
@@ -1878,6 +1997,7 @@ phx_status phx_profile_push(phx_ctx* c, double dt, int mode, int reps, double* a
         return PHX_ERR_ARG;
     if (!fastPath(c))
         return fail(c, PHX_ERR_STATE, "phx_profile_push needs a periodic level without ghost arrays");
+    PHX_TRY(fused3dGuard(c, "phx_profile_push"));
     // the push kernel alone, captured in a graph and replayed back to back between two events
     // on the context stream (the mean over the replays is the kernel time; per-launch event
     // pairs around direct launches read high on this runtime)

@@ -64,6 +64,12 @@ struct Geom
     int shape[NQ][3];
     // local cells from which every E/B gather stencil lies inside its array, whatever delta
     int safeLo[3], safeHi[3];
+    // the context's first-error word (DevError::code): once a push has raised an error, every
+    // kernel given this geometry returns at entry, so that the state stays as the failing stage
+    // left it (the reference throws out of SolverPPC::moveIons_, solver_ppc.hpp:558-563)
+    unsigned int const* stop;
+
+    __host__ __device__ bool stopped() const { return stop && *stop != 0u; }
 
     __host__ __device__ int size(int q) const { return shape[q][0] * shape[q][1] * shape[q][2]; }
     __host__ __device__ int idx(int q, int i, int j, int k) const
@@ -267,8 +273,6 @@ struct EMPtrs
 {
     double const* E[3];
     double const* B[3];
-    unsigned int off[6]; // byte offsets of Ex..Bz from E[0] (one block per set)
-    unsigned int bytes;  // size of the block
 };
 
 // ---------------------------------------------------------------------------------------------

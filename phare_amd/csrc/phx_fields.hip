@@ -32,6 +32,8 @@ struct GeomD : Geom
 template<class GT>
 __device__ __forceinline__ bool nodeOf(GT const& G, int q, long t, int& i, int& j, int& k)
 {
+    if (G.stopped())
+        return false;
     int const s1 = G.shape[q][1], s2 = G.shape[q][2];
     long const plane = static_cast<long>(s1) * s2;
     i                = static_cast<int>(t / plane);

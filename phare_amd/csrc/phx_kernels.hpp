@@ -135,6 +135,14 @@ struct PushArgs
                    // leaves the charge density out and its flush takes charge x n
 };
 
+// box list of a particle selector (UpdaterSelectionBoxing, ion_updater.hpp:119-163): a cell is
+// selected when it lies in one of the n boxes (inclusive AMR cells)
+struct SelBoxes
+{
+    int n;
+    int lo[MAX_NLGB][3], hi[MAX_NLGB][3];
+};
+
 struct Wrap
 {
     int on;
@@ -177,6 +185,12 @@ void launchSegCopy(PView32 const& src, PView32 const& dst, unsigned long long n,
 // deposit only); the LDS tile of segTileShape32
 void launchPush32(PushArgs const& A, PView32 const& v, hipStream_t s);
 TileSpec segTileShape32(int dim, int order, SegGeom const& sg);
+// BorisPusher::move in place over A.n slots with box-list selectors; status[i] gets the slot's
+// fate for the partition (0 both selectors, 1 first only, 2 neither, 3 empty slot)
+void launchMove(PushArgs const& A, PView const& v, SelBoxes const& first, SelBoxes const& second,
+                uint8_t* status, hipStream_t s);
+// E and B interpolated at every live slot of v (A.n slots): [3 * slot + component]
+void launchGather(PushArgs const& A, PView const& v, double* E, double* B, hipStream_t s);
 void launchCompact(PView const& src, PView const& dst, uint8_t const* status, unsigned long long n,
                    uint8_t want, unsigned int* blockCounts, unsigned long long* total,
                    unsigned long long dstOffset, int dim, Wrap const& wrap, hipStream_t s);

@@ -112,93 +112,12 @@ __device__ __forceinline__ double nodeWeight(DepositData<D, O> const& dd, double
         return dep * dd.w[0][a] * dd.w[1][b] * dd.w[2][c];
 }
 
-#ifdef PHX_BUFGATHER
-// one FP64 from the E/B block at a byte offset (a buffer load: 32-bit offsets, no 64-bit address
-// arithmetic per stencil row)
-__device__ __forceinline__ double ldEM(__amdgpu_buffer_rsrc_t r, unsigned int byteOff)
-{
-    auto const v = __builtin_amdgcn_raw_buffer_load_b64(r, byteOff, 0, 0);
-    return __builtin_bit_cast(double, v);
-}
-
-// interpolation of component q (byte offset off0 in the block) in the order of interpStrided
-template<int D, int O>
-__device__ __forceinline__ double interpBuf(__amdgpu_buffer_rsrc_t r, unsigned int off0, int const* st,
-                                            double const* const* w, int s1, int s2)
-{
-    constexpr int S = O + 1;
-    if constexpr (D == 1)
-    {
-        double acc = 0.;
-#pragma unroll
-        for (int i = 0; i < S; ++i)
-            acc += ldEM(r, off0 + 8u * static_cast<unsigned int>(st[0] + i)) * w[0][i];
-        return acc;
-    }
-    else if constexpr (D == 2)
-    {
-        double acc = 0.;
-        unsigned int ro = off0 + 8u * static_cast<unsigned int>(st[0] * s1 + st[1]);
-#pragma unroll
-        for (int i = 0; i < S; ++i)
-        {
-            double yi = 0.;
-#pragma unroll
-            for (int j = 0; j < S; ++j)
-                yi += ldEM(r, ro + 8u * j) * w[1][j];
-            acc += yi * w[0][i];
-            ro += 8u * static_cast<unsigned int>(s1);
-        }
-        return acc;
-    }
-    else
-    {
-        double acc = 0.;
-#pragma unroll
-        for (int i = 0; i < S; ++i)
-        {
-            double yi = 0.;
-#pragma unroll
-            for (int j = 0; j < S; ++j)
-            {
-                unsigned int const po
-                    = off0 + 8u * static_cast<unsigned int>(((st[0] + i) * s1 + st[1] + j) * s2 + st[2]);
-                double zi = 0.;
-#pragma unroll
-                for (int k = 0; k < S; ++k)
-                    zi += ldEM(r, po + 8u * k) * w[2][k];
-                yi += zi * w[1][j];
-            }
-            acc += yi * w[0][i];
-        }
-        return acc;
-    }
-}
-#endif
 
 // E and B at the particle (Interpolator::operator() for the fields, interpolator.hpp:420-456)
 template<int D, int O>
 __device__ __forceinline__ void gatherEB(Geom const& G, EMPtrs const& em, Stencil<D, O> const& s,
                                          double* E, double* B)
 {
-#ifdef PHX_BUFGATHER
-    __amdgpu_buffer_rsrc_t const r
-        = __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(em.E[0]), 0, em.bytes, 0x00020000);
-#pragma unroll
-    for (int c = 0; c < 6; ++c)
-    {
-        int const q = c < 3 ? QEX + c : QBX + c - 3;
-        int st[D];
-        double const* w[D];
-        stencilOf<D, O>(q, s, st, w);
-        double const v = interpBuf<D, O>(r, em.off[c], st, w, G.shape[q][1], G.shape[q][2]);
-        if (c < 3)
-            E[c] = v;
-        else
-            B[c - 3] = v;
-    }
-    return;
-#endif
     E[0] = interp<D, O>(G, em.E[0], QEX, s);
     E[1] = interp<D, O>(G, em.E[1], QEY, s);
     E[2] = interp<D, O>(G, em.E[2], QEZ, s);
@@ -642,6 +561,8 @@ __global__ __launch_bounds__(BLOCK) void k_push(PushArgs A, PView in, PView out)
     constexpr int PPT   = C::PPT;
     constexpr int CHUNK = C::CHUNK;
     extern __shared__ double lds[];
+    if (A.G.stopped())
+        return;
 
     Geom const& G  = A.G;
     int const t    = threadIdx.x;
@@ -724,11 +645,14 @@ void k_push_seg(PushArgs A, PView in)
     if (item >= nItems)
         return;
     pc::ktBegin(A.kt);
-    unsigned int const seg = A.st.items[2 * item];
-    unsigned int const R   = A.st.rows[seg];
-    unsigned int r0, rEnd;
-    pc::itemRows(A.st.items[2 * item + 1], R, r0, rEnd);
-    pushPiece<D, O, RN>(A, in, lds, seg, r0, rEnd);
+    if (!A.G.stopped())
+    {
+        unsigned int const seg = A.st.items[2 * item];
+        unsigned int const R   = A.st.rows[seg];
+        unsigned int r0, rEnd;
+        pc::itemRows(A.st.items[2 * item + 1], R, r0, rEnd);
+        pushPiece<D, O, RN>(A, in, lds, seg, r0, rEnd);
+    }
     if (A.kt)
     {
         __syncthreads();
@@ -1431,6 +1355,147 @@ void launchCompact(PView const& src, PView const& dst, uint8_t const* status, un
     k_scan_counts<<<1, 1024, 0, s>>>(blockCounts, nb, total);
     k_scatter_status<<<nb, BLOCK, 0, s>>>(src, dst, status, n, want, blockCounts, dstOffset, dim,
                                           wrap);
+}
+
+// ---------------------------------------------------------------------------------------------
+// drop-in Pusher::move and Interpolator gather over a whole particle array
+// ---------------------------------------------------------------------------------------------
+namespace
+{
+__device__ __forceinline__ bool inAny(int const* c, SelBoxes const& S, int D)
+{
+    for (int b = 0; b < S.n; ++b)
+        if (inBox(c, S.lo[b], S.hi[b], D))
+            return true;
+    return false;
+}
+} // namespace
+
+// BorisPusher::move (boris.hpp:93-138) in place, one thread per slot: prePush of every particle;
+// gather, Boris and postPush of those the first selector keeps (S.n == 0: every particle); the
+// fate of each slot for the partition that follows: 0 kept by both selectors, 1 by the first
+// only, 2 by neither, 3 an empty slot
+template<int D, int O>
+__global__ __launch_bounds__(BLOCK) void k_move(PushArgs A, PView v, SelBoxes first, SelBoxes second,
+                                                uint8_t* __restrict__ status)
+{
+    unsigned long long const i = static_cast<unsigned long long>(blockIdx.x) * BLOCK + threadIdx.x;
+    if (i >= A.n)
+        return;
+    Geom const& G = A.G;
+    Part<D> P;
+    loadPart<D>(v, i, P);
+    if (P.cell[0] == DEAD_CELL)
+    {
+        status[i] = 3;
+        return;
+    }
+    int cell[D];
+    double delta[D], vel[3] = {P.v[0], P.v[1], P.v[2]};
+#pragma unroll
+    for (int d = 0; d < D; ++d)
+    {
+        cell[d]  = P.cell[d];
+        delta[d] = P.delta[d];
+    }
+    double bd, bv;
+    status[i] = 2;
+    if (!advancePosition<D>(cell, delta, vel, A.hdtdx, bd, bv))
+    {
+        raiseError(A.err, 3u, 1u, i, bd, bv, A.kind, A.pop);
+        return;
+    }
+    bool const sel1 = first.n == 0 || inAny(cell, first, D);
+    bool sel2       = false;
+    if (sel1)
+    {
+        Stencil<D, O> s;
+        makeStencil<D, O>(G, cell, delta, s);
+        if (!stencilInside<D, O>(G, s, QEX) || !stencilInside<D, O>(G, s, QBX)
+            || !stencilInside<D, O>(G, s, QEY) || !stencilInside<D, O>(G, s, QBY)
+            || !stencilInside<D, O>(G, s, QEZ) || !stencilInside<D, O>(G, s, QBZ))
+        {
+            raiseError(A.err, 4u, 2u, i, delta[0], vel[0], A.kind, A.pop);
+            return;
+        }
+        double E[3], B[3];
+        gatherEB<D, O>(G, A.em, s, E, B);
+        borisAccelerate(vel, E, B, P.q, A.dto2m);
+        if (!advancePosition<D>(cell, delta, vel, A.hdtdx, bd, bv))
+        {
+            raiseError(A.err, 3u, 2u, i, bd, bv, A.kind, A.pop);
+            return;
+        }
+        sel2 = second.n == 0 || inAny(cell, second, D);
+    }
+#pragma unroll
+    for (int d = 0; d < D; ++d)
+    {
+        v.ic[d][i] = cell[d];
+        v.de[d][i] = delta[d];
+    }
+    v.v[0][i] = vel[0];
+    v.v[1][i] = vel[1];
+    v.v[2][i] = vel[2];
+    status[i] = sel2 ? 0 : (sel1 ? 1 : 2);
+}
+
+// Interpolator::operator()(particle, em, layout) (interpolator.hpp:420-456): E and B at every
+// live slot, written as [3 * slot + component]
+template<int D, int O>
+__global__ __launch_bounds__(BLOCK) void k_gather(PushArgs A, PView v, double* __restrict__ Eo,
+                                                  double* __restrict__ Bo)
+{
+    unsigned long long const i = static_cast<unsigned long long>(blockIdx.x) * BLOCK + threadIdx.x;
+    if (i >= A.n)
+        return;
+    Part<D> P;
+    loadPart<D>(v, i, P);
+    if (P.cell[0] == DEAD_CELL)
+        return;
+    Stencil<D, O> s;
+    makeStencil<D, O>(A.G, P.cell, P.delta, s);
+    if (!stencilInside<D, O>(A.G, s, QEX) || !stencilInside<D, O>(A.G, s, QBX)
+        || !stencilInside<D, O>(A.G, s, QEY) || !stencilInside<D, O>(A.G, s, QBY)
+        || !stencilInside<D, O>(A.G, s, QEZ) || !stencilInside<D, O>(A.G, s, QBZ))
+    {
+        raiseError(A.err, 4u, 2u, i, P.delta[0], P.v[0], A.kind, A.pop);
+        return;
+    }
+    double E[3], B[3];
+    gatherEB<D, O>(A.G, A.em, s, E, B);
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+    {
+        Eo[3 * i + c] = E[c];
+        Bo[3 * i + c] = B[c];
+    }
+}
+
+template<int D, int O>
+static void launchMoveT(PushArgs const& A, PView const& v, SelBoxes const& f, SelBoxes const& s2,
+                        uint8_t* status, hipStream_t s)
+{
+    if (A.n)
+        k_move<D, O><<<compactBlocks(A.n), BLOCK, 0, s>>>(A, v, f, s2, status);
+}
+
+template<int D, int O>
+static void launchGatherT(PushArgs const& A, PView const& v, double* E, double* B, hipStream_t s)
+{
+    if (A.n)
+        k_gather<D, O><<<compactBlocks(A.n), BLOCK, 0, s>>>(A, v, E, B);
+}
+
+void launchMove(PushArgs const& A, PView const& v, SelBoxes const& first, SelBoxes const& second,
+                uint8_t* status, hipStream_t s)
+{
+    PHX_DISPATCH_DO(launchMoveT, A, v, first, second, status, s)
+}
+
+void launchGather(PushArgs const& A, PView const& v, double* E, double* B, hipStream_t s)
+{
+    PHX_DISPATCH_DO(launchGatherT, A, v, E, B, s)
 }
 
 } // namespace phx

@@ -579,11 +579,14 @@ void k_push_seg32(PushArgs A, PView32 v)
     if (item >= nItems)
         return;
     pc::ktBegin(A.kt);
-    unsigned int const seg = A.st.items[2 * item];
-    unsigned int const R   = A.st.rows[seg];
-    unsigned int r0, rEnd;
-    pc::itemRows(A.st.items[2 * item + 1], R, r0, rEnd);
-    pushPiece32<D, O>(A, v, lds32, seg, r0, rEnd);
+    if (!A.G.stopped())
+    {
+        unsigned int const seg = A.st.items[2 * item];
+        unsigned int const R   = A.st.rows[seg];
+        unsigned int r0, rEnd;
+        pc::itemRows(A.st.items[2 * item + 1], R, r0, rEnd);
+        pushPiece32<D, O>(A, v, lds32, seg, r0, rEnd);
+    }
     if (A.kt)
     {
         __syncthreads();

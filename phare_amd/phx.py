@@ -89,6 +89,10 @@ def lib():
         L.phx_profile_push.argtypes = [vp, C.c_double, C.c_int, C.c_int, C.POINTER(C.c_double)]
         L.phx_set_push_timing.argtypes = [vp, C.c_int]
         L.phx_push_timing.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int)]
+        L.phx_push.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, _ip, _ip, C.c_int,
+                               _ip, _ip, C.POINTER(C.c_size_t * 2)]
+        L.phx_gather.argtypes = [vp, C.c_int, C.c_int, C.c_int, _dp, _dp]
+        L.phx_deposit.argtypes = [vp, C.c_int, C.c_int, C.c_double]
         _lib = L
     return _lib
 
@@ -101,7 +105,7 @@ EXPORTED = ["phx_create", "phx_destroy", "phx_last_error", "phx_synchronize", "p
             "phx_fill_ghosts", "phx_faraday", "phx_ampere", "phx_electrons_update", "phx_ohm",
             "phx_average", "phx_move_ions", "phx_initialize", "phx_advance", "phx_diagnostics",
             "phx_stream", "phx_profile_push", "phx_sort_particles", "phx_set_sort_interval",
-            "phx_set_push_timing", "phx_push_timing"]
+            "phx_set_push_timing", "phx_push_timing", "phx_push", "phx_gather", "phx_deposit"]
 
 
 class Level:
@@ -262,6 +266,34 @@ class Level:
         ms, n = C.c_double(0.0), C.c_int(0)
         self._rc(lib().phx_push_timing(self.h, pop, mode, C.byref(ms), C.byref(n)))
         return ms.value, n.value
+
+    def push(self, dt, first=(), second=(), pop=0, kind=DOMAIN, emset=EM_AVG):
+        """BorisPusher::move in place (phx_push): selectors are lists of (lower, upper) AMR boxes
+        (empty: noop); returns (end of the second selection, end of the first); the array is left
+        partitioned [both | first only | neither]"""
+        def boxes(bl):
+            lo = np.zeros(3 * max(1, len(bl)), np.int32)
+            hi = np.zeros(3 * max(1, len(bl)), np.int32)
+            for b, (l, h) in enumerate(bl):
+                lo[3 * b:3 * b + len(l)] = l
+                hi[3 * b:3 * b + len(h)] = h
+            return len(bl), lo, hi
+        nf, flo, fhi = boxes(first)
+        ns, slo, shi = boxes(second)
+        out = (C.c_size_t * 2)()
+        self._rc(lib().phx_push(self.h, pop, kind, emset, dt, nf, flo, fhi, ns, slo, shi, C.byref(out)))
+        return int(out[0]), int(out[1])
+
+    def gather(self, emset=EM_AVG, pop=0, kind=DOMAIN):
+        """E and B at every particle (phx_gather), in the order of get_particles"""
+        n = self.count(pop, kind)
+        E, B = np.zeros((n, 3)), np.zeros((n, 3))
+        if n:
+            self._rc(lib().phx_gather(self.h, emset, pop, kind, E.reshape(-1), B.reshape(-1)))
+        return E, B
+
+    def deposit(self, pop=0, kind=DOMAIN, coef=1.0):
+        self._rc(lib().phx_deposit(self.h, pop, kind, coef))
 
     def diagnostics(self):
         out = np.zeros(4)

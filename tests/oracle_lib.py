@@ -81,6 +81,9 @@ def lib():
         L.ora_fill_ghosts.argtypes = [C.c_void_p, C.c_int]
         L.ora_gather.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, _dp, _dp]
         L.ora_push_error.argtypes = [C.c_void_p, _dp]
+        L.ora_push.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, _ip, _ip,
+                               C.c_int, _ip, _ip, _lp]
+        L.ora_deposit.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double]
         L.ora_diagnostics.argtypes = [C.c_void_p, _dp]
         L.ora_weights.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.POINTER(C.c_int), _dp]
         L.ora_layout_shape.argtypes = [C.c_int, C.c_int, _ip, C.c_int, _ip]
@@ -117,6 +120,10 @@ def ref():
         R.ref_ampere.argtypes = [C.c_int, C.c_int, _ip, _dp, _ip, C.c_void_p, C.c_void_p]
         R.ref_laplacian.argtypes = [C.c_int, C.c_int, _ip, _dp, _ip, C.c_int, _dp, _dp]
         R.ref_project.argtypes = [C.c_int, C.c_int, _ip, _dp, _ip, C.c_int, C.c_int, C.c_int, _dp, _dp]
+        R.ref_load_maxwellian.restype = C.c_long
+        R.ref_load_maxwellian.argtypes = [C.c_int, _ip, _ip, _dp, C.c_void_p, C.c_void_p, C.c_double,
+                                          C.c_uint, C.c_ulonglong, C.c_double, C.c_long, _ip, _dp,
+                                          _dp, _dp, _dp]
         R.ref_partition.restype = C.c_long
         R.ref_partition.argtypes = [C.c_int, _ip, _ip, C.c_long, _ip, _ip, _ip, _ip, _lp]
         _ref = R
@@ -252,7 +259,49 @@ class Sim:
         lib().ora_gather(self.h, emset, pop, kind, E.reshape(-1), B.reshape(-1))
         return E, B
 
+    def push(self, dt, first=(), second=(), pop=0, kind=0, emset=2):
+        """BorisPusher::move in place with box-list selectors (empty: noop); returns
+        (end of the second selection, end of the first)"""
+        def boxes(bl):
+            lo = np.zeros(3 * max(1, len(bl)), np.int32)
+            hi = np.zeros(3 * max(1, len(bl)), np.int32)
+            for b, (l, h) in enumerate(bl):
+                lo[3 * b:3 * b + len(l)] = l
+                hi[3 * b:3 * b + len(h)] = h
+            return len(bl), lo, hi
+        nf, flo, fhi = boxes(first)
+        ns, slo, shi = boxes(second)
+        out = np.zeros(2, np.int64)
+        self._rc(lib().ora_push(self.h, emset, pop, kind, dt, nf, flo, fhi, ns, slo, shi, out))
+        return int(out[0]), int(out[1])
+
+    def deposit(self, pop=0, kind=0, coef=1.0):
+        self._rc(lib().ora_deposit(self.h, pop, kind, coef))
+
     def diagnostics(self):
         out = np.zeros(4)
         lib().ora_diagnostics(self.h, out)
         return dict(kinetic=out[0], magnetic=out[1], charge=out[2], count=out[3])
+
+
+def ref_load_maxwellian(ncells, lower, n, V, Vth, charge, ppc, seed, cutoff=1e-16):
+    """the loader's particles from the reference's own draws (oracle/_ref), in load order"""
+    R = ref()
+    dim = len(ncells)
+    nc = np.ascontiguousarray(ncells, np.int32)
+    lo = np.ascontiguousarray(lower, np.int32)
+    dens = np.ascontiguousarray(n, np.float64).reshape(-1)
+    Va = [np.ascontiguousarray(a, np.float64).reshape(-1) for a in V]
+    Ta = [np.ascontiguousarray(a, np.float64).reshape(-1) for a in Vth]
+    Vp = (C.c_void_p * 3)(*[a.ctypes.data for a in Va])
+    Tp = (C.c_void_p * 3)(*[a.ctypes.data for a in Ta])
+    cap = int(len(dens) * ppc)
+    ic = np.zeros(max(1, cap * dim), np.int32)
+    de = np.zeros(max(1, cap * dim))
+    v = np.zeros(max(1, cap * 3))
+    w = np.zeros(max(1, cap))
+    q = np.zeros(max(1, cap))
+    m = R.ref_load_maxwellian(dim, nc, lo, dens, Vp, Tp, charge, ppc, seed, cutoff, cap, ic, de, v, w, q)
+    assert 0 <= m <= cap
+    return dict(icell=ic[:m * dim].reshape(m, dim), delta=de[:m * dim].reshape(m, dim),
+                v=v[:m * 3].reshape(m, 3), weight=w[:m], charge=q[:m])

@@ -316,14 +316,35 @@ def test_advance_mixed_charges_2d():
 
 
 @pytest.mark.skip(reason="one GPU run faulted the GPU in the 3D order-1 case of this test (first 3D "
-                         "advance on the fused periodic path); the cause is not yet shown (an "
-                         "AddressSanitizer CPU emulation of the same scenario is clean), so the 3D "
-                         "advance stays off the GPU until it is")
+                         "advance on the fused periodic path); the cause is not shown (CPU emulations "
+                         "of the library with poison-filled allocations and AddressSanitizer, of this "
+                         "tree and of the tree that faulted, are clean), so the fused 3D path is "
+                         "switched off (phx_advance / phx_move_ions return PHX_ERR_STATE in 3D)")
 @pytest.mark.parametrize("order", [1, 2, 3])
 def test_advance_uniform3d_small(order):
     _say(f"[gpu] advance uniform3d small order={order}")
     wl = W.uniform3d(ncells=(10, 9, 11), ppc=6, order=order)
     g, o = _run_pair(wl, 8)
+    _check_state(g, o, wl)
+    g.close()
+
+
+def test_fused_3d_path_is_switched_off():
+    """the fused 3D periodic path stays off the GPU: phx_advance refuses a periodic 3D level"""
+    _say("[gpu] fused 3D path refused")
+    g = phx.Level(3, 1, [9, 10, 11], [0.2] * 3)
+    with pytest.raises(phx.PhxError) as e:
+        g.advance(0.01, 1)
+    assert e.value.status == 6 and "disabled" in str(e.value)
+    g.close()
+
+
+def test_advance_harris2d_full_size_three_steps():
+    """the full 2D Harris workload (256x128 cells, 100 ppc, 3.28 M particles, order 2) over 3 PPC
+    steps against the oracle: identical cells, 1e-12 on fields, moments and particles"""
+    _say("[gpu] advance harris2d full size, 3 steps")
+    wl = W.harris2d()
+    g, o = _run_pair(wl, 3)
     _check_state(g, o, wl)
     g.close()
 
