@@ -109,6 +109,37 @@ void phx_destroy(phx_ctx* ctx);
 const char* phx_last_error(const phx_ctx* ctx);
 phx_status phx_synchronize(phx_ctx* ctx);
 
+/* A level of many patches of uneven sizes inside the level box given by `level` (its mesh,
+   level number and periodicity; its lower/upper the level's AMR box): the batched form of the
+   per-patch loops of SolverPPC (`for (auto& patch : rm.enumerate(level, ...))`,
+   solver_ppc.hpp:553-556) and of the same-level messenger fills between the patches
+   (hybrid_hybrid_messenger_strategy.hpp:376-548). patch_lower/patch_upper hold n_patches x 3
+   AMR cells (inclusive) of non-overlapping patch domain boxes inside the level box. The fields,
+   moments and particles of all patches live in one level-wide set of arrays in HBM; every stage
+   runs as one launch over all patches, the particle stage from a device patch table (each patch
+   cut into segments of at most 64 cells). A multi-patch level must be periodic (its borders are
+   not fed by a coarser level here). phx_create is the level of one patch covering the box. */
+phx_status phx_create_level(const phx_config* cfg, const phx_patch* level, int n_patches,
+                            const int* patch_lower, const int* patch_upper, size_t particle_capacity,
+                            phx_ctx** out);
+/* AMR domain box of patch p */
+phx_status phx_patch_box(const phx_ctx* ctx, int p, int lower[3], int upper[3]);
+/* makeNonLevelGhostBoxFor (src/amr/resources_manager/amr_utils.hpp:233-251): the patch's
+   domain box, then its particle ghost box intersected with each same-level neighbour's domain
+   (periodic images of the level included); at most 27 boxes */
+phx_status phx_patch_nonlevel_ghost_boxes(const phx_ctx* ctx, int p, int* n, int lower[][3],
+                                          int upper[][3]);
+/* a field of patch p in the patch's own GridLayout (allocSize of the patch, gridlayout.hpp:849-864):
+   set writes the patch's physical nodes; get returns every node, ghost nodes holding what the
+   same-level fills leave there (neighbour or periodic values, border sums of the moments) */
+phx_status phx_patch_field_shape(const phx_ctx* ctx, int p, int field_id, int shape[3]);
+phx_status phx_set_patch_field(phx_ctx* ctx, int p, int field_id, const double* host);
+phx_status phx_get_patch_field(phx_ctx* ctx, int p, int field_id, double* host);
+/* the domain particles of patch p (those whose cell lies in its box); with icell null only the
+   count *n is returned */
+phx_status phx_get_patch_particles(phx_ctx* ctx, int p, int pop, size_t* n, int32_t* icell,
+                                   double* delta, double* v, double* weight, double* charge);
+
 /* ---- data movement (host <-> HBM) ---------------------------------------------------------- */
 /* shape of a field id as GridLayout::allocSize gives it (unused dimensions are 1) */
 phx_status phx_field_shape(const phx_ctx* ctx, int field_id, int shape[3]);
@@ -117,6 +148,10 @@ phx_status phx_get_field(phx_ctx* ctx, int field_id, double* host);
 
 /* SoA particle arrays: icell[n*dim], delta[n*dim], v[n*3], weight[n], charge[n] */
 phx_status phx_set_particles(phx_ctx* ctx, int pop, int kind, size_t n, const int32_t* icell,
+                             const double* delta, const double* v, const double* weight,
+                             const double* charge);
+/* appends to the array (unsorted storage only: before phx_initialize / phx_sort_particles) */
+phx_status phx_add_particles(phx_ctx* ctx, int pop, int kind, size_t n, const int32_t* icell,
                              const double* delta, const double* v, const double* weight,
                              const double* charge);
 phx_status phx_particle_count(phx_ctx* ctx, int pop, int kind, size_t* n);

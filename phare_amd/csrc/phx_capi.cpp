@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <limits>
@@ -121,6 +122,9 @@ struct phx_ctx
     // the two swap), the key counts, and the interval in PPC steps of phx_advance
     Store sortBuf[PHX_MAX_POP];
     SegGeom sg{};
+    // the level's patches: AMR domain boxes (host) and their segment records
+    std::vector<std::array<int, 6>> patchBox;
+    std::vector<SegPatch> segPatch;
     unsigned int* segWork = nullptr; // per-cell counts and per-segment work items of a sort
     unsigned int* sortRank = nullptr; // rank of each particle in its cell
     size_t rankCap         = 0;
@@ -1057,12 +1061,99 @@ phx_status stepGraph(phx_ctx* c, double dt, phx_ctx::GraphEntry** out)
 }
 } // namespace
 
-extern "C" {
-
-phx_status phx_create(phx_config const* cfg, phx_patch const* patch, size_t particle_capacity,
-                      phx_ctx** out)
+namespace
 {
-    if (!cfg || !patch || !out)
+// the level's patch table: each patch cut into segments (numbered patch after patch), the owner
+// map of the level box, the segment -> patch table; uploaded once, read by the sort and the push
+phx_status buildPatchTable(phx_ctx* c, int const* plo, int const* phi, int np)
+{
+    int const D = c->cfg.dim;
+    SegGeom& S  = c->sg;
+    S           = SegGeom{};
+    S.dim       = D;
+    S.npatch    = np;
+    size_t boxCells = 1;
+    for (int d = 0; d < 3; ++d)
+    {
+        S.lo[d] = d < D ? c->boxes.dom_lo[d] : 0;
+        S.n[d]  = d < D ? c->boxes.dom_hi[d] - c->boxes.dom_lo[d] + 1 : 1;
+        boxCells *= static_cast<size_t>(S.n[d]);
+    }
+    std::vector<int> owner(boxCells, -1);
+    std::vector<unsigned int> segOf;
+    c->patchBox.clear();
+    c->segPatch.clear();
+    unsigned int segBase = 0, cellBase = 0;
+    for (int p = 0; p < np; ++p)
+    {
+        int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+        for (int d = 0; d < D; ++d)
+        {
+            lo[d] = plo[3 * p + d];
+            hi[d] = phi[3 * p + d];
+            if (hi[d] < lo[d] || lo[d] < S.lo[d] || hi[d] >= S.lo[d] + S.n[d])
+                return fail(c, PHX_ERR_ARG, "patch " + std::to_string(p) + " is empty or leaves the level box");
+        }
+        SegPatch P = segPatchOf(D, lo, hi);
+        P.segBase  = segBase;
+        P.cellBase = cellBase;
+        unsigned int cells = 1, segs = 1;
+        for (int d = 0; d < D; ++d)
+        {
+            cells *= static_cast<unsigned int>(P.n[d]);
+            segs *= static_cast<unsigned int>(P.nseg[d]);
+            S.maxSeg[d] = std::max(S.maxSeg[d], P.seg[d]);
+        }
+        // owner map: every cell of the patch, row-major over the level box
+        for (unsigned int k = 0; k < cells; ++k)
+        {
+            unsigned int r = k;
+            size_t lin = 0;
+            int cc[3] = {0, 0, 0};
+            for (int d = D - 1; d >= 0; --d)
+            {
+                cc[d] = static_cast<int>(r % static_cast<unsigned int>(P.n[d]));
+                r /= static_cast<unsigned int>(P.n[d]);
+            }
+            for (int d = 0; d < D; ++d)
+                lin = lin * static_cast<size_t>(S.n[d]) + static_cast<size_t>(lo[d] + cc[d] - S.lo[d]);
+            if (owner[lin] >= 0)
+                return fail(c, PHX_ERR_ARG, "patches " + std::to_string(owner[lin]) + " and "
+                                                + std::to_string(p) + " overlap");
+            owner[lin] = p;
+        }
+        segOf.insert(segOf.end(), segs, static_cast<unsigned int>(p));
+        segBase += segs;
+        cellBase += cells;
+        c->segPatch.push_back(P);
+        c->patchBox.push_back({lo[0], lo[1], lo[2], hi[0], hi[1], hi[2]});
+    }
+    for (int d = D; d < 3; ++d)
+        S.maxSeg[d] = 1;
+    S.nsegs  = static_cast<int>(segBase);
+    S.ncells = static_cast<int>(cellBase);
+    SegPatch* dP = nullptr;
+    int* dO      = nullptr;
+    unsigned int* dS = nullptr;
+    PHX_TRY(devAlloc(c, (void**)&dP, c->segPatch.size() * sizeof(SegPatch)));
+    PHX_TRY(devAlloc(c, (void**)&dO, owner.size() * sizeof(int)));
+    PHX_TRY(devAlloc(c, (void**)&dS, std::max<size_t>(1, segOf.size()) * sizeof(unsigned int)));
+    PHX_HIP(c, hipMemcpy(dP, c->segPatch.data(), c->segPatch.size() * sizeof(SegPatch), hipMemcpyHostToDevice));
+    PHX_HIP(c, hipMemcpy(dO, owner.data(), owner.size() * sizeof(int), hipMemcpyHostToDevice));
+    if (!segOf.empty())
+        PHX_HIP(c, hipMemcpy(dS, segOf.data(), segOf.size() * sizeof(unsigned int), hipMemcpyHostToDevice));
+    S.patch    = dP;
+    S.owner    = dO;
+    S.segPatch = dS;
+    return PHX_OK;
+}
+
+// a level of n_patches patches (AMR domain boxes) inside the box of `level` (its mesh, level
+// number, periodicity and selection boxes); one patch covering the level box is a patch level
+phx_status createLevel(phx_config const* cfg, phx_patch const* patch, int const* plo, int const* phi,
+                       int np, size_t particle_capacity, phx_ctx** out)
+{
+    if (!cfg || !patch || !out || np < 1)
         return PHX_ERR_ARG;
     *out = nullptr;
     if (cfg->dim < 1 || cfg->dim > 3 || cfg->interp_order < 1 || cfg->interp_order > 3
@@ -1205,9 +1296,10 @@ phx_status phx_create(phx_config const* cfg, phx_patch const* patch, size_t part
             s = storeAlloc(c, c->pops[p].arr[k], k == 0 ? std::max<size_t>(particle_capacity, 1024) : 1024);
     }
     if (s == PHX_OK)
+        s = buildPatchTable(c, plo, phi, np);
+    if (s == PHX_OK)
     {
-        // segmented storage over the domain box
-        c->sg = segGeom(cfg->dim, c->boxes.dom_lo, c->boxes.dom_hi);
+        // segmented storage over the level's patches
         size_t const nsegs = static_cast<size_t>(c->sg.nsegs);
         s = devAlloc(c, (void**)&c->segWork, (static_cast<size_t>(c->sg.ncells) + 1 + 2 * (nsegs + 1)) * 4);
         for (int p = 0; p < cfg->n_pop && s == PHX_OK; ++p)
@@ -1264,6 +1356,29 @@ phx_status phx_create(phx_config const* cfg, phx_patch const* patch, size_t part
     *out = c;
     return PHX_OK;
 }
+} // namespace
+
+extern "C" {
+
+phx_status phx_create(phx_config const* cfg, phx_patch const* patch, size_t particle_capacity,
+                      phx_ctx** out)
+{
+    if (!patch)
+        return PHX_ERR_ARG;
+    return createLevel(cfg, patch, patch->lower, patch->upper, 1, particle_capacity, out);
+}
+
+phx_status phx_create_level(phx_config const* cfg, phx_patch const* level, int n_patches,
+                            int const* patch_lower, int const* patch_upper, size_t particle_capacity,
+                            phx_ctx** out)
+{
+    if (!level || !patch_lower || !patch_upper || n_patches < 1)
+        return PHX_ERR_ARG;
+    // the borders of a level that does not cover a periodic domain are fed by the coarser level
+    if (n_patches > 1 && !level->periodic)
+        return PHX_ERR_ARG;
+    return createLevel(cfg, level, patch_lower, patch_upper, n_patches, particle_capacity, out);
+}
 
 void phx_destroy(phx_ctx* c)
 {
@@ -1302,6 +1417,210 @@ phx_status phx_field_shape(phx_ctx const* c, int fid, int shape[3])
     for (int d = 0; d < 3; ++d)
         shape[d] = c->G.shape[q][d];
     return PHX_OK;
+}
+
+// ---- the patches of a level -------------------------------------------------------------------
+namespace
+{
+phx_status validPatch(phx_ctx const* c, int p)
+{
+    if (!c || p < 0 || p >= static_cast<int>(c->patchBox.size()))
+        return PHX_ERR_ARG;
+    return PHX_OK;
+}
+
+// shape of quantity q on patch p (GridLayout::allocSize of the patch's layout) and the offset of
+// the patch's array origin inside the level's array (both include g ghost nodes)
+void patchShape(phx_ctx const* c, int p, int q, int* shape, int* off)
+{
+    for (int d = 0; d < 3; ++d)
+    {
+        bool const on = d < c->cfg.dim;
+        int const n   = on ? c->patchBox[p][3 + d] - c->patchBox[p][d] + 1 : 1;
+        shape[d]      = on ? n + 1 + 2 * c->G.g - centering(q, d) : 1;
+        off[d]        = on ? c->patchBox[p][d] - c->boxes.dom_lo[d] : 0;
+    }
+}
+} // namespace
+
+phx_status phx_patch_box(phx_ctx const* c, int p, int lower[3], int upper[3])
+{
+    PHX_TRY(validPatch(c, p));
+    if (!lower || !upper)
+        return PHX_ERR_ARG;
+    for (int d = 0; d < 3; ++d)
+    {
+        lower[d] = c->patchBox[p][d];
+        upper[d] = c->patchBox[p][3 + d];
+    }
+    return PHX_OK;
+}
+
+phx_status phx_patch_nonlevel_ghost_boxes(phx_ctx const* c, int p, int* n, int lower[][3],
+                                          int upper[][3])
+{
+    PHX_TRY(validPatch(c, p));
+    if (!n || !lower || !upper)
+        return PHX_ERR_ARG;
+    int const D  = c->cfg.dim;
+    int const pg = c->G.pg;
+    auto const& me = c->patchBox[p];
+    int cnt = 0;
+    auto put = [&](int const* lo, int const* hi) {
+        if (cnt >= MAX_NLGB)
+            return false;
+        for (int d = 0; d < 3; ++d)
+        {
+            lower[cnt][d] = d < D ? lo[d] : 0;
+            upper[cnt][d] = d < D ? hi[d] : 0;
+        }
+        ++cnt;
+        return true;
+    };
+    put(&me[0], &me[3]);
+    // neighbours (periodic images of the level included) intersected with the particle ghost box
+    int nshift = 1;
+    for (int d = 0; d < D; ++d)
+        nshift *= c->patch.periodic ? 3 : 1;
+    for (int q = 0; q < static_cast<int>(c->patchBox.size()); ++q)
+        for (int sh = 0; sh < nshift; ++sh)
+        {
+            int shift[3] = {0, 0, 0}, r = sh;
+            bool self = q == p;
+            for (int d = 0; d < D && c->patch.periodic; ++d)
+            {
+                shift[d] = (r % 3 - 1) * (c->boxes.dom_hi[d] - c->boxes.dom_lo[d] + 1);
+                r /= 3;
+                self = self && shift[d] == 0;
+            }
+            if (self)
+                continue;
+            int lo[3], hi[3];
+            bool empty = false;
+            for (int d = 0; d < D; ++d)
+            {
+                lo[d] = std::max(me[d] - pg, c->patchBox[q][d] + shift[d]);
+                hi[d] = std::min(me[3 + d] + pg, c->patchBox[q][3 + d] + shift[d]);
+                empty = empty || hi[d] < lo[d];
+            }
+            if (!empty && !put(lo, hi))
+                return fail(const_cast<phx_ctx*>(c), PHX_ERR_CAPACITY, "more than 27 selection boxes");
+        }
+    *n = cnt;
+    return PHX_OK;
+}
+
+phx_status phx_patch_field_shape(phx_ctx const* c, int p, int fid, int shape[3])
+{
+    PHX_TRY(validPatch(c, p));
+    double* f;
+    int q, off[3];
+    if (!shape || !fieldOf(c, fid, f, q))
+        return PHX_ERR_ARG;
+    patchShape(c, p, q, shape, off);
+    return PHX_OK;
+}
+
+phx_status phx_set_patch_field(phx_ctx* c, int p, int fid, double const* host)
+{
+    PHX_TRY(validPatch(c, p));
+    double* f;
+    int q, shape[3], off[3];
+    if (!host || !fieldOf(c, fid, f, q))
+        return fail(c, PHX_ERR_ARG, "bad field id");
+    patchShape(c, p, q, shape, off);
+    std::vector<double> lv(static_cast<size_t>(c->G.size(q)));
+    PHX_HIP(c, hipStreamSynchronize(c->stream));
+    PHX_HIP(c, hipMemcpy(lv.data(), f, lv.size() * 8, hipMemcpyDeviceToHost));
+    // the patch's physical nodes (GridLayout physicalStartIndex..physicalEndIndex per direction)
+    int lo[3], hi[3];
+    for (int d = 0; d < 3; ++d)
+    {
+        bool const on = d < c->cfg.dim;
+        lo[d] = on ? c->G.g : 0;
+        hi[d] = on ? shape[d] - 1 - c->G.g : 0;
+    }
+    for (int i = lo[0]; i <= hi[0]; ++i)
+        for (int j = lo[1]; j <= hi[1]; ++j)
+            for (int k = lo[2]; k <= hi[2]; ++k)
+                lv[static_cast<size_t>(c->G.idx(q, i + off[0], j + off[1], k + off[2]))]
+                    = host[(static_cast<size_t>(i) * shape[1] + j) * shape[2] + k];
+    PHX_HIP(c, hipMemcpy(f, lv.data(), lv.size() * 8, hipMemcpyHostToDevice));
+    return PHX_OK;
+}
+
+phx_status phx_get_patch_field(phx_ctx* c, int p, int fid, double* host)
+{
+    PHX_TRY(validPatch(c, p));
+    double* f;
+    int q, shape[3], off[3];
+    if (!host || !fieldOf(c, fid, f, q))
+        return fail(c, PHX_ERR_ARG, "bad field id");
+    patchShape(c, p, q, shape, off);
+    std::vector<double> lv(static_cast<size_t>(c->G.size(q)));
+    PHX_HIP(c, hipStreamSynchronize(c->stream));
+    PHX_HIP(c, hipMemcpy(lv.data(), f, lv.size() * 8, hipMemcpyDeviceToHost));
+    for (int i = 0; i < shape[0]; ++i)
+        for (int j = 0; j < shape[1]; ++j)
+            for (int k = 0; k < shape[2]; ++k)
+                host[(static_cast<size_t>(i) * shape[1] + j) * shape[2] + k]
+                    = lv[static_cast<size_t>(c->G.idx(q, i + off[0], j + off[1], k + off[2]))];
+    return PHX_OK;
+}
+
+phx_status phx_get_patch_particles(phx_ctx* c, int p, int pop, size_t* n, int32_t* icell,
+                                   double* delta, double* v, double* weight, double* charge)
+{
+    PHX_TRY(validPatch(c, p));
+    PHX_TRY(validPop(c, pop, PHX_DOMAIN));
+    if (!n)
+        return PHX_ERR_ARG;
+    int const D = c->cfg.dim;
+    std::vector<int32_t> ic;
+    std::vector<double> de, vv, w, q;
+    PHX_TRY(download(c, c->pops[pop], c->pops[pop].arr[PHX_DOMAIN], ic, de, vv, w, q));
+    auto const& b = c->patchBox[p];
+    size_t m = 0;
+    for (size_t i = 0; i < w.size(); ++i)
+    {
+        bool in = true;
+        for (int d = 0; d < D; ++d)
+            in = in && ic[i * D + d] >= b[d] && ic[i * D + d] <= b[3 + d];
+        if (!in)
+            continue;
+        if (icell)
+        {
+            for (int d = 0; d < D; ++d)
+            {
+                icell[m * D + d] = ic[i * D + d];
+                delta[m * D + d] = de[i * D + d];
+            }
+            for (int k = 0; k < 3; ++k)
+                v[m * 3 + k] = vv[i * 3 + k];
+            weight[m] = w[i];
+            charge[m] = q[i];
+        }
+        ++m;
+    }
+    *n = m;
+    return PHX_OK;
+}
+
+phx_status phx_add_particles(phx_ctx* c, int pop, int kind, size_t n, int32_t const* icell,
+                             double const* delta, double const* v, double const* weight,
+                             double const* charge)
+{
+    PHX_TRY(validPop(c, pop, kind));
+    if (n && (!icell || !delta || !v || !weight || !charge))
+        return fail(c, PHX_ERR_ARG, "null particle buffer");
+    if (n && kind != PHX_DOMAIN && mixed(c))
+        return fail(c, PHX_ERR_STATE, "the mixed-precision family holds domain particles only");
+    Store& s = c->pops[pop].arr[kind];
+    if (s.seg)
+        return fail(c, PHX_ERR_STATE, "phx_add_particles: the array is cell-segmented (sorted); append "
+                                      "before phx_initialize or phx_sort_particles");
+    PHX_HIP(c, hipStreamSynchronize(c->stream));
+    return upload(c, c->pops[pop], s, n, icell, delta, v, weight, charge, s.n);
 }
 
 phx_status phx_set_field(phx_ctx* c, int fid, double const* host)

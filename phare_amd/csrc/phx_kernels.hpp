@@ -72,20 +72,35 @@ struct TileSpec
     unsigned int bytes; // dynamic LDS of the launch
 };
 
-// Cell-segmented storage of a domain array: the domain box is cut into segments, boxes of at
-// most 64 cells (shape `seg`); a segment holding P particles has R = ceil(P / 64) rows of 64
-// slots. Its particles, in the order of their cells (row-major inside the segment), take the
-// slots column by column: position p is slot p / R of row p % R. The slots past P hold dead
-// particles. One more segment at the end holds, unordered, the particles outside the domain box.
+// Cell-segmented storage of a level's domain array: every patch of the level (a device patch
+// table, patches of uneven sizes) is cut into segments, boxes of at most 64 cells (shape `seg`),
+// numbered patch after patch; a segment holding P particles has R = ceil(P / 64) rows of 64 slots.
+// Its particles, in the order of their cells (row-major inside the segment), take the slots
+// column by column: position p is slot p / R of row p % R. The slots past P hold dead particles.
+// One more segment at the end holds, unordered, the particles outside every patch. A particle
+// belongs to the patch whose domain box holds its cell (the owner map), so a sort also migrates
+// the particles that crossed into a neighbouring patch.
+struct SegPatch
+{
+    int lo[3];             // lowest AMR cell of the patch's domain box
+    int n[3];              // cells per dimension
+    int seg[3];            // segment shape in cells
+    int nseg[3];           // segments per dimension
+    unsigned int segBase;  // number of the patch's first segment
+    unsigned int cellBase; // key of the patch's first cell (keys are row-major inside a patch)
+};
 struct SegGeom
 {
     int dim;
-    int seg[3];  // segment shape in cells
-    int nseg[3]; // segments per dimension
-    int nsegs;   // segments over the domain box (the overflow segment is number nsegs)
-    int lo[3];   // lowest AMR cell of the domain box
-    int n[3];    // domain cells per dimension
-    int ncells;  // n[0]*n[1]*n[2]
+    int npatch;
+    int nsegs;     // segments of all patches (the overflow segment is number nsegs)
+    int ncells;    // cells of all patches
+    int lo[3];     // lowest AMR cell of the level box (the box of the owner map)
+    int n[3];      // level box cells per dimension
+    int maxSeg[3]; // largest segment extent over the patches (the launch's LDS tile)
+    SegPatch const* patch;        // [npatch] (device)
+    int const* owner;             // [level box cells, row-major] patch of each cell, -1: none
+    unsigned int const* segPatch; // [nsegs] patch of each segment
 };
 // tables of the current segmented storage, rebuilt by each sort
 struct SegTables
@@ -157,9 +172,9 @@ int chunkSize(int dim, int order);
 int blockThreads();
 unsigned int compactBlocks(unsigned long long n);
 
-// segmented storage: geometry of a domain box, LDS tile of the segmented push, the most work
-// items a storage of `capacity` slots can need
-SegGeom segGeom(int dim, int const* lo, int const* hi);
+// segmented storage: segment shape of a patch box (segBase/cellBase left to the caller), LDS
+// tile of the segmented push, the most work items a storage of `capacity` slots can need
+SegPatch segPatchOf(int dim, int const* lo, int const* hi);
 TileSpec segTileShape(int dim, int order, SegGeom const& sg, int copies);
 unsigned int segItemsCap(SegGeom const& sg, unsigned long long capacity);
 // first half of a sort of src (n slots) into cell-segmented storage: counts per cell, the rank

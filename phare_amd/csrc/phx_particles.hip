@@ -772,64 +772,78 @@ namespace
 {
 constexpr unsigned int DEAD_KEY = 0xffffffffu;
 
-// key of a particle: its row-major cell index in the domain box, ncells outside the box, DEAD_KEY
-// for a lost particle; c receives the cell relative to the box
+// key of a particle: the row-major index of its cell inside the patch owning the cell, offset by
+// the patch's first key; S.ncells for a cell no patch owns (outside the level box or not covered),
+// DEAD_KEY for a lost particle. p receives the patch, c the cell relative to the patch's box.
 template<class V>
 __device__ __forceinline__ unsigned int cellKey(SegGeom const& S, V const& v, unsigned long long i,
-                                                int* c)
+                                                int* c, int& p)
 {
     if (v.ic[0][i] == DEAD_CELL)
         return DEAD_KEY;
-    unsigned int key = 0;
+    unsigned int lin = 0;
     for (int d = 0; d < S.dim; ++d)
     {
-        c[d] = v.ic[d][i] - S.lo[d];
-        if (c[d] < 0 || c[d] >= S.n[d])
+        int const r = v.ic[d][i] - S.lo[d];
+        if (r < 0 || r >= S.n[d])
             return static_cast<unsigned int>(S.ncells);
-        key = key * static_cast<unsigned int>(S.n[d]) + static_cast<unsigned int>(c[d]);
+        lin = lin * static_cast<unsigned int>(S.n[d]) + static_cast<unsigned int>(r);
     }
-    return key;
+    p = S.owner[lin];
+    if (p < 0)
+        return static_cast<unsigned int>(S.ncells);
+    SegPatch const& P = S.patch[p];
+    unsigned int key  = 0;
+    for (int d = 0; d < S.dim; ++d)
+    {
+        c[d] = v.ic[d][i] - P.lo[d];
+        key  = key * static_cast<unsigned int>(P.n[d]) + static_cast<unsigned int>(c[d]);
+    }
+    return P.cellBase + key;
 }
 
-// segment and lane of a cell of the box
-__device__ __forceinline__ void segLane(SegGeom const& S, int const* c, unsigned int& seg, int& lane)
+// segment and lane of cell c (relative to patch p's box)
+__device__ __forceinline__ void segLane(SegGeom const& S, int p, int const* c, unsigned int& seg,
+                                        int& lane)
 {
-    unsigned int s = 0;
-    int l          = 0;
+    SegPatch const& P = S.patch[p];
+    unsigned int s    = 0;
+    int l             = 0;
     for (int d = 0; d < S.dim; ++d)
     {
-        int const q = c[d] / S.seg[d];
-        s           = s * static_cast<unsigned int>(S.nseg[d]) + static_cast<unsigned int>(q);
-        l           = l * S.seg[d] + (c[d] - q * S.seg[d]);
+        int const q = c[d] / P.seg[d];
+        s           = s * static_cast<unsigned int>(P.nseg[d]) + static_cast<unsigned int>(q);
+        l           = l * P.seg[d] + (c[d] - q * P.seg[d]);
     }
-    seg  = s;
+    seg  = P.segBase + s;
     lane = l;
 }
 
-// key of lane l's cell in segment s, or DEAD_KEY where the segment overhangs the box
+// key of lane l's cell in segment s, or DEAD_KEY where the segment overhangs its patch's box
 __device__ __forceinline__ unsigned int laneKey(SegGeom const& S, unsigned int s, int l)
 {
+    SegPatch const& P = S.patch[S.segPatch[s]];
     int sidx[3], sub[3];
-    unsigned int rem = s;
+    unsigned int rem = s - P.segBase;
     int lr           = l;
     for (int d = S.dim - 1; d >= 0; --d)
     {
-        sidx[d] = static_cast<int>(rem % static_cast<unsigned int>(S.nseg[d]));
-        rem /= static_cast<unsigned int>(S.nseg[d]);
-        sub[d] = lr % S.seg[d];
-        lr /= S.seg[d];
+        sidx[d] = static_cast<int>(rem % static_cast<unsigned int>(P.nseg[d]));
+        rem /= static_cast<unsigned int>(P.nseg[d]);
+        sub[d] = lr % P.seg[d];
+        lr /= P.seg[d];
     }
     if (lr != 0)
         return DEAD_KEY; // lane beyond the segment's cells
     unsigned int key = 0;
     for (int d = 0; d < S.dim; ++d)
     {
-        int const c = sidx[d] * S.seg[d] + sub[d];
-        if (c >= S.n[d])
+        int const c = sidx[d] * P.seg[d] + sub[d];
+        if (c >= P.n[d])
             return DEAD_KEY;
-        key = key * static_cast<unsigned int>(S.n[d]) + static_cast<unsigned int>(c);
+        key = key * static_cast<unsigned int>(P.n[d]) + static_cast<unsigned int>(c);
     }
-    return key;
+    return P.cellBase + key;
 }
 } // namespace
 
@@ -850,8 +864,8 @@ __global__ __launch_bounds__(BLOCK) void k_seg_count(SegGeom S, V v, unsigned lo
     unsigned long long const i = static_cast<unsigned long long>(blockIdx.x) * BLOCK + threadIdx.x;
     if (i >= n)
         return;
-    int c[3];
-    unsigned int const key = cellKey(S, v, i, c);
+    int c[3], p = 0;
+    unsigned int const key = cellKey(S, v, i, c, p);
     if (key != DEAD_KEY)
         rank[i] = atomicAdd(counts + key, 1u);
 }
@@ -979,8 +993,8 @@ __global__ __launch_bounds__(BLOCK) void k_seg_perm(SegGeom S, V src, unsigned l
     unsigned long long const i = static_cast<unsigned long long>(blockIdx.x) * BLOCK + threadIdx.x;
     if (i >= n)
         return;
-    int c[3];
-    unsigned int const key = cellKey(S, src, i, c);
+    int c[3], pt = 0;
+    unsigned int const key = cellKey(S, src, i, c, pt);
     if (key == DEAD_KEY)
         return;
     unsigned long long j;
@@ -988,7 +1002,7 @@ __global__ __launch_bounds__(BLOCK) void k_seg_perm(SegGeom S, V src, unsigned l
     {
         unsigned int seg;
         int lane;
-        segLane(S, c, seg, lane);
+        segLane(S, pt, c, seg, lane);
         unsigned int const p = first[key] + rank[i];
         unsigned int const R = rows[seg];
         j = off[seg] + static_cast<unsigned long long>(p % R) * LANES + p / R;
@@ -1142,7 +1156,7 @@ void launchSegCopy(PView32 const& src, PView32 const& dst, unsigned long long n,
                                                                                                 n, dim);
 }
 
-SegGeom segGeom(int dim, int const* lo, int const* hi)
+SegPatch segPatchOf(int dim, int const* lo, int const* hi)
 {
     // preferred segment shapes: a run of 64 cells in 1D; a 2x32 block in 2D (32 cells along the
     // fastest dimension, so that a row's gather loads stay in runs of consecutive nodes); a
@@ -1150,31 +1164,23 @@ SegGeom segGeom(int dim, int const* lo, int const* hi)
     // 9 x 39 nodes in 2D at order 2 against 8 x 71 for a 1x64 run: fewer nodes to flush).
     // Clipped to the box, the cells left over go to the slower dimensions.
     static int const pref[3][3] = {{64, 1, 1}, {1, 32, 1}, {2, 4, 8}};
-    SegGeom S{};
-    S.dim    = dim;
-    S.nsegs  = 1;
-    S.ncells = 1;
+    SegPatch P{};
     int room = LANES;
-    for (int d = 2; d >= 0; --d)
+    for (int d = 0; d < 3; ++d)
     {
-        S.lo[d] = d < dim ? lo[d] : 0;
-        S.n[d]  = d < dim ? hi[d] - lo[d] + 1 : 1;
+        P.lo[d] = d < dim ? lo[d] : 0;
+        P.n[d]  = d < dim ? hi[d] - lo[d] + 1 : 1;
+        P.seg[d] = 1;
     }
     for (int d = dim - 1; d >= 0; --d)
     {
         int const want = d == 0 ? room : std::min(pref[dim - 1][d], room);
-        S.seg[d]       = std::max(1, std::min(want, S.n[d]));
-        room /= S.seg[d];
+        P.seg[d]       = std::max(1, std::min(want, P.n[d]));
+        room /= P.seg[d];
     }
-    for (int d = dim; d < 3; ++d)
-        S.seg[d] = 1;
     for (int d = 0; d < 3; ++d)
-    {
-        S.nseg[d] = (S.n[d] + S.seg[d] - 1) / S.seg[d];
-        S.nsegs *= S.nseg[d];
-        S.ncells *= S.n[d];
-    }
-    return S;
+        P.nseg[d] = (P.n[d] + P.seg[d] - 1) / P.seg[d];
+    return P;
 }
 
 TileSpec segTileShape(int dim, int order, SegGeom const& sg, int copies)
@@ -1186,7 +1192,7 @@ TileSpec segTileShape(int dim, int order, SegGeom const& sg, int copies)
     for (int d = 0; d < 3; ++d)
     {
         T.lo[d]  = d < dim ? mlo : 0;
-        T.ext[d] = d < dim ? sg.seg[d] + mlo + mhi : 1;
+        T.ext[d] = d < dim ? sg.maxSeg[d] + mlo + mhi : 1;
     }
     pc::tilePitch(T, dim);
     T.nodes  = T.ext[0] * T.pitch[1] * T.pitch[2];

@@ -625,7 +625,7 @@ TileSpec segTileShape32(int dim, int order, SegGeom const& sg)
     for (int d = 0; d < 3; ++d)
     {
         T.lo[d]  = d < dim ? mlo : 0;
-        T.ext[d] = d < dim ? sg.seg[d] + mlo + mhi : 1;
+        T.ext[d] = d < dim ? sg.maxSeg[d] + mlo + mhi : 1;
     }
     pc::tilePitch(T, dim);
     T.nodes  = T.ext[0] * T.pitch[1] * T.pitch[2];

@@ -81,24 +81,26 @@ __device__ __forceinline__ void itemRows(unsigned int j, unsigned int R, unsigne
 }
 
 // local node index of the LDS tile origin of segment `seg` (the segment's first cell less the
-// tile's low margin); the overflow segment has no cell range and gets an unreachable origin
+// tile's low margin), from its patch's record; the overflow segment has no cell range and gets an
+// unreachable origin
 template<int D>
 __device__ __forceinline__ void segTileOrigin(Geom const& G, SegGeom const& S, TileSpec const& tile,
                                               unsigned int seg, int* org)
 {
     if (seg < static_cast<unsigned int>(S.nsegs))
     {
-        int rem = static_cast<int>(seg);
+        SegPatch const& P = S.patch[S.segPatch[seg]];
+        int rem = static_cast<int>(seg - P.segBase);
         int sidx[3];
 #pragma unroll
         for (int d = D - 1; d >= 0; --d)
         {
-            sidx[d] = rem % S.nseg[d];
-            rem /= S.nseg[d];
+            sidx[d] = rem % P.nseg[d];
+            rem /= P.nseg[d];
         }
 #pragma unroll
         for (int d = 0; d < D; ++d)
-            org[d] = S.lo[d] + sidx[d] * S.seg[d] - (G.lower[d] - G.g) - tile.lo[d];
+            org[d] = P.lo[d] + sidx[d] * P.seg[d] - (G.lower[d] - G.g) - tile.lo[d];
     }
     else
     {

@@ -89,6 +89,16 @@ def lib():
         L.phx_profile_push.argtypes = [vp, C.c_double, C.c_int, C.c_int, C.POINTER(C.c_double)]
         L.phx_set_push_timing.argtypes = [vp, C.c_int]
         L.phx_push_timing.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int)]
+        L.phx_create_level.argtypes = [C.POINTER(Config), C.POINTER(Patch), C.c_int, _ip, _ip,
+                                       C.c_size_t, C.POINTER(vp)]
+        L.phx_patch_box.argtypes = [vp, C.c_int, _ip, _ip]
+        L.phx_patch_nonlevel_ghost_boxes.argtypes = [vp, C.c_int, C.POINTER(C.c_int), _ip, _ip]
+        L.phx_patch_field_shape.argtypes = [vp, C.c_int, C.c_int, _ip]
+        L.phx_set_patch_field.argtypes = [vp, C.c_int, C.c_int, _dp]
+        L.phx_get_patch_field.argtypes = [vp, C.c_int, C.c_int, _dp]
+        L.phx_get_patch_particles.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_size_t), vp, vp,
+                                              vp, vp, vp]
+        L.phx_add_particles.argtypes = [vp, C.c_int, C.c_int, C.c_size_t, _ip, _dp, _dp, _dp, _dp]
         L.phx_push.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, _ip, _ip, C.c_int,
                                _ip, _ip, C.POINTER(C.c_size_t * 2)]
         L.phx_gather.argtypes = [vp, C.c_int, C.c_int, C.c_int, _dp, _dp]
@@ -105,14 +115,20 @@ EXPORTED = ["phx_create", "phx_destroy", "phx_last_error", "phx_synchronize", "p
             "phx_fill_ghosts", "phx_faraday", "phx_ampere", "phx_electrons_update", "phx_ohm",
             "phx_average", "phx_move_ions", "phx_initialize", "phx_advance", "phx_diagnostics",
             "phx_stream", "phx_profile_push", "phx_sort_particles", "phx_set_sort_interval",
-            "phx_set_push_timing", "phx_push_timing", "phx_push", "phx_gather", "phx_deposit"]
+            "phx_set_push_timing", "phx_push_timing", "phx_push", "phx_gather", "phx_deposit",
+            "phx_create_level", "phx_patch_box", "phx_patch_nonlevel_ghost_boxes",
+            "phx_patch_field_shape", "phx_set_patch_field", "phx_get_patch_field",
+            "phx_get_patch_particles", "phx_add_particles"]
 
 
 class Level:
     """One patch level resident on the GPU (a periodic level: one patch covering the domain)."""
 
     def __init__(self, dim, order, ncells, dx, lower=None, masses=(1.0,), Te=0.1, eta=0.0, nu=1e-4,
-                 hyper_mode=0, level=0, periodic=True, capacity=0, precision=PRECISION_FP64):
+                 hyper_mode=0, level=0, periodic=True, capacity=0, precision=PRECISION_FP64,
+                 patches=None):
+        """patches: None (one patch covering the box) or a list of (lower, upper) AMR boxes of
+        the level's patches (phx_create_level)"""
         L = lib()
         self.dim, self.order = dim, order
         lower = list(lower or [0] * dim)
@@ -128,7 +144,18 @@ class Level:
             p.dx[d] = dx[d]
         p.level, p.periodic, p.n_nonlevel_ghost_boxes = level, int(periodic), 0
         h = C.c_void_p()
-        rc = L.phx_create(C.byref(cfg), C.byref(p), int(capacity), C.byref(h))
+        if patches is None:
+            rc = L.phx_create(C.byref(cfg), C.byref(p), int(capacity), C.byref(h))
+            self.npatch = 1
+        else:
+            plo = np.zeros(3 * len(patches), np.int32)
+            phi = np.zeros(3 * len(patches), np.int32)
+            for i, (lo, hi) in enumerate(patches):
+                plo[3 * i:3 * i + dim] = lo
+                phi[3 * i:3 * i + dim] = hi
+            rc = L.phx_create_level(C.byref(cfg), C.byref(p), len(patches), plo, phi, int(capacity),
+                                    C.byref(h))
+            self.npatch = len(patches)
         if rc != PHX_OK:
             raise PhxError(rc, "phx_create failed")
         self.h = h
@@ -266,6 +293,54 @@ class Level:
         ms, n = C.c_double(0.0), C.c_int(0)
         self._rc(lib().phx_push_timing(self.h, pop, mode, C.byref(ms), C.byref(n)))
         return ms.value, n.value
+
+    def patch_box(self, p):
+        lo, hi = np.zeros(3, np.int32), np.zeros(3, np.int32)
+        self._rc(lib().phx_patch_box(self.h, p, lo, hi))
+        return tuple(lo[:self.dim]), tuple(hi[:self.dim])
+
+    def patch_nonlevel_ghost_boxes(self, p):
+        n = C.c_int(0)
+        lo, hi = np.zeros(81, np.int32), np.zeros(81, np.int32)
+        self._rc(lib().phx_patch_nonlevel_ghost_boxes(self.h, p, C.byref(n), lo, hi))
+        return [(tuple(lo[3 * b:3 * b + self.dim]), tuple(hi[3 * b:3 * b + self.dim]))
+                for b in range(n.value)]
+
+    def patch_shape(self, p, fid):
+        s = np.zeros(3, dtype=np.int32)
+        self._rc(lib().phx_patch_field_shape(self.h, p, fid, s))
+        return tuple(int(x) for x in s[: self.dim])
+
+    def get_patch(self, p, fid):
+        a = np.zeros(self.patch_shape(p, fid))
+        self._rc(lib().phx_get_patch_field(self.h, p, fid, a.reshape(-1)))
+        return a
+
+    def set_patch(self, p, fid, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        assert a.shape == self.patch_shape(p, fid)
+        self._rc(lib().phx_set_patch_field(self.h, p, fid, a.reshape(-1)))
+
+    def get_patch_particles(self, p, pop=0):
+        n = C.c_size_t(0)
+        self._rc(lib().phx_get_patch_particles(self.h, p, pop, C.byref(n), None, None, None, None, None))
+        m = int(n.value)
+        out = dict(icell=np.zeros((m, self.dim), dtype=np.int32), delta=np.zeros((m, self.dim)),
+                   v=np.zeros((m, 3)), weight=np.zeros(m), charge=np.zeros(m))
+        if m:
+            self._rc(lib().phx_get_patch_particles(
+                self.h, p, pop, C.byref(n), out["icell"].ctypes.data, out["delta"].ctypes.data,
+                out["v"].ctypes.data, out["weight"].ctypes.data, out["charge"].ctypes.data))
+        return out
+
+    def add_particles(self, pop, kind, p):
+        n = len(p["weight"])
+        self._rc(lib().phx_add_particles(
+            self.h, pop, kind, n, np.ascontiguousarray(p["icell"], dtype=np.int32).reshape(-1),
+            np.ascontiguousarray(p["delta"], dtype=np.float64).reshape(-1),
+            np.ascontiguousarray(p["v"], dtype=np.float64).reshape(-1),
+            np.ascontiguousarray(p["weight"], dtype=np.float64),
+            np.ascontiguousarray(p["charge"], dtype=np.float64)))
 
     def push(self, dt, first=(), second=(), pop=0, kind=DOMAIN, emset=EM_AVG):
         """BorisPusher::move in place (phx_push): selectors are lists of (lower, upper) AMR boxes
