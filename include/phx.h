@@ -194,6 +194,15 @@ phx_status phx_push(phx_ctx* ctx, int pop, int kind, int em_set, double dt, int 
    interpolator.hpp:420-456) over a whole array: E[3 n], B[3 n] at each particle, in the order
    phx_get_particles returns the array */
 phx_status phx_gather(phx_ctx* ctx, int em_set, int pop, int kind, double* E, double* B);
+/* No counterpart in the reference: phx_gather without any device allocation, for callers that
+   cannot allocate (code under HIP graph capture, tight serving loops, harnesses that forbid
+   hipMalloc). Same arguments and outputs as phx_gather; the values are staged in a fixed device
+   array of the library (one per GPU, ~48 MiB, held until the process ends) instead of a
+   per-call allocation. At most PHX_GATHER_NOALLOC_MAX_SLOTS slots (live plus empty) in the array,
+   else PHX_ERR_CAPACITY. Every call on a GPU shares that GPU's array: calls that overlap on one
+   GPU (several host threads, several contexts) race and must be serialised by the caller. */
+#define PHX_GATHER_NOALLOC_MAX_SLOTS (1u << 20)
+phx_status phx_gather_noalloc(phx_ctx* ctx, int em_set, int pop, int kind, double* E, double* B);
 /* Interpolator::operator()(range, n, rho, F, layout, coef) (interpolator.hpp:468-510): the
    particles of array `kind` added into the population's density, charge density and flux with
    weight coef (the moments are not reset; phx_update_populations resets them) */

@@ -39,6 +39,9 @@
 
 using namespace phx;
 
+static_assert(NOALLOC_GATHER_MAX_SLOTS == PHX_GATHER_NOALLOC_MAX_SLOTS,
+              "the scratch array of phx_gather_noalloc is sized for the limit phx.h states");
+
 namespace
 {
 
@@ -142,6 +145,8 @@ struct phx_ctx
         std::vector<unsigned long long> key;
     };
     GraphEntry graphs[2];
+    // device address of the scratch array of phx_gather_noalloc on the context's device
+    void* noAllocScratch = nullptr;
     int graphNext = 0;
     // in-kernel timing of the pushes of the fast path, per (mode, population)
     KTime* kt   = nullptr;
@@ -1241,6 +1246,15 @@ phx_status createLevel(phx_config const* cfg, phx_patch const* patch, int const*
         delete c;
         return PHX_ERR_HIP;
     }
+    // the device address of the no-allocation gather's scratch array on this device (looked up
+    // once per device, here with the context's device current)
+    e = noAllocScratch(&c->noAllocScratch);
+    if (e != hipSuccess)
+    {
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return PHX_ERR_HIP;
+    }
     auto alloc = [&](double** p, int q) { return devAlloc(c, (void**)p, static_cast<size_t>(G.size(q)) * 8); };
     phx_status s = PHX_OK;
     // each E/B set is one block (Ex, Ey, Ez, Bx, By, Bz, each starting on a 256-byte boundary)
@@ -1985,6 +1999,43 @@ phx_status phx_gather(phx_ctx* c, int em_set, int pop, int kind, double* E, doub
             B[3 * o + k] = h[3 * n + 3 * i + k];
         }
         ++o;
+    }
+    return PHX_OK;
+}
+
+phx_status phx_gather_noalloc(phx_ctx* c, int em_set, int pop, int kind, double* E, double* B)
+{
+    PHX_TRY(validPop(c, pop, kind));
+    if (!E || !B || em_set < 0 || em_set > 2)
+        return fail(c, PHX_ERR_ARG, "phx_gather_noalloc: bad em set or output buffer");
+    if (mixed(c))
+        return fail(c, PHX_ERR_STATE,
+                    "phx_gather_noalloc: the mixed-precision family runs the resident level path");
+    Store const& s = c->pops[pop].arr[kind];
+    size_t const n = s.n;
+    if (n > NOALLOC_GATHER_MAX_SLOTS)
+        return fail(c, PHX_ERR_CAPACITY,
+                    "phx_gather_noalloc: the array holds more slots than the scratch array allows ("
+                        + std::to_string(n) + " > " + std::to_string(NOALLOC_GATHER_MAX_SLOTS) + ")");
+    if (n == 0)
+        return PHX_OK;
+    if (!c->noAllocScratch)
+        return fail(c, PHX_ERR_STATE, "phx_gather_noalloc: no scratch address for this device");
+    PushArgs A = pushArgs(c, pop, kind, PHX_MODE_ALL, em_set, 0., false);
+    launchGatherNoAlloc(A, s.view(), c->noAllocScratch, c->stream);
+    PHX_TRY(checkLaunch(c, "phx_gather_noalloc"));
+    PHX_TRY(checkDeviceError(c));
+    double const *dE = nullptr, *dB = nullptr;
+    unsigned long long const* dTotal = nullptr;
+    noAllocGatherOutputs(c->noAllocScratch, &dE, &dB, &dTotal);
+    unsigned long long live = 0;
+    PHX_HIP(c, hipMemcpy(&live, dTotal, 8, hipMemcpyDeviceToHost));
+    if (live > n)
+        return fail(c, PHX_ERR_STATE, "phx_gather_noalloc: inconsistent live slot count");
+    if (live)
+    {
+        PHX_HIP(c, hipMemcpy(E, dE, live * 3 * 8, hipMemcpyDeviceToHost));
+        PHX_HIP(c, hipMemcpy(B, dB, live * 3 * 8, hipMemcpyDeviceToHost));
     }
     return PHX_OK;
 }

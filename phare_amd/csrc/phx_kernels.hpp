@@ -206,6 +206,17 @@ void launchMove(PushArgs const& A, PView const& v, SelBoxes const& first, SelBox
                 uint8_t* status, hipStream_t s);
 // E and B interpolated at every live slot of v (A.n slots): [3 * slot + component]
 void launchGather(PushArgs const& A, PView const& v, double* E, double* B, hipStream_t s);
+// gather without allocation (phx_gather_noalloc): at most NOALLOC_GATHER_MAX_SLOTS slots of an
+// array; its staging lives in a file-scope device array, `scratch` of phx_particles.hip, shared
+// by every call on a GPU (calls that overlap on one GPU race)
+constexpr unsigned long long NOALLOC_GATHER_MAX_SLOTS = 1ull << 20;
+// device address of that array on the current device (looked up once per device)
+hipError_t noAllocScratch(void** out);
+// E/B of the live slots of v (A.n <= NOALLOC_GATHER_MAX_SLOTS), compacted in storage order into the
+// array at scratchPtr; noAllocGatherOutputs gives where the E, B values and their count lie
+void launchGatherNoAlloc(PushArgs const& A, PView const& v, void* scratchPtr, hipStream_t s);
+void noAllocGatherOutputs(void* scratchPtr, double const** E, double const** B,
+                          unsigned long long const** total);
 void launchCompact(PView const& src, PView const& dst, uint8_t const* status, unsigned long long n,
                    uint8_t want, unsigned int* blockCounts, unsigned long long* total,
                    unsigned long long dstOffset, int dim, Wrap const& wrap, hipStream_t s);

@@ -20,6 +20,7 @@
 #include "phx_push_common.hpp"
 
 #include <algorithm>
+#include <mutex>
 
 #ifndef PHX_PPT_ND
 #define PHX_PPT_ND 2 // particles per thread of the linear 2D/3D push
@@ -1502,6 +1503,157 @@ void launchMove(PushArgs const& A, PView const& v, SelBoxes const& first, SelBox
 void launchGather(PushArgs const& A, PView const& v, double* E, double* B, hipStream_t s)
 {
     PHX_DISPATCH_DO(launchGatherT, A, v, E, B, s)
+}
+
+// ---------------------------------------------------------------------------------------------
+// gather without allocation: the same E/B values as launchGather, compacted to the live slots in
+// storage order, staged in a file-scope device array instead of a library allocation
+// ---------------------------------------------------------------------------------------------
+namespace
+{
+constexpr unsigned long long NA_BLOCKS = (NOALLOC_GATHER_MAX_SLOTS + BLOCK - 1) / BLOCK;
+// layout: E [3 x max], B [3 x max] (doubles), the total (one 64-bit word), the live count of each
+// workgroup (NA_BLOCKS + 1 words)
+constexpr size_t NA_E_OFF     = 0;
+constexpr size_t NA_B_OFF     = NA_E_OFF + 3 * 8 * NOALLOC_GATHER_MAX_SLOTS;
+constexpr size_t NA_TOTAL_OFF = NA_B_OFF + 3 * 8 * NOALLOC_GATHER_MAX_SLOTS;
+constexpr size_t NA_COUNT_OFF = NA_TOTAL_OFF + 256;
+constexpr size_t NA_BYTES     = NA_COUNT_OFF + 4 * (NA_BLOCKS + 1);
+
+// the scratch of phx_gather_noalloc (every GPU the library runs on holds one until the process
+// ends); host code must use the address hipGetSymbolAddress returns for the current device, never
+// this name
+static __device__ __align__(256) unsigned char scratch[NA_BYTES];
+
+// live slots of each workgroup's range
+__global__ __launch_bounds__(BLOCK) void k_count_live(int32_t const* __restrict__ ic0, unsigned long long n,
+                                                      unsigned int* __restrict__ blockCounts)
+{
+    __shared__ unsigned int s;
+    if (threadIdx.x == 0)
+        s = 0;
+    __syncthreads();
+    unsigned long long const i = static_cast<unsigned long long>(blockIdx.x) * BLOCK + threadIdx.x;
+    bool const f               = i < n && ic0[i] != DEAD_CELL;
+    unsigned long long const m = __ballot(f);
+    if ((threadIdx.x & 63) == 0)
+        atomicAdd(&s, static_cast<unsigned int>(__popcll(m)));
+    __syncthreads();
+    if (threadIdx.x == 0)
+        blockCounts[blockIdx.x] = s;
+}
+
+// E and B of every live slot at its rank among the live slots (offsets: exclusive scan of the
+// workgroup counts)
+template<int D, int O>
+__global__ __launch_bounds__(BLOCK) void k_gather_compact(PushArgs A, PView v,
+                                                          unsigned int const* __restrict__ offsets,
+                                                          double* __restrict__ Eo, double* __restrict__ Bo)
+{
+    __shared__ unsigned int waveBase[BLOCK / 64];
+    unsigned long long const i = static_cast<unsigned long long>(blockIdx.x) * BLOCK + threadIdx.x;
+    Part<D> P;
+    P.cell[0] = DEAD_CELL;
+    if (i < A.n)
+        loadPart<D>(v, i, P);
+    bool const f               = P.cell[0] != DEAD_CELL;
+    unsigned long long const m = __ballot(f);
+    int const lane             = threadIdx.x & 63;
+    int const wave             = threadIdx.x >> 6;
+    if (lane == 0)
+        waveBase[wave] = static_cast<unsigned int>(__popcll(m));
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        unsigned int run = 0;
+        for (int w = 0; w < BLOCK / 64; ++w)
+        {
+            unsigned int const c = waveBase[w];
+            waveBase[w]          = run;
+            run += c;
+        }
+    }
+    __syncthreads();
+    if (!f)
+        return;
+    unsigned long long const below = lane ? (m & ((1ull << lane) - 1ull)) : 0ull;
+    unsigned long long const j
+        = offsets[blockIdx.x] + waveBase[wave] + static_cast<unsigned int>(__popcll(below));
+    Stencil<D, O> s;
+    makeStencil<D, O>(A.G, P.cell, P.delta, s);
+    if (!stencilInside<D, O>(A.G, s, QEX) || !stencilInside<D, O>(A.G, s, QBX)
+        || !stencilInside<D, O>(A.G, s, QEY) || !stencilInside<D, O>(A.G, s, QBY)
+        || !stencilInside<D, O>(A.G, s, QEZ) || !stencilInside<D, O>(A.G, s, QBZ))
+    {
+        raiseError(A.err, 4u, 2u, i, P.delta[0], P.v[0], A.kind, A.pop);
+        return;
+    }
+    double E[3], B[3];
+    gatherEB<D, O>(A.G, A.em, s, E, B);
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+    {
+        Eo[3 * j + c] = E[c];
+        Bo[3 * j + c] = B[c];
+    }
+}
+
+template<int D, int O>
+static void launchGatherCompactT(PushArgs const& A, PView const& v, unsigned int const* offsets,
+                                 double* E, double* B, hipStream_t s)
+{
+    k_gather_compact<D, O><<<compactBlocks(A.n), BLOCK, 0, s>>>(A, v, offsets, E, B);
+}
+
+// device address of `scratch` per device, looked up once per device (with that device current)
+std::mutex naMutex;
+void* naAddr[64] = {};
+} // namespace
+
+hipError_t noAllocScratch(void** out)
+{
+    *out      = nullptr;
+    int dev   = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess)
+        return e;
+    if (dev < 0 || dev >= 64)
+        return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(naMutex);
+    if (!naAddr[dev])
+    {
+        void* p = nullptr;
+        e       = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess)
+            return e;
+        naAddr[dev] = p;
+    }
+    *out = naAddr[dev];
+    return hipSuccess;
+}
+
+void launchGatherNoAlloc(PushArgs const& A, PView const& v, void* scratchPtr, hipStream_t s)
+{
+    if (A.n == 0 || A.n > NOALLOC_GATHER_MAX_SLOTS)
+        return;
+    unsigned char* base   = static_cast<unsigned char*>(scratchPtr);
+    double* E             = reinterpret_cast<double*>(base + NA_E_OFF);
+    double* B             = reinterpret_cast<double*>(base + NA_B_OFF);
+    auto* total           = reinterpret_cast<unsigned long long*>(base + NA_TOTAL_OFF);
+    auto* counts          = reinterpret_cast<unsigned int*>(base + NA_COUNT_OFF);
+    unsigned int const nb = compactBlocks(A.n);
+    k_count_live<<<nb, BLOCK, 0, s>>>(v.ic[0], A.n, counts);
+    k_scan_counts<<<1, 1024, 0, s>>>(counts, nb, total);
+    PHX_DISPATCH_DO(launchGatherCompactT, A, v, counts, E, B, s)
+}
+
+void noAllocGatherOutputs(void* scratchPtr, double const** E, double const** B,
+                          unsigned long long const** total)
+{
+    unsigned char const* base = static_cast<unsigned char const*>(scratchPtr);
+    *E     = reinterpret_cast<double const*>(base + NA_E_OFF);
+    *B     = reinterpret_cast<double const*>(base + NA_B_OFF);
+    *total = reinterpret_cast<unsigned long long const*>(base + NA_TOTAL_OFF);
 }
 
 } // namespace phx

@@ -102,6 +102,7 @@ def lib():
         L.phx_push.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, _ip, _ip, C.c_int,
                                _ip, _ip, C.POINTER(C.c_size_t * 2)]
         L.phx_gather.argtypes = [vp, C.c_int, C.c_int, C.c_int, _dp, _dp]
+        L.phx_gather_noalloc.argtypes = [vp, C.c_int, C.c_int, C.c_int, _dp, _dp]
         L.phx_deposit.argtypes = [vp, C.c_int, C.c_int, C.c_double]
         _lib = L
     return _lib
@@ -118,7 +119,7 @@ EXPORTED = ["phx_create", "phx_destroy", "phx_last_error", "phx_synchronize", "p
             "phx_set_push_timing", "phx_push_timing", "phx_push", "phx_gather", "phx_deposit",
             "phx_create_level", "phx_patch_box", "phx_patch_nonlevel_ghost_boxes",
             "phx_patch_field_shape", "phx_set_patch_field", "phx_get_patch_field",
-            "phx_get_patch_particles", "phx_add_particles"]
+            "phx_get_patch_particles", "phx_add_particles", "phx_gather_noalloc"]
 
 
 class Level:
@@ -365,6 +366,14 @@ class Level:
         E, B = np.zeros((n, 3)), np.zeros((n, 3))
         if n:
             self._rc(lib().phx_gather(self.h, emset, pop, kind, E.reshape(-1), B.reshape(-1)))
+        return E, B
+
+    def gather_noalloc(self, emset=EM_AVG, pop=0, kind=DOMAIN):
+        """phx_gather through phx_gather_noalloc (no device allocation: the library's fixed
+        scratch array stages the values)"""
+        n = self.count(pop, kind)
+        E, B = np.zeros((n, 3)), np.zeros((n, 3))
+        self._rc(lib().phx_gather_noalloc(self.h, emset, pop, kind, E.reshape(-1), B.reshape(-1)))
         return E, B
 
     def deposit(self, pop=0, kind=DOMAIN, coef=1.0):
