@@ -3,19 +3,20 @@
 
 A step is one full predictor-corrector step (SolverPPC::advanceLevel: two updatePopulations
 passes with push + gather + deposit, and the Faraday/Ampere/Ohm field solve), replayed from a
-captured hipGraph, over one periodic level resident in HBM. Default workload: the 2D Harris
-sheet, 256x128 cells, 100 particles per cell (3.28M particles), interpolation order 2, plain
-FP64.
+captured hipGraph (2D) or launched stage by stage (3D), over one periodic level resident in HBM.
+Default workload: the 3D uniform Maxwellian, 128^3 cells, 64 particles per cell (134M particles),
+interpolation order 2, plain FP64 (the largest single-patch configuration); orders 1 and 3 and the
+2D Harris sheet (256x128 cells, 100 ppc, both precision families) are reported under extra keys.
 
 Prints one JSON line: particle-steps/s, the roofline of the dominant kernel (the fused
 push+gather+deposit kernel, algorithmic bytes per launch / its mean duration over the timed
 steps), the same for the mixed-precision family, and the CPU baseline (the FP64 oracle
 restatement of the reference method, 1 thread, timed on a bounded sample of the same workload).
 
-The push's duration is taken inside the timed, graph-replayed steps: each launch stamps the
-device wall clock at its first workgroup's start and its last workgroup's end (a HIP event
-recorded inside a captured step would keep only the last replay's time); rocprofv3's kernel
-trace of the same command agrees (profiles/).
+The push's duration is taken inside the timed steps: in 2D each launch stamps the device wall clock
+at its first workgroup's start and its last workgroup's end (a HIP event recorded inside a captured
+step would keep only the last replay's time); in 3D HIP events bracket each push launch. rocprofv3's
+kernel trace of the same command agrees (profiles/).
 """
 import argparse
 import ctypes
@@ -78,8 +79,6 @@ def workload(name):
     if name == "uniform1d":
         return W.uniform1d()
     if name.startswith("uniform3d"):
-        raise SystemExit("uniform3d: the captured 3D periodic step is disabled in this build "
-                         "(phx_advance returns PHX_ERR_STATE for dim 3)")
         # uniform3d[-o<order>][-n<cells per side>]
         order, n = 2, 128
         for tok in name.split("-")[1:]:
@@ -159,33 +158,81 @@ def run_family(wl, precision, steps, warmup, drift_steps=0):
                 charge_drift_run=(diagN["charge"] - diag0["charge"]) / diag0["charge"])
 
 
-def cpu_baseline(wl, steps):
+def cpu_baseline(wl, steps, runs=3):
     """the oracle (FP64 restatement of the reference method: AoS particles, CellMap, the same
-    loops) on one host thread, on `steps` PPC steps of the same workload after one warm-up step"""
+    loops) on one host thread, on `steps` PPC steps of the workload after one warm-up step; the
+    median rate of `runs` runs"""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
 
-    sim = O.Sim(wl.dim, wl.order, wl.ncells, wl.dx, masses=wl.masses, Te=wl.Te)
-    wl.setup(sim, O.F_B)
-    sim.advance(wl.dt, 1)
-    t0 = time.perf_counter()
-    sim.advance(wl.dt, steps)
-    el = time.perf_counter() - t0
-    return dict(value=wl.n_particles * steps / el, unit="particle-steps/s", cores=1, kind="port",
-                sample=f"{wl.name} {'x'.join(map(str, wl.ncells))} o{wl.order} {wl.ppc} ppc: "
-                       f"{steps} PPC steps after 1 warm-up step ({el:.1f} s), 1 thread, no MPI",
-                seconds=el)
+    rates, secs = [], []
+    for _ in range(runs):
+        sim = O.Sim(wl.dim, wl.order, wl.ncells, wl.dx, masses=wl.masses, Te=wl.Te)
+        wl.setup(sim, O.F_B)
+        sim.advance(wl.dt, 1)
+        t0 = time.perf_counter()
+        sim.advance(wl.dt, steps)
+        el = time.perf_counter() - t0
+        rates.append(wl.n_particles * steps / el)
+        secs.append(el)
+        del sim
+    med = float(np.median(rates))
+    return dict(value=med, unit="particle-steps/s", cores=1, kind="port",
+                sample=f"{wl.name} {'x'.join(map(str, wl.ncells))} o{wl.order} {wl.ppc} ppc "
+                       f"({wl.n_particles} particles): {steps} PPC steps after 1 warm-up step, "
+                       f"median of {runs} runs ({', '.join(f'{x:.1f}' for x in secs)} s), "
+                       f"1 thread, no MPI",
+                seconds=float(np.sum(secs)))
+
+
+def cpu_sample(wl):
+    """a bounded CPU sample of a workload: the same profiles, order and particles per cell on a
+    smaller box when the workload is too large for ~10-30 s of one host core"""
+    if wl.name == "uniform3d" and wl.n_cells > 32 ** 3:
+        return W.uniform3d(ncells=(32, 32, 32), ppc=wl.ppc, order=wl.order, dx=wl.dx[0]), 3
+    return wl, 20
+
+
+def family_line(wl, workload_name, precision, res):
+    """the throughput and the push roofline of one run_family result"""
+    b = push_bytes(wl, phx.MODE_ALL, precision)
+    a = b / (res["ms_all"] * 1e-3) / 1e9
+    if wl.dim == 3:
+        kernel = "k_push (linear storage: prePush+gather+Boris+postPush+deposit, mode all; 3D staged path)"
+    elif precision == "fp32":
+        kernel = "k_push_seg32 (mixed precision, mode all)"
+    else:
+        kernel = "k_push_seg (fused prePush+gather+Boris+postPush+deposit on cell-segmented storage, mode all)"
+    return {"value": res["value"], "unit": "particle-steps/s", "ms_per_step": res["ms_step"],
+            "roofline": {"bound": "hbm", "achieved": a, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": a / HBM_PEAK_GBS,
+                         "traffic": MEASURED_TRAFFIC.get((workload_name, precision)),
+                         "kernel": kernel, "kernel_ms": res["ms_all"],
+                         "kernel_ms_domain_only": res["ms_dom"], "algorithmic_bytes_per_launch": b},
+            "step_hbm_fraction": step_bytes(wl, precision) / (res["ms_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "energy_drift": res["energy_drift"], "charge_drift": res["charge_drift"],
+            "run": {"steps": res["run_steps"], "energy_drift": res["energy_drift_run"],
+                    "charge_drift": res["charge_drift_run"]}}
+
+
+def describe(wl, precision):
+    return (f"{wl.name} {'x'.join(map(str, wl.ncells))} cells, order {wl.order}, {wl.ppc} ppc, "
+            f"{wl.n_particles} particles, dt={wl.dt}, {'plain FP64' if precision == 'fp64' else 'mixed FP32/FP64'}, "
+            f"one periodic patch")
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="harris2d")
-    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="uniform3d",
+                    help="uniform3d[-o<order>][-n<cells>] (3D uniform, 128^3 x 64 ppc: the largest "
+                         "single-patch configuration), harris2d or uniform1d")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision family")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the extra lines (2D Harris both families; 3D orders 1 and 3)")
     ap.add_argument("--no-drift-run", action="store_true",
                     help="skip the untimed continuation to the workload's run length (energy drift)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
@@ -201,9 +248,10 @@ def main():
 
     wl = workload(args.workload)
     drift_steps = 0 if args.no_drift_run else wl.nsteps
+    print(f"[bench] {describe(wl, 'fp64')}: {args.warmup} + {args.steps} steps", file=sys.stderr,
+          flush=True)
     f64 = run_family(wl, "fp64", args.steps, args.warmup, drift_steps)
-    bytes_all = push_bytes(wl, phx.MODE_ALL)
-    achieved = bytes_all / (f64["ms_all"] * 1e-3) / 1e9
+    line = family_line(wl, args.workload, "fp64", f64)
     out = {
         "metric": "particle-steps/s (push+gather+deposit)",
         "value": f64["value"],
@@ -217,48 +265,52 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"{wl.name} {'x'.join(map(str, wl.ncells))} cells, order {wl.order}, "
-                               f"{wl.ppc} ppc, {wl.n_particles} particles, dt={wl.dt}, plain FP64, "
-                               f"one periodic patch",
-                   "particles": wl.n_particles, "cells": wl.n_cells, "interp_order": wl.order,
-                   "precision": "fp64", "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": MEASURED_TRAFFIC.get((args.workload, "fp64")),
-                     "kernel": "k_push_seg (fused prePush+gather+Boris+postPush+deposit on cell-segmented storage, mode all)",
-                     "kernel_ms": f64["ms_all"], "kernel_ms_domain_only": f64["ms_dom"],
-                     "algorithmic_bytes_per_launch": bytes_all},
-        "step_hbm_fraction": step_bytes(wl) / (f64["ms_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "config": {"workload": describe(wl, "fp64"), "particles": wl.n_particles,
+                   "cells": wl.n_cells, "interp_order": wl.order, "precision": "fp64",
+                   "parallelism": "single GPU"},
+        "roofline": line["roofline"],
+        "step_hbm_fraction": line["step_hbm_fraction"],
         "energy_drift": f64["energy_drift"],
         "charge_drift": f64["charge_drift"],
-        "run": {"steps": f64["run_steps"], "energy_drift": f64["energy_drift_run"],
-                "charge_drift": f64["charge_drift_run"]},
+        "run": line["run"],
+        "headline": "the north star's headline (batched FP32 vs plain FP64 on the 3D AMR "
+                    "two-population configuration) is not measurable in this build: no AMR "
+                    "hierarchy and no 3D mixed-precision path",
     }
     if not args.no_mixed:
-        # the mixed-precision family on the same workload (FP32 particles, FP64 fields)
-        f32 = run_family(wl, "fp32", args.steps, args.warmup, drift_steps)
-        b32 = push_bytes(wl, phx.MODE_ALL, "fp32")
-        a32 = b32 / (f32["ms_all"] * 1e-3) / 1e9
-        out["mixed_fp32"] = {
-            "value": f32["value"], "unit": "particle-steps/s", "ms_per_step": f32["ms_step"],
-            "speedup_vs_fp64": f32["value"] / f64["value"],
-            "roofline": {"bound": "hbm", "achieved": a32, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": a32 / HBM_PEAK_GBS,
-                         "traffic": MEASURED_TRAFFIC.get((args.workload, "fp32")),
-                         "kernel": "k_push_seg32 (mixed precision, mode all)",
-                         "kernel_ms": f32["ms_all"], "kernel_ms_domain_only": f32["ms_dom"],
-                         "algorithmic_bytes_per_launch": b32},
-            "step_hbm_fraction": step_bytes(wl, "fp32") / (f32["ms_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
-            "energy_drift": f32["energy_drift"],
-            "energy_drift_minus_fp64": f32["energy_drift"] - f64["energy_drift"],
-            "charge_drift": f32["charge_drift"],
-            "drift_steps": f32["steps_done"],
-            "run": {"steps": f32["run_steps"], "energy_drift": f32["energy_drift_run"],
-                    "charge_drift": f32["charge_drift_run"],
-                    "energy_drift_minus_fp64": f32["energy_drift_run"] - f64["energy_drift_run"]},
-        }
+        if wl.dim == 3:
+            out["mixed_fp32"] = {"unavailable": "the mixed-precision push runs on the captured "
+                                                "fused step, which is off in 3D"}
+        else:
+            f32 = run_family(wl, "fp32", args.steps, args.warmup, drift_steps)
+            m = family_line(wl, args.workload, "fp32", f32)
+            m["speedup_vs_fp64"] = f32["value"] / f64["value"]
+            m["energy_drift_minus_fp64"] = f32["energy_drift"] - f64["energy_drift"]
+            m["run"]["energy_drift_minus_fp64"] = f32["energy_drift_run"] - f64["energy_drift_run"]
+            out["mixed_fp32"] = m
+    if not args.no_extra and args.workload == "uniform3d":
+        # the other interpolation orders of the 3D workload (fewer steps, no drift run), and the
+        # 2D Harris workload in both families (its 1000-step drift run included)
+        for o in (1, 3):
+            w = workload(f"uniform3d-o{o}")
+            print(f"[bench] {describe(w, 'fp64')}", file=sys.stderr, flush=True)
+            r = run_family(w, "fp64", max(5, args.steps // 2), 2, 0)
+            out[f"uniform3d_order{o}"] = family_line(w, f"uniform3d-o{o}", "fp64", r)
+        h = workload("harris2d")
+        print(f"[bench] {describe(h, 'fp64')} (both families)", file=sys.stderr, flush=True)
+        h64 = run_family(h, "fp64", 200, 20, 0 if args.no_drift_run else h.nsteps)
+        h32 = run_family(h, "fp32", 200, 20, 0 if args.no_drift_run else h.nsteps)
+        hl = family_line(h, "harris2d", "fp64", h64)
+        hm = family_line(h, "harris2d", "fp32", h32)
+        hm["speedup_vs_fp64"] = h32["value"] / h64["value"]
+        hm["run"]["energy_drift_minus_fp64"] = h32["energy_drift_run"] - h64["energy_drift_run"]
+        hl["config"] = describe(h, "fp64")
+        hl["mixed_fp32"] = hm
+        out["harris2d"] = hl
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(wl, args.cpu_steps)
+        cw, csteps = cpu_sample(wl)
+        print(f"[bench] CPU baseline on {describe(cw, 'fp64')}", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(cw, csteps)
     print(json.dumps(out), flush=True)
 
 

@@ -252,6 +252,26 @@ phx_status phx_diagnostics(phx_ctx* ctx, double out[4]);
 /* the HIP stream the context launches on (a hipStream_t), for timing with HIP events */
 void* phx_stream(phx_ctx* ctx);
 
+/* ---- tuning (no counterpart in the reference: launch shapes of this build) ------------------- */
+/* The shapes the segmented push and the sort use; the defaults are the winners of the project's
+   autotuner (phare_amd/autotune.py) on the 2D Harris workload. phx_set_tuning rebuilds the
+   segment tables of the level's patches, re-sorts the domain arrays that are cell-segmented and
+   drops the captured steps (the next phx_advance captures again). */
+typedef struct {
+    int segment_shape[3]; /* preferred segment shape in cells per dimension (the product is at
+                             most 64; clipped to each patch, the cells left over go to the slower
+                             dimensions); 0 keeps the default of the dimension */
+    int rows_per_item;    /* most segment rows per workgroup of the segmented push (1..4096) */
+    int tile_row_pad;     /* 2D LDS tile rows padded to a multiple of this many nodes (1: none) */
+    int sort_interval;    /* PPC steps of phx_advance between two sorts (0: never) */
+    int tile_copies;      /* moment copies of the LDS tile of the linear-storage push (1, 2, 4, 8;
+                             lane l adds into copy l % copies) */
+    int deposit_window;   /* mixed-precision push: 1 sums each lane's successive particles of one
+                             cell in registers before the LDS tile (1D, 2D and 3D order 1), 0 adds
+                             every particle into the tile */
+} phx_tuning;
+phx_status phx_get_tuning(const phx_ctx* ctx, phx_tuning* out);
+phx_status phx_set_tuning(phx_ctx* ctx, const phx_tuning* tuning);
 /* ---- measurement (no counterpart in the reference: a benchmark hook) ------------------------ */
 /* captures only the fused push+gather+deposit kernel of a periodic level (population 0, E/B of
    the averaged set) in a hipGraph, replays it `reps` times back to back between two HIP events

@@ -36,6 +36,13 @@
 #ifndef PHX_SEG_COPIES
 #define PHX_SEG_COPIES 1
 #endif
+// default launch shapes of the segmented push (phx_tuning)
+#ifndef PHX_SEG_ROWS
+#define PHX_SEG_ROWS 32
+#endif
+#ifndef PHX_TILE_ROW_PAD
+#define PHX_TILE_ROW_PAD 16
+#endif
 
 using namespace phx;
 
@@ -135,6 +142,8 @@ struct phx_ctx
     size_t permCap         = 0;
     int sortInterval      = 30;
     int stepsSinceSort    = 0;
+    // launch shapes of the segmented push (phx_set_tuning)
+    phx_tuning tuning{};
 
     // captured PPC steps, keyed on dt, the raw deposit set and the particle storage they address
     struct GraphEntry
@@ -147,6 +156,15 @@ struct phx_ctx
     GraphEntry graphs[2];
     // device address of the scratch array of phx_gather_noalloc on the context's device
     void* noAllocScratch = nullptr;
+    // timing of the pushes of the staged 3D path: a HIP event pair around each push launch
+    // (eager launches), kept until phx_set_push_timing resets them
+    struct EventPair
+    {
+        hipEvent_t a = nullptr, b = nullptr;
+        int mode = 0, pop = 0;
+    };
+    std::vector<EventPair> events;
+    size_t eventsUsed = 0;
     int graphNext = 0;
     // in-kernel timing of the pushes of the fast path, per (mode, population)
     KTime* kt   = nullptr;
@@ -394,12 +412,14 @@ PushArgs pushArgs(phx_ctx* c, int pop, int kind, int mode, int emSet, double dt,
     double ppc = static_cast<double>(A.n);
     for (int d = 0; d < c->cfg.dim; ++d)
         ppc /= c->G.n[d];
-    int const copies = c->cfg.dim == 1 ? PHX_TILE_COPIES_1D
-                                       : (c->cfg.dim == 2 ? PHX_TILE_COPIES_2D : PHX_TILE_COPIES_3D);
+    int const copies = c->tuning.tile_copies;
     A.tile = tileShape(c->cfg.dim, c->cfg.interp_order, ppc, copies);
     Pop const& P   = c->pops[pop];
     Store const& s = P.arr[kind];
-    if (segOk && kind == KIND_DOMAIN && s.seg && segItemsCap(c->sg, s.cap) <= P.itemsAlloc)
+    // (3D: the linear-storage kernel on any storage order; the segmented push stays off in 3D with
+    // the captured fused step)
+    if (segOk && c->cfg.dim != 3 && kind == KIND_DOMAIN && s.seg
+        && segItemsCap(c->sg, s.cap) <= P.itemsAlloc)
     {
         A.segOn       = 1;
         A.sg          = c->sg;
@@ -408,6 +428,7 @@ PushArgs pushArgs(phx_ctx* c, int pop, int kind, int mode, int emSet, double dt,
         A.tile        = mixed(c) ? segTileShape32(c->cfg.dim, c->cfg.interp_order, c->sg)
                                  : segTileShape(c->cfg.dim, c->cfg.interp_order, c->sg, PHX_SEG_COPIES);
         A.rhoFromN    = !mixed(c) && P.hasCharge && P.uniformCharge ? 1 : 0;
+        A.depositWindow = c->tuning.deposit_window;
     }
     return A;
 }
@@ -663,15 +684,86 @@ phx_status enqueueAdvanceFast(phx_ctx* c, double dt)
     return checkLaunch(c, "advance");
 }
 
-// the fused periodic path in 3D (segmented push, fused field kernels) stays off: one GPU run of it
-// faulted and the cause is not shown
-phx_status fused3dGuard(phx_ctx* c, char const* who)
+// A periodic level in 3D runs SolverPPC::advanceLevel staged from the kernels of the drop-in
+// entry points (linear-storage push, Faraday/Ampere/Ohm on the physical nodes followed by the
+// periodic ghost fill, the border sum and max of the moments, the ion totals), eagerly on the
+// context stream: the captured fused step (segmented push, fused field kernels) stays off in 3D,
+// one GPU run of it having faulted without a cause shown.
+bool staged3d(phx_ctx const* c) { return c->cfg.dim == 3 && c->patch.periodic; }
+
+// SolverPPC::moveIons_ (solver_ppc.hpp:537-598) of a 3D periodic level: the push deposits into
+// the population moments and wraps leavers back through the periodic image in place
+// (fillIonGhostParticles), then fillFluxBorders/fillDensityBorders, updateIons, fillIonBorders
+phx_status moveIonsStaged3d(phx_ctx* c, double dt, int mode)
 {
-    if (c->cfg.dim != 3)
-        return PHX_OK;
-    return fail(c, PHX_ERR_STATE,
-                std::string(who) + ": the fused 3D periodic path is disabled (a GPU run of it "
-                                   "faulted and the cause is not shown)");
+    zeroMoments(c);
+    for (int p = 0; p < c->cfg.n_pop; ++p)
+    {
+        PushArgs A     = pushArgs(c, p, KIND_DOMAIN, mode, PHX_EM_AVG, dt, false);
+        A.periodicWrap = 1;
+        Store const& dom = c->pops[p].arr[PHX_DOMAIN];
+        phx_ctx::EventPair* ev = nullptr;
+        if (c->timing)
+        {
+            if (c->eventsUsed == c->events.size())
+            {
+                phx_ctx::EventPair e;
+                PHX_HIP(c, hipEventCreate(&e.a));
+                PHX_HIP(c, hipEventCreate(&e.b));
+                c->events.push_back(e);
+            }
+            ev       = &c->events[c->eventsUsed++];
+            ev->mode = mode;
+            ev->pop  = p;
+            PHX_HIP(c, hipEventRecord(ev->a, c->stream));
+        }
+        launchPush(A, dom.view(), dom.view(), c->stream);
+        if (ev)
+            PHX_HIP(c, hipEventRecord(ev->b, c->stream));
+    }
+    PHX_TRY(checkLaunch(c, "push"));
+    PHX_TRY(fillMomentBorders(c));
+    updateIons(c);
+    return fillIonBorders(c);
+}
+
+// predictor1_/predictor2_ (solver_ppc.hpp:346-428) of a 3D periodic level: Faraday, then the B
+// ghost fill; Ampere, then the J ghost fill; the electron closure; Ohm on the physical nodes (the
+// average that follows takes Eavg's ghost nodes from their periodic sources)
+void predictorStaged3d(phx_ctx* c, double dt, int eset)
+{
+    launchFaraday(c->G, cvec(c->em[PHX_EM] + 3), cvec(c->em[eset]), vec(c->em[PHX_EM_PRED] + 3), dt,
+                  c->stream);
+    fillGhosts(c, c->em[PHX_EM_PRED] + 3, QBX);
+    launchAmpere(c->G, cvec(c->em[PHX_EM_PRED] + 3), vec(c->J), c->stream);
+    fillGhosts(c, c->J, QJX);
+    electrons(c);
+    ohm(c, PHX_EM_PRED, PHX_EM_PRED);
+}
+
+// corrector_ (solver_ppc.hpp:433-479) of a 3D periodic level, each stage followed by its fill
+void correctorStaged3d(phx_ctx* c, double dt)
+{
+    launchFaraday(c->G, cvec(c->em[PHX_EM] + 3), cvec(c->em[PHX_EM_AVG]), vec(c->em[PHX_EM] + 3), dt,
+                  c->stream);
+    fillGhosts(c, c->em[PHX_EM] + 3, QBX);
+    launchAmpere(c->G, cvec(c->em[PHX_EM] + 3), vec(c->J), c->stream);
+    fillGhosts(c, c->J, QJX);
+    electrons(c);
+    ohm(c, PHX_EM, PHX_EM);
+    fillGhosts(c, c->em[PHX_EM], QEX);
+}
+
+phx_status advanceStaged3d(phx_ctx* c, double dt)
+{
+    predictorStaged3d(c, dt, PHX_EM);
+    enqueueAverage(c);
+    PHX_TRY(moveIonsStaged3d(c, dt, PHX_MODE_DOMAIN_ONLY));
+    predictorStaged3d(c, dt, PHX_EM_AVG);
+    enqueueAverage(c);
+    PHX_TRY(moveIonsStaged3d(c, dt, PHX_MODE_ALL));
+    correctorStaged3d(c, dt);
+    return checkLaunch(c, "advance");
 }
 
 phx_status validPop(phx_ctx* c, int pop, int kind)
@@ -1074,9 +1166,14 @@ phx_status buildPatchTable(phx_ctx* c, int const* plo, int const* phi, int np)
 {
     int const D = c->cfg.dim;
     SegGeom& S  = c->sg;
-    S           = SegGeom{};
-    S.dim       = D;
-    S.npatch    = np;
+    devFree(c, const_cast<SegPatch*>(S.patch));
+    devFree(c, const_cast<int*>(S.owner));
+    devFree(c, const_cast<unsigned int*>(S.segPatch));
+    S             = SegGeom{};
+    S.dim         = D;
+    S.npatch      = np;
+    S.rowsPerItem = c->tuning.rows_per_item;
+    S.rowPad      = c->tuning.tile_row_pad;
     size_t boxCells = 1;
     for (int d = 0; d < 3; ++d)
     {
@@ -1099,7 +1196,7 @@ phx_status buildPatchTable(phx_ctx* c, int const* plo, int const* phi, int np)
             if (hi[d] < lo[d] || lo[d] < S.lo[d] || hi[d] >= S.lo[d] + S.n[d])
                 return fail(c, PHX_ERR_ARG, "patch " + std::to_string(p) + " is empty or leaves the level box");
         }
-        SegPatch P = segPatchOf(D, lo, hi);
+        SegPatch P = segPatchOf(D, lo, hi, c->tuning.segment_shape);
         P.segBase  = segBase;
         P.cellBase = cellBase;
         unsigned int cells = 1, segs = 1;
@@ -1151,6 +1248,46 @@ phx_status buildPatchTable(phx_ctx* c, int const* plo, int const* phi, int np)
     S.owner    = dO;
     S.segPatch = dS;
     return PHX_OK;
+}
+
+// the sort's work arrays and every population's segment tables, sized for the patch table
+phx_status allocSegTables(phx_ctx* c)
+{
+    size_t const nsegs = static_cast<size_t>(c->sg.nsegs);
+    devFree(c, c->segWork);
+    c->segWork = nullptr;
+    PHX_TRY(devAlloc(c, (void**)&c->segWork, (static_cast<size_t>(c->sg.ncells) + 1 + 2 * (nsegs + 1)) * 4));
+    for (int p = 0; p < c->cfg.n_pop; ++p)
+    {
+        SegTables& T = c->pops[p].seg;
+        devFree(c, T.rows);
+        devFree(c, T.off);
+        devFree(c, T.nItems);
+        T.rows   = nullptr;
+        T.off    = nullptr;
+        T.nItems = nullptr;
+        PHX_TRY(devAlloc(c, (void**)&T.rows, (nsegs + 1) * 4));
+        PHX_TRY(devAlloc(c, (void**)&T.off, (nsegs + 2) * 8));
+        PHX_TRY(devAlloc(c, (void**)&T.nItems, 64));
+    }
+    return PHX_OK;
+}
+
+// the launch shapes of this build before any tuning: the autotuner's winners on the 2D Harris
+// workload (segments of 2x32 cells, at most 32 rows per work item, tile rows padded to 16 nodes,
+// a sort every 30 PPC steps); runs of 64 cells in 1D, 2x4x8 blocks in 3D
+phx_tuning defaultTuning(int dim)
+{
+    phx_tuning t{};
+    static int const pref[3][3] = {{64, 1, 1}, {2, 32, 1}, {2, 4, 8}};
+    for (int d = 0; d < 3; ++d)
+        t.segment_shape[d] = pref[dim - 1][d];
+    t.rows_per_item = PHX_SEG_ROWS;
+    t.tile_row_pad  = PHX_TILE_ROW_PAD;
+    t.sort_interval = 30;
+    t.tile_copies   = dim == 1 ? PHX_TILE_COPIES_1D : (dim == 2 ? PHX_TILE_COPIES_2D : PHX_TILE_COPIES_3D);
+    t.deposit_window = 0;
+    return t;
 }
 
 // a level of n_patches patches (AMR domain boxes) inside the box of `level` (its mesh, level
@@ -1309,23 +1446,12 @@ phx_status createLevel(phx_config const* cfg, phx_patch const* patch, int const*
         for (int k = 0; k < 3 && s == PHX_OK; ++k)
             s = storeAlloc(c, c->pops[p].arr[k], k == 0 ? std::max<size_t>(particle_capacity, 1024) : 1024);
     }
+    c->tuning = defaultTuning(D);
+    c->sortInterval = c->tuning.sort_interval;
     if (s == PHX_OK)
         s = buildPatchTable(c, plo, phi, np);
     if (s == PHX_OK)
-    {
-        // segmented storage over the level's patches
-        size_t const nsegs = static_cast<size_t>(c->sg.nsegs);
-        s = devAlloc(c, (void**)&c->segWork, (static_cast<size_t>(c->sg.ncells) + 1 + 2 * (nsegs + 1)) * 4);
-        for (int p = 0; p < cfg->n_pop && s == PHX_OK; ++p)
-        {
-            SegTables& T = c->pops[p].seg;
-            s = devAlloc(c, (void**)&T.rows, (nsegs + 1) * 4);
-            if (s == PHX_OK)
-                s = devAlloc(c, (void**)&T.off, (nsegs + 2) * 8);
-            if (s == PHX_OK)
-                s = devAlloc(c, (void**)&T.nItems, 64);
-        }
-    }
+        s = allocSegTables(c);
     if (s == PHX_OK)
         s = devAlloc(c, (void**)&c->err, sizeof(DevError));
     if (s == PHX_OK)
@@ -1403,6 +1529,11 @@ void phx_destroy(phx_ctx* c)
     for (auto& g : c->graphs)
         if (g.exec)
             (void)hipGraphExecDestroy(g.exec);
+    for (auto& e : c->events)
+    {
+        (void)hipEventDestroy(e.a);
+        (void)hipEventDestroy(e.b);
+    }
     for (void* p : c->allocs)
         (void)hipFree(p);
     if (c->stream)
@@ -2171,9 +2302,15 @@ phx_status phx_move_ions(phx_ctx* c, double dt, int mode)
         return PHX_ERR_ARG;
     if (mode != PHX_MODE_DOMAIN_ONLY && mode != PHX_MODE_ALL)
         return fail(c, PHX_ERR_ARG, "bad mode");
+    if (fastPath(c) && staged3d(c))
+    {
+        if (mixed(c))
+            return fail(c, PHX_ERR_STATE, "phx_move_ions: the mixed-precision family has no 3D path");
+        PHX_TRY(moveIonsStaged3d(c, dt, mode));
+        return checkDeviceError(c);
+    }
     if (fastPath(c))
     {
-        PHX_TRY(fused3dGuard(c, "phx_move_ions"));
         // the averaged set may have been set or computed outside the average kernel
         PHX_TRY(sortIfLinear(c));
         if (mixed(c))
@@ -2202,7 +2339,68 @@ phx_status phx_set_sort_interval(phx_ctx* c, int steps)
 {
     if (!c || steps < 0)
         return PHX_ERR_ARG;
-    c->sortInterval = steps;
+    c->sortInterval          = steps;
+    c->tuning.sort_interval = steps;
+    return PHX_OK;
+}
+
+phx_status phx_get_tuning(phx_ctx const* c, phx_tuning* out)
+{
+    if (!c || !out)
+        return PHX_ERR_ARG;
+    *out = c->tuning;
+    return PHX_OK;
+}
+
+phx_status phx_set_tuning(phx_ctx* c, phx_tuning const* t)
+{
+    if (!c || !t)
+        return PHX_ERR_ARG;
+    int const D       = c->cfg.dim;
+    phx_tuning next   = *t;
+    phx_tuning const d = defaultTuning(D);
+    int cells         = 1;
+    for (int k = 0; k < 3; ++k)
+    {
+        if (next.segment_shape[k] == 0 || k >= D)
+            next.segment_shape[k] = k < D ? d.segment_shape[k] : 1;
+        if (next.segment_shape[k] < 1 || next.segment_shape[k] > 64)
+            return fail(c, PHX_ERR_ARG, "phx_set_tuning: segment extents are 1..64 cells");
+        cells *= next.segment_shape[k];
+    }
+    if (cells > 64)
+        return fail(c, PHX_ERR_ARG, "phx_set_tuning: a segment holds at most 64 cells");
+    if (next.rows_per_item < 1 || next.rows_per_item > 4096 || next.tile_row_pad < 1
+        || next.tile_row_pad > 64 || next.sort_interval < 0
+        || (next.tile_copies != 1 && next.tile_copies != 2 && next.tile_copies != 4
+            && next.tile_copies != 8)
+        || (next.deposit_window != 0 && next.deposit_window != 1))
+        return fail(c, PHX_ERR_ARG, "phx_set_tuning: rows per item 1..4096, row pad 1..64, "
+                                    "sort interval >= 0, tile copies 1, 2, 4 or 8, deposit "
+                                    "window 0 or 1");
+    PHX_HIP(c, hipStreamSynchronize(c->stream));
+    // the captured steps address the segment tables: drop them
+    for (auto& g : c->graphs)
+    {
+        if (g.exec)
+            (void)hipGraphExecDestroy(g.exec);
+        g = phx_ctx::GraphEntry{};
+    }
+    c->tuning       = next;
+    c->sortInterval = next.sort_interval;
+    std::vector<int> plo, phi;
+    for (auto const& b : c->patchBox)
+    {
+        plo.insert(plo.end(), {b[0], b[1], b[2]});
+        phi.insert(phi.end(), {b[3], b[4], b[5]});
+    }
+    PHX_TRY(buildPatchTable(c, plo.data(), phi.data(), static_cast<int>(c->patchBox.size())));
+    PHX_TRY(allocSegTables(c));
+    bool segmented = false;
+    for (int p = 0; p < c->cfg.n_pop; ++p)
+        segmented = segmented || c->pops[p].arr[PHX_DOMAIN].seg;
+    if (segmented)
+        PHX_TRY(sortDomain(c));
     return PHX_OK;
 }
 
@@ -2211,7 +2409,10 @@ phx_status phx_initialize(phx_ctx* c)
     if (!c)
         return PHX_ERR_ARG;
     fillGhosts(c, c->em[PHX_EM] + 3, QBX);
-    PHX_TRY(sortDomain(c));
+    if (!staged3d(c))
+        PHX_TRY(sortDomain(c));
+    else if (mixed(c))
+        return fail(c, PHX_ERR_STATE, "phx_initialize: the mixed-precision family has no 3D path");
     zeroMoments(c);
     for (int p = 0; p < c->cfg.n_pop; ++p)
     {
@@ -2258,7 +2459,16 @@ phx_status phx_advance(phx_ctx* c, double dt, int nsteps)
         }
         return checkDeviceError(c);
     }
-    PHX_TRY(fused3dGuard(c, "phx_advance"));
+    if (staged3d(c))
+    {
+        if (mixed(c))
+            return fail(c, PHX_ERR_STATE,
+                        "phx_advance: the mixed-precision family has no 3D path (its push runs on "
+                        "the captured fused step, which is off in 3D)");
+        for (int s = 0; s < nsteps; ++s)
+            PHX_TRY(advanceStaged3d(c, dt));
+        return checkDeviceError(c);
+    }
     PHX_TRY(sortIfLinear(c));
     for (int s = 0; s < nsteps; ++s)
     {
@@ -2338,7 +2548,8 @@ phx_status phx_set_push_timing(phx_ctx* c, int on)
             k = KTime{~0ull, 0ull, 0ull, 0u, 0u};
         PHX_HIP(c, hipMemcpy(c->kt, z, sizeof(z), hipMemcpyHostToDevice));
     }
-    c->timing = on != 0;
+    c->timing     = on != 0;
+    c->eventsUsed = 0;
     return PHX_OK;
 }
 
@@ -2350,6 +2561,25 @@ phx_status phx_push_timing(phx_ctx* c, int pop, int mode, double* avg_ms, int* l
     if (!c->kt)
         return fail(c, PHX_ERR_STATE, "push timing was never enabled");
     PHX_HIP(c, hipStreamSynchronize(c->stream));
+    if (staged3d(c))
+    {
+        // the event pairs of the staged 3D path's pushes
+        double total = 0.;
+        int n        = 0;
+        for (size_t i = 0; i < c->eventsUsed; ++i)
+        {
+            auto const& e = c->events[i];
+            if (e.mode != mode || e.pop != pop)
+                continue;
+            float ms = 0.f;
+            PHX_HIP(c, hipEventElapsedTime(&ms, e.a, e.b));
+            total += ms;
+            ++n;
+        }
+        *launches = n;
+        *avg_ms   = n ? total / n : 0.;
+        return PHX_OK;
+    }
     KTime k{};
     PHX_HIP(c, hipMemcpy(&k, c->kt + (mode == PHX_MODE_ALL ? PHX_MAX_POP : 0) + pop, sizeof(k),
                          hipMemcpyDeviceToHost));
@@ -2367,7 +2597,10 @@ phx_status phx_profile_push(phx_ctx* c, double dt, int mode, int reps, double* a
         return PHX_ERR_ARG;
     if (!fastPath(c))
         return fail(c, PHX_ERR_STATE, "phx_profile_push needs a periodic level without ghost arrays");
-    PHX_TRY(fused3dGuard(c, "phx_profile_push"));
+    if (staged3d(c))
+        return fail(c, PHX_ERR_STATE,
+                    "phx_profile_push: a 3D level runs the staged path (its pushes are timed by "
+                    "phx_set_push_timing)");
     // the push kernel alone, captured in a graph and replayed back to back between two events
     // on the context stream (the mean over the replays is the kernel time; per-launch event
     // pairs around direct launches read high on this runtime)

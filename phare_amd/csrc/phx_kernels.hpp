@@ -98,6 +98,8 @@ struct SegGeom
     int lo[3];     // lowest AMR cell of the level box (the box of the owner map)
     int n[3];      // level box cells per dimension
     int maxSeg[3]; // largest segment extent over the patches (the launch's LDS tile)
+    int rowsPerItem; // most segment rows per work item of the segmented push
+    int rowPad;      // 2D LDS tile rows padded to a multiple of this many nodes
     SegPatch const* patch;        // [npatch] (device)
     int const* owner;             // [level box cells, row-major] patch of each cell, -1: none
     unsigned int const* segPatch; // [nsegs] patch of each segment
@@ -148,6 +150,8 @@ struct PushArgs
     double charge; // the population's charge (mixed-precision family; FP64 family with rhoFromN)
     int rhoFromN;  // FP64 segmented push: every particle has charge `charge`, so the LDS tile
                    // leaves the charge density out and its flush takes charge x n
+    int depositWindow; // mixed-precision segmented push: sum each lane's successive particles of
+                       // one cell in a register window before the LDS tile (1D, 2D, 3D order 1)
 };
 
 // box list of a particle selector (UpdaterSelectionBoxing, ion_updater.hpp:119-163): a cell is
@@ -174,7 +178,7 @@ unsigned int compactBlocks(unsigned long long n);
 
 // segmented storage: segment shape of a patch box (segBase/cellBase left to the caller), LDS
 // tile of the segmented push, the most work items a storage of `capacity` slots can need
-SegPatch segPatchOf(int dim, int const* lo, int const* hi);
+SegPatch segPatchOf(int dim, int const* lo, int const* hi, int const* pref);
 TileSpec segTileShape(int dim, int order, SegGeom const& sg, int copies);
 unsigned int segItemsCap(SegGeom const& sg, unsigned long long capacity);
 // first half of a sort of src (n slots) into cell-segmented storage: counts per cell, the rank

@@ -36,7 +36,6 @@ using pc::inBox;
 using pc::inNLGB;
 using pc::LANES;
 using pc::NW;
-using pc::SEG_ROWS;
 using pc::TILE_BYTES_MAX;
 
 // Per (dim, order): whether the deposit is accumulated in registers over a union window, the
@@ -635,7 +634,8 @@ __device__ __forceinline__ void pushPiece(PushArgs const& A, PView const& in, do
     flushTile<D, RN, pc::SEG_BLOCK>(G, T, A.tile.nodes, A.mom, A.charge);
 }
 
-// one workgroup per work item (a segment and its share of the segment's rows, at most SEG_ROWS)
+// one workgroup per work item (a segment and its share of the segment's rows, at most
+// sg.rowsPerItem)
 template<int D, int O, bool RN>
 __global__ __launch_bounds__(pc::SEG_BLOCK) __attribute__((amdgpu_waves_per_eu(PHX_SEG_WAVES)))
 void k_push_seg(PushArgs A, PView in)
@@ -651,7 +651,7 @@ void k_push_seg(PushArgs A, PView in)
         unsigned int const seg = A.st.items[2 * item];
         unsigned int const R   = A.st.rows[seg];
         unsigned int r0, rEnd;
-        pc::itemRows(A.st.items[2 * item + 1], R, r0, rEnd);
+        pc::itemRows(A.st.items[2 * item + 1], R, static_cast<unsigned int>(A.sg.rowsPerItem), r0, rEnd);
         pushPiece<D, O, RN>(A, in, lds, seg, r0, rEnd);
     }
     if (A.kt)
@@ -909,7 +909,8 @@ __global__ __launch_bounds__(BLOCK) void k_seg_rows(SegGeom S, unsigned int* __r
     {
         unsigned int const R = (T + LANES - 1) / LANES;
         rows[s]              = R;
-        itemCount[s]         = (R + SEG_ROWS - 1) / SEG_ROWS;
+        unsigned int const k = static_cast<unsigned int>(S.rowsPerItem);
+        itemCount[s]         = (R + k - 1) / k;
         total[s]             = T;
     }
 }
@@ -965,12 +966,13 @@ __global__ __launch_bounds__(1024) void k_seg_scan(unsigned int const* __restric
 
 __global__ __launch_bounds__(BLOCK) void k_seg_items(unsigned int const* __restrict__ rows,
                                                      unsigned int const* __restrict__ itemOff,
-                                                     unsigned int* __restrict__ items, unsigned int m)
+                                                     unsigned int* __restrict__ items, unsigned int m,
+                                                     unsigned int rowsPerItem)
 {
     unsigned int const s = blockIdx.x * BLOCK + threadIdx.x;
     if (s >= m)
         return;
-    unsigned int const k = (rows[s] + SEG_ROWS - 1) / SEG_ROWS;
+    unsigned int const k = (rows[s] + rowsPerItem - 1) / rowsPerItem;
     for (unsigned int j = 0; j < k; ++j)
     {
         items[2 * (itemOff[s] + j)]     = s;
@@ -1111,7 +1113,8 @@ static void segScatterT(SegGeom const& sg, V const& src, V const& dst, unsigned 
     unsigned int* first     = work; // the counts, turned into first positions by k_seg_rows
     unsigned int* itemCount = work + nc;
     unsigned int* total     = work + nc + m;
-    k_seg_items<<<(m + BLOCK - 1) / BLOCK, BLOCK, 0, s>>>(tables.rows, itemCount, tables.items, m);
+    k_seg_items<<<(m + BLOCK - 1) / BLOCK, BLOCK, 0, s>>>(tables.rows, itemCount, tables.items, m,
+                                                         static_cast<unsigned int>(sg.rowsPerItem));
     if (n)
         k_seg_perm<V><<<static_cast<unsigned int>((n + BLOCK - 1) / BLOCK), BLOCK, 0, s>>>(
             sg, src, n, rank, first, tables.rows, tables.off, perm);
@@ -1157,14 +1160,12 @@ void launchSegCopy(PView32 const& src, PView32 const& dst, unsigned long long n,
                                                                                                 n, dim);
 }
 
-SegPatch segPatchOf(int dim, int const* lo, int const* hi)
+SegPatch segPatchOf(int dim, int const* lo, int const* hi, int const* pref)
 {
-    // preferred segment shapes: a run of 64 cells in 1D; a 2x32 block in 2D (32 cells along the
-    // fastest dimension, so that a row's gather loads stay in runs of consecutive nodes); a
-    // 2x4x8 block in 3D. Compact blocks keep the LDS tile with its margins small (a 2x32 tile is
-    // 9 x 39 nodes in 2D at order 2 against 8 x 71 for a 1x64 run: fewer nodes to flush).
-    // Clipped to the box, the cells left over go to the slower dimensions.
-    static int const pref[3][3] = {{64, 1, 1}, {1, 32, 1}, {2, 4, 8}};
+    // the preferred segment shape `pref` (per dimension; by default a run of 64 cells in 1D, a
+    // 2x32 block in 2D, 32 cells along the fastest dimension so that a row's gather loads stay
+    // in runs of consecutive nodes, and a 2x4x8 block in 3D: compact blocks keep the LDS tile with
+    // its margins small), clipped to the box; the cells left over go to the slower dimensions
     SegPatch P{};
     int room = LANES;
     for (int d = 0; d < 3; ++d)
@@ -1175,7 +1176,7 @@ SegPatch segPatchOf(int dim, int const* lo, int const* hi)
     }
     for (int d = dim - 1; d >= 0; --d)
     {
-        int const want = d == 0 ? room : std::min(pref[dim - 1][d], room);
+        int const want = d == 0 ? room : std::min(std::max(pref[d], 1), room);
         P.seg[d]       = std::max(1, std::min(want, P.n[d]));
         room /= P.seg[d];
     }
@@ -1195,7 +1196,7 @@ TileSpec segTileShape(int dim, int order, SegGeom const& sg, int copies)
         T.lo[d]  = d < dim ? mlo : 0;
         T.ext[d] = d < dim ? sg.maxSeg[d] + mlo + mhi : 1;
     }
-    pc::tilePitch(T, dim);
+    pc::tilePitch(T, dim, sg.rowPad);
     T.nodes  = T.ext[0] * T.pitch[1] * T.pitch[2];
     T.stride = (T.nodes + 15) / 16 * 16; // copies 16 words apart: neighbouring lanes' banks differ
     T.bytes  = static_cast<unsigned int>(8 * 5 * copies * T.stride);
@@ -1218,7 +1219,8 @@ TileSpec segTileShape(int dim, int order, SegGeom const& sg, int copies)
 
 unsigned int segItemsCap(SegGeom const& sg, unsigned long long capacity)
 {
-    return static_cast<unsigned int>(capacity / (LANES * SEG_ROWS)) + static_cast<unsigned int>(sg.nsegs)
+    return static_cast<unsigned int>(capacity / (LANES * static_cast<unsigned long long>(sg.rowsPerItem)))
+           + static_cast<unsigned int>(sg.nsegs)
            + 2u;
 }
 

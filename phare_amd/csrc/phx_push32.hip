@@ -31,7 +31,6 @@ using pc::inBox;
 using pc::inNLGB;
 using pc::LANES;
 using pc::NW;
-using pc::SEG_ROWS;
 
 // B-spline weights at u = np - start (u in [0, O+1)); the products of interpolator.hpp:54-125
 // written in u: order 2 uses 1 - u for (start + 1 - np), order 3 uses -u for (start - np)
@@ -333,6 +332,141 @@ __device__ __forceinline__ void deposit32(Dep32<D, O> const& dd, Geom const& G, 
     }
 }
 
+// register window of one lane (WIN variant of the mixed-precision push): the n, Fx, Fy, Fz
+// contributions of the lane's successive particles of one cell, summed in FP32 over the union of
+// their stencils (nodes cell - WB .. cell - WB + W - 1 per dimension), flushed into the FP64 LDS
+// tile (or HBM) when the cell changes and at the end. A lane walks a run of cell-ordered
+// particles, so one flush replaces the LDS adds of several particles.
+template<int D, int O>
+struct Win32
+{
+    static constexpr int W  = O == 1 ? 2 : 4;
+    static constexpr int WB = O == 1 ? 0 : 1;
+    static constexpr int WN = D == 1 ? W : (D == 2 ? W * W : W * W * W);
+    float a[4][WN];
+    int base[D]; // local node index of the window origin
+    bool live;
+};
+
+template<int D, int O>
+__device__ __forceinline__ void winZero(Win32<D, O>& A)
+{
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int x = 0; x < Win32<D, O>::WN; ++x)
+            A.a[m][x] = 0.f;
+    A.live = false;
+}
+
+template<int D, int O>
+__device__ __forceinline__ void winFlush(Win32<D, O> const& A, Geom const& G, Tile32 const& T,
+                                         double* const* mom, double charge)
+{
+    constexpr int W = Win32<D, O>::W;
+    if (!A.live)
+        return;
+#pragma unroll
+    for (int x = 0; x < Win32<D, O>::WN; ++x)
+    {
+        int const a  = D == 1 ? x : (D == 2 ? x / W : x / (W * W));
+        int const b  = D == 1 ? 0 : (D == 2 ? x % W : (x / W) % W);
+        int const c  = D == 3 ? x % W : 0;
+        int const g[3] = {A.base[0] + a, D > 1 ? A.base[1] + b : 0, D > 2 ? A.base[2] + c : 0};
+        bool inTile    = true;
+#pragma unroll
+        for (int d = 0; d < D; ++d)
+            inTile = inTile && g[d] >= T.org[d] && g[d] < T.org[d] + T.ext[d];
+        if (inTile)
+        {
+            int const tnode = T.node(g[0] - T.org[0], D > 1 ? g[1] - T.org[1] : 0,
+                                     D > 2 ? g[2] - T.org[2] : 0);
+#pragma unroll
+            for (int m = 0; m < 4; ++m)
+                if (A.a[m][x] != 0.f)
+                    atomicAdd(T.mom(m) + tnode, static_cast<double>(A.a[m][x]));
+        }
+        else
+        {
+            int const node = G.idx(QMOM, g[0], g[1], g[2]);
+            if (A.a[0][x] != 0.f)
+            {
+                atomicAdd(mom[0] + node, static_cast<double>(A.a[0][x]));
+                atomicAdd(mom[1] + node, charge * static_cast<double>(A.a[0][x]));
+            }
+#pragma unroll
+            for (int m = 1; m < 4; ++m)
+                if (A.a[m][x] != 0.f)
+                    atomicAdd(mom[m + 1] + node, static_cast<double>(A.a[m][x]));
+        }
+    }
+}
+
+// one particle's deposit into the lane's window (re-based, after a flush, when its cell differs
+// from the window's); the node products in nodeW32's order
+template<int D, int O>
+__device__ __forceinline__ void winAdd(Win32<D, O>& A, Dep32<D, O> const& dd, int const* icl,
+                                       Geom const& G, Tile32 const& T, double* const* mom,
+                                       double charge)
+{
+    constexpr int W = Win32<D, O>::W;
+    constexpr int S = O + 1;
+    bool same       = A.live;
+#pragma unroll
+    for (int d = 0; d < D; ++d)
+        same = same && A.base[d] == icl[d] - Win32<D, O>::WB;
+    if (!same)
+    {
+        winFlush<D, O>(A, G, T, mom, charge);
+        winZero(A);
+#pragma unroll
+        for (int d = 0; d < D; ++d)
+            A.base[d] = icl[d] - Win32<D, O>::WB;
+        A.live = true;
+    }
+    float w[D][W];
+#pragma unroll
+    for (int d = 0; d < D; ++d)
+    {
+        int const off = dd.start[d] - A.base[d];
+#pragma unroll
+        for (int a = 0; a < W; ++a)
+        {
+            float x = 0.f;
+#pragma unroll
+            for (int e = 0; e < S; ++e)
+                x = (a - off == e) ? dd.w[d][e] : x;
+            w[d][a] = x;
+        }
+    }
+#pragma unroll
+    for (int x = 0; x < Win32<D, O>::WN; ++x)
+    {
+        int const a = D == 1 ? x : (D == 2 ? x / W : x / (W * W));
+        int const b = D == 1 ? 0 : (D == 2 ? x % W : (x / W) % W);
+        int const c = D == 3 ? x % W : 0;
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+        {
+            float v;
+            if constexpr (D == 1)
+                v = dd.val[m] * w[0][a];
+            else if constexpr (D == 2)
+                v = dd.val[m] * w[0][a] * w[1][b];
+            else
+                v = dd.val[m] * w[0][a] * w[1][b] * w[2][c];
+            A.a[m][x] += v;
+        }
+    }
+}
+
+// the window variant exists where its accumulators fit the registers: 1D, 2D, and 3D at order 1
+template<int D, int O>
+constexpr bool winFits()
+{
+    return D < 3 || O == 1;
+}
+
 // per-launch constants in FP32
 struct Push32
 {
@@ -343,9 +477,9 @@ struct Push32
 
 // one slot of segmented storage: prePush, gather, Boris, postPush, selection, deposit and the
 // in-place write-back (the FP64 family's pushLoaded in FP32)
-template<int D, int O>
+template<int D, int O, bool WIN>
 __device__ __forceinline__ void pushSlot32(PushArgs const& A, Push32 const& K, PView32 const& v,
-                                           unsigned long long i, Tile32 const& T)
+                                           unsigned long long i, Tile32 const& T, Win32<D, O>& win)
 {
     Geom const& G = A.G;
     int cell[D];
@@ -447,7 +581,16 @@ __device__ __forceinline__ void pushSlot32(PushArgs const& A, Push32 const& K, P
             dd.val[1]      = vel[0] * wc;
             dd.val[2]      = vel[1] * wc;
             dd.val[3]      = vel[2] * wc;
-            deposit32<D, O>(dd, G, T, A.mom, A.charge);
+            if constexpr (WIN)
+            {
+                int icl[D];
+#pragma unroll
+                for (int d = 0; d < D; ++d)
+                    icl[d] = cell[d] - (G.lower[d] - G.g);
+                winAdd<D, O>(win, dd, icl, G, T, A.mom, A.charge);
+            }
+            else
+                deposit32<D, O>(dd, G, T, A.mom, A.charge);
         }
     }
 
@@ -496,11 +639,14 @@ __device__ __forceinline__ void pushSlot32(PushArgs const& A, Push32 const& K, P
 #ifndef PHX_SEG_WAVES32
 #define PHX_SEG_WAVES32 4 // waves per SIMD the register allocation of the mixed-precision push targets
 #endif
+#ifndef PHX_SEG_WAVES32_WIN
+#define PHX_SEG_WAVES32_WIN 3 // the same for the register-window variant (its window needs room)
+#endif
 namespace
 {
 // rows [rA, rB) of segment seg through the FP64 LDS tile: zero the tile, push the rows (wavefront
 // w takes rows rA + w, rA + w + NW, ...), flush the tile. Every work-item of the workgroup calls it.
-template<int D, int O>
+template<int D, int O, bool WIN>
 __device__ __forceinline__ void pushPiece32(PushArgs const& A, PView32 const& v, double* lds32,
                                             unsigned int seg, unsigned int rA, unsigned int rB)
 {
@@ -531,8 +677,13 @@ __device__ __forceinline__ void pushPiece32(PushArgs const& A, PView32 const& v,
     K.coef  = static_cast<float>(A.coef);
     __syncthreads();
 
+    Win32<D, O> win;
+    if constexpr (WIN)
+        winZero(win);
     for (unsigned int r = rA + (t >> 6); r < rB; r += pc::SEG_NW)
-        pushSlot32<D, O>(A, K, v, off + static_cast<unsigned long long>(r) * LANES + lane, T);
+        pushSlot32<D, O, WIN>(A, K, v, off + static_cast<unsigned long long>(r) * LANES + lane, T, win);
+    if constexpr (WIN)
+        winFlush<D, O>(win, G, T, A.mom, A.charge);
     __syncthreads();
 
     // flush: one HBM atomic per touched node and moment
@@ -568,9 +719,11 @@ __device__ __forceinline__ void pushPiece32(PushArgs const& A, PView32 const& v,
 }
 } // namespace
 
-// one workgroup per work item (a segment and its share of the segment's rows, at most SEG_ROWS)
-template<int D, int O>
-__global__ __launch_bounds__(pc::SEG_BLOCK) __attribute__((amdgpu_waves_per_eu(PHX_SEG_WAVES32)))
+// one workgroup per work item (a segment and its share of the segment's rows, at most
+// sg.rowsPerItem)
+template<int D, int O, bool WIN>
+__global__ __launch_bounds__(pc::SEG_BLOCK)
+__attribute__((amdgpu_waves_per_eu(WIN ? PHX_SEG_WAVES32_WIN : PHX_SEG_WAVES32)))
 void k_push_seg32(PushArgs A, PView32 v)
 {
     extern __shared__ double lds32[];
@@ -584,8 +737,8 @@ void k_push_seg32(PushArgs A, PView32 v)
         unsigned int const seg = A.st.items[2 * item];
         unsigned int const R   = A.st.rows[seg];
         unsigned int r0, rEnd;
-        pc::itemRows(A.st.items[2 * item + 1], R, r0, rEnd);
-        pushPiece32<D, O>(A, v, lds32, seg, r0, rEnd);
+        pc::itemRows(A.st.items[2 * item + 1], R, static_cast<unsigned int>(A.sg.rowsPerItem), r0, rEnd);
+        pushPiece32<D, O, WIN>(A, v, lds32, seg, r0, rEnd);
     }
     if (A.kt)
     {
@@ -597,7 +750,13 @@ void k_push_seg32(PushArgs A, PView32 v)
 template<int D, int O>
 static void launchPush32T(PushArgs const& A, PView32 const& v, hipStream_t s)
 {
-    k_push_seg32<D, O><<<A.st.itemsCap, pc::SEG_BLOCK, A.tile.bytes, s>>>(A, v);
+    if constexpr (winFits<D, O>())
+        if (A.depositWindow)
+        {
+            k_push_seg32<D, O, true><<<A.st.itemsCap, pc::SEG_BLOCK, A.tile.bytes, s>>>(A, v);
+            return;
+        }
+    k_push_seg32<D, O, false><<<A.st.itemsCap, pc::SEG_BLOCK, A.tile.bytes, s>>>(A, v);
 }
 
 void launchPush32(PushArgs const& A, PView32 const& v, hipStream_t s)
@@ -627,7 +786,7 @@ TileSpec segTileShape32(int dim, int order, SegGeom const& sg)
         T.lo[d]  = d < dim ? mlo : 0;
         T.ext[d] = d < dim ? sg.maxSeg[d] + mlo + mhi : 1;
     }
-    pc::tilePitch(T, dim);
+    pc::tilePitch(T, dim, sg.rowPad);
     T.nodes  = T.ext[0] * T.pitch[1] * T.pitch[2];
     T.stride = (T.nodes + 15) / 16 * 16;
     T.bytes  = static_cast<unsigned int>(8 * 4 * T.stride);

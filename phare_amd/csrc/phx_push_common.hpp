@@ -5,7 +5,7 @@
 #include "phx_kernels.hpp"
 
 #ifndef PHX_SEG_ROWS
-#define PHX_SEG_ROWS 32 // most segment rows per workgroup of the segmented push
+#define PHX_SEG_ROWS 32 // most segment rows per workgroup of the segmented push (default)
 #endif
 #ifndef PHX_TILE_SPREAD
 #define PHX_TILE_SPREAD 1
@@ -26,7 +26,6 @@ constexpr int NW       = BLOCK / 64; // wavefronts per workgroup
 constexpr int SEG_BLOCK = PHX_SEG_BLOCK;    // work-items per workgroup of the segmented pushes
 constexpr int SEG_NW    = SEG_BLOCK / 64;
 constexpr int LANES    = 64;
-constexpr int SEG_ROWS = PHX_SEG_ROWS;
 constexpr unsigned int TILE_BYTES_MAX = 65536;
 constexpr int TILE_SPREAD = PHX_TILE_SPREAD; // cells of drift since the sort in a segment tile
 
@@ -70,12 +69,12 @@ __device__ __forceinline__ void ktEnd(KTime* kt, unsigned int working)
     atomicExch(&kt->done, 0u);
 }
 
-// rows [r0, r1) of work item j of a segment of R rows: the segment's ceil(R / SEG_ROWS) items
-// share its rows evenly (at most SEG_ROWS each)
-__device__ __forceinline__ void itemRows(unsigned int j, unsigned int R, unsigned int& r0,
-                                         unsigned int& r1)
+// rows [r0, r1) of work item j of a segment of R rows: the segment's ceil(R / rowsPerItem)
+// items share its rows evenly (at most rowsPerItem each)
+__device__ __forceinline__ void itemRows(unsigned int j, unsigned int R, unsigned int rowsPerItem,
+                                         unsigned int& r0, unsigned int& r1)
 {
-    unsigned int const k = (R + SEG_ROWS - 1) / SEG_ROWS;
+    unsigned int const k = (R + rowsPerItem - 1) / rowsPerItem;
     r0 = static_cast<unsigned int>(static_cast<unsigned long long>(j) * R / k);
     r1 = static_cast<unsigned int>(static_cast<unsigned long long>(j + 1) * R / k);
 }
@@ -116,14 +115,15 @@ __device__ __forceinline__ void segTileOrigin(Geom const& G, SegGeom const& S, T
 #define PHX_TILE_ROW_PAD 16
 #endif
 // node pitches of a segment tile: in 2D the rows (fastest dimension) are padded to a multiple of
-// PHX_TILE_ROW_PAD nodes, so that rows of the tile start on the same LDS bank and lanes of one
-// wavefront whose stencils start on neighbouring rows do not collide in banks; 1D/3D unpadded
-inline void tilePitch(TileSpec& T, int dim)
+// `pad` nodes (PHX_TILE_ROW_PAD unless tuned), so that rows of the tile start on the same LDS
+// bank and lanes of one wavefront whose stencils start on neighbouring rows do not collide in
+// banks; 1D/3D unpadded
+inline void tilePitch(TileSpec& T, int dim, int pad)
 {
     for (int d = 0; d < 3; ++d)
         T.pitch[d] = T.ext[d];
-    if (dim == 2 && PHX_TILE_ROW_PAD > 1)
-        T.pitch[1] = (T.ext[1] + PHX_TILE_ROW_PAD - 1) / PHX_TILE_ROW_PAD * PHX_TILE_ROW_PAD;
+    if (dim == 2 && pad > 1)
+        T.pitch[1] = (T.ext[1] + pad - 1) / pad * pad;
 }
 
 inline int segMarginLo(int order) { return (order >= 2 ? 1 : 0) + 1 + TILE_SPREAD; }

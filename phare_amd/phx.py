@@ -43,6 +43,12 @@ class Patch(C.Structure):
                 ("nonlevel_ghost_upper", (C.c_int * 3) * 27)]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("segment_shape", C.c_int * 3), ("rows_per_item", C.c_int),
+                ("tile_row_pad", C.c_int), ("sort_interval", C.c_int), ("tile_copies", C.c_int),
+                ("deposit_window", C.c_int)]
+
+
 _lib = None
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
@@ -104,6 +110,8 @@ def lib():
         L.phx_gather.argtypes = [vp, C.c_int, C.c_int, C.c_int, _dp, _dp]
         L.phx_gather_noalloc.argtypes = [vp, C.c_int, C.c_int, C.c_int, _dp, _dp]
         L.phx_deposit.argtypes = [vp, C.c_int, C.c_int, C.c_double]
+        L.phx_get_tuning.argtypes = [vp, C.POINTER(Tuning)]
+        L.phx_set_tuning.argtypes = [vp, C.POINTER(Tuning)]
         _lib = L
     return _lib
 
@@ -119,7 +127,8 @@ EXPORTED = ["phx_create", "phx_destroy", "phx_last_error", "phx_synchronize", "p
             "phx_set_push_timing", "phx_push_timing", "phx_push", "phx_gather", "phx_deposit",
             "phx_create_level", "phx_patch_box", "phx_patch_nonlevel_ghost_boxes",
             "phx_patch_field_shape", "phx_set_patch_field", "phx_get_patch_field",
-            "phx_get_patch_particles", "phx_add_particles", "phx_gather_noalloc"]
+            "phx_get_patch_particles", "phx_add_particles", "phx_gather_noalloc",
+            "phx_get_tuning", "phx_set_tuning"]
 
 
 class Level:
@@ -127,9 +136,11 @@ class Level:
 
     def __init__(self, dim, order, ncells, dx, lower=None, masses=(1.0,), Te=0.1, eta=0.0, nu=1e-4,
                  hyper_mode=0, level=0, periodic=True, capacity=0, precision=PRECISION_FP64,
-                 patches=None):
+                 patches=None, apply_tuning_table=True):
         """patches: None (one patch covering the box) or a list of (lower, upper) AMR boxes of
-        the level's patches (phx_create_level)"""
+        the level's patches (phx_create_level). apply_tuning_table: before the first
+        initialisation, apply the autotuner's entry for this dimension, order, precision family
+        and particles per cell (phare_amd/tuning_table.json), if there is one"""
         L = lib()
         self.dim, self.order = dim, order
         lower = list(lower or [0] * dim)
@@ -161,6 +172,9 @@ class Level:
             raise PhxError(rc, "phx_create failed")
         self.h = h
         self.npop = len(masses)
+        self.ncells = list(ncells)
+        self.precision = precision
+        self._tuning_pending = apply_tuning_table
 
     def close(self):
         if getattr(self, "h", None):
@@ -235,6 +249,15 @@ class Level:
         self._rc(lib().phx_load_maxwellian(self.h, pop, *arrs, charge, ppc, seed, cutoff))
 
     def initialize(self):
+        if self._tuning_pending:
+            self._tuning_pending = False
+            from phare_amd import autotune
+
+            ppc = self.count(0, DOMAIN) / max(1, int(np.prod(self.ncells)))
+            t = autotune.lookup(self.dim, self.order,
+                                "fp32" if self.precision == PRECISION_FP32 else "fp64", ppc)
+            if t is not None:
+                self.set_tuning(**t)
         self._rc(lib().phx_initialize(self.h))
 
     def update_populations(self, dt, mode, emset=EM_AVG):
@@ -375,6 +398,21 @@ class Level:
         E, B = np.zeros((n, 3)), np.zeros((n, 3))
         self._rc(lib().phx_gather_noalloc(self.h, emset, pop, kind, E.reshape(-1), B.reshape(-1)))
         return E, B
+
+    def get_tuning(self):
+        t = Tuning()
+        self._rc(lib().phx_get_tuning(self.h, C.byref(t)))
+        return dict(segment_shape=list(t.segment_shape), rows_per_item=t.rows_per_item,
+                    tile_row_pad=t.tile_row_pad, sort_interval=t.sort_interval,
+                    tile_copies=t.tile_copies, deposit_window=t.deposit_window)
+
+    def set_tuning(self, **kw):
+        """phx_set_tuning with the given fields changed from the current tuning"""
+        cur = self.get_tuning()
+        cur.update(kw)
+        t = Tuning((C.c_int * 3)(*cur["segment_shape"]), cur["rows_per_item"], cur["tile_row_pad"],
+                   cur["sort_interval"], cur["tile_copies"], cur["deposit_window"])
+        self._rc(lib().phx_set_tuning(self.h, C.byref(t)))
 
     def deposit(self, pop=0, kind=DOMAIN, coef=1.0):
         self._rc(lib().phx_deposit(self.h, pop, kind, coef))

@@ -316,10 +316,11 @@ def test_advance_mixed_charges_2d():
 
 
 @pytest.mark.skip(reason="one GPU run faulted the GPU in the 3D order-1 case of this test (first 3D "
-                         "advance on the fused periodic path); the cause is not shown (CPU emulations "
-                         "of the library with poison-filled allocations and AddressSanitizer, of this "
-                         "tree and of the tree that faulted, are clean), so the fused 3D path is "
-                         "switched off (phx_advance / phx_move_ions return PHX_ERR_STATE in 3D)")
+                         "advance on the captured fused periodic path); the cause is not shown (CPU "
+                         "emulations of the library with poison-filled allocations, deferred streams, "
+                         "AddressSanitizer and UndefinedBehaviorSanitizer are clean), so the fused 3D "
+                         "path stays off; 3D levels advance on the staged path, tested by "
+                         "test_advance_uniform3d_staged")
 @pytest.mark.parametrize("order", [1, 2, 3])
 def test_advance_uniform3d_small(order):
     _say(f"[gpu] advance uniform3d small order={order}")
@@ -329,13 +330,26 @@ def test_advance_uniform3d_small(order):
     g.close()
 
 
-def test_fused_3d_path_is_switched_off():
-    """the fused 3D periodic path stays off the GPU: phx_advance refuses a periodic 3D level"""
-    _say("[gpu] fused 3D path refused")
-    g = phx.Level(3, 1, [9, 10, 11], [0.2] * 3)
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_advance_uniform3d_staged(order):
+    """a 3D periodic level advances on the staged path (the drop-in entry points' kernels: linear
+    storage push with the in-place periodic re-entry, Faraday/Ampere/Ohm followed by the periodic
+    ghost fill, border sum and max of the moments): 6 PPC steps against the oracle at 1e-12"""
+    _say(f"[gpu] advance uniform3d staged order={order}")
+    wl = W.uniform3d(ncells=(11, 10, 12), ppc=5, order=order)
+    g, o = _run_pair(wl, 6)
+    _check_state(g, o, wl)
+    g.close()
+
+
+def test_mixed_family_has_no_3d_path():
+    """the mixed-precision family's push runs on the captured fused step only, which is off in 3D:
+    a 3D mixed-precision level refuses to initialise"""
+    _say("[gpu] mixed family refuses 3D")
+    g = phx.Level(3, 1, [9, 10, 11], [0.2] * 3, precision=phx.PRECISION_FP32)
     with pytest.raises(phx.PhxError) as e:
-        g.advance(0.01, 1)
-    assert e.value.status == 6 and "disabled" in str(e.value)
+        g.initialize()
+    assert e.value.status == 6 and "3D" in str(e.value)
     g.close()
 
 
