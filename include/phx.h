@@ -52,8 +52,11 @@ enum { PHX_MODE_DOMAIN_ONLY = 1, PHX_MODE_ALL = 2 };
 /* HyperMode (src/core/numerics/ohm/ohm.hpp:15) */
 enum { PHX_HYPER_CONSTANT = 0, PHX_HYPER_SPATIAL = 1 };
 
-/* particle arrays of a population: ParticlesPack (src/core/data/ions/particle_pack.hpp:20-114) */
-enum { PHX_DOMAIN = 0, PHX_PATCH_GHOST = 1, PHX_LEVEL_GHOST = 2 };
+/* particle arrays of a population: ParticlesPack (src/core/data/ions/particle_pack.hpp:20-114):
+   domain, patch ghost, level ghost, and the level ghost particles split from the coarser level
+   at the start (old) and the end (new) of its step (IonPopulation::levelGhostParticlesOld/New) */
+enum { PHX_DOMAIN = 0, PHX_PATCH_GHOST = 1, PHX_LEVEL_GHOST = 2, PHX_LEVEL_GHOST_OLD = 3,
+       PHX_LEVEL_GHOST_NEW = 4 };
 
 /* electromagnetic field sets of SolverPPC (solver_ppc.hpp:52-53): state, predicted, averaged */
 enum { PHX_EM = 0, PHX_EM_PRED = 1, PHX_EM_AVG = 2 };
@@ -208,6 +211,30 @@ phx_status phx_gather_noalloc(phx_ctx* ctx, int em_set, int pop, int kind, doubl
    weight coef (the moments are not reset; phx_update_populations resets them) */
 phx_status phx_deposit(phx_ctx* ctx, int pop, int kind, double coef);
 
+/* ---- coarse-to-fine coupling of two levels (both contexts on the same GPU) ------------------ */
+/* ParticlesRefineOperator::refine_ (src/amr/data/particles/refine/particles_data_split.hpp:157-260)
+   with the Splitter of (dim, interp order, refined_particle_nbr)
+   (src/amr/data/particles/refine/split_{1,2,3}d.hpp; 0 = PHARE's default: 2 in 1D, 4 in 2D, 6 in
+   3D; every 1D variant is available): the domain particles of population `pop` of the coarse level
+   are put on the fine grid (refinement ratio 2), those whose fine cell lies within
+   ceil((order + 1) / 2) cells of a destination box are split, and the refined particles inside a
+   destination box are appended to array dest_kind of the fine level (PHX_DOMAIN: interior
+   refinement; PHX_LEVEL_GHOST / _OLD / _NEW: the coarse-to-fine boundary). box_lower/box_upper:
+   n_boxes x 3 fine AMR cells, inclusive, disjoint (at most 32). FP64 levels only. */
+phx_status phx_refine_particles(phx_ctx* fine, const phx_ctx* coarse, int pop, int dest_kind,
+                                int n_boxes, const int* box_lower, const int* box_upper,
+                                int refined_particle_nbr);
+/* HybridHybridMessengerStrategy::lastStep (hybrid_hybrid_messenger_strategy.hpp:610-633): per
+   population, levelGhostNew and levelGhostOld swap, levelGhostNew is emptied and levelGhost
+   becomes a copy of levelGhostOld. The moments of fillIonPopMomentGhosts (:507-548) are
+   phx_deposit of PHX_LEVEL_GHOST_OLD with coef 1 - alpha and of PHX_LEVEL_GHOST_NEW with alpha. */
+phx_status phx_level_ghost_last_step(phx_ctx* ctx);
+/* linear_time_interpolate (src/amr/data/field/time_interpolate/field_linear_time_interpolate.hpp:
+   20-32): dst = (1 - alpha) old + alpha new on the nodes lower..upper (inclusive, local node
+   indices of the field arrays, which must have one shape); the contexts may be the same one */
+phx_status phx_time_interpolate(phx_ctx* dst, int dst_field, const phx_ctx* old_ctx, int old_field,
+                                const phx_ctx* new_ctx, int new_field, const int lower[3],
+                                const int upper[3], double alpha);
 /* IonUpdater::updateIons (ion_updater.hpp:111-116) */
 phx_status phx_update_ions(phx_ctx* ctx);
 /* same-level fills of HybridHybridMessengerStrategy (hybrid_hybrid_messenger_strategy.hpp):

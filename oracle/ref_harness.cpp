@@ -18,6 +18,7 @@
 #include "core/numerics/faraday/faraday.hpp"
 #include "core/numerics/interpolator/interpolator.hpp"
 #include "core/utilities/types.hpp"
+#include "amr/data/particles/refine/split.hpp"
 
 // defined in the reference's maxwellian_particle_initializer.cpp, compiled beside this harness
 // (its header includes the un-vendored cppdict, so the declaration is restated here)
@@ -468,6 +469,86 @@ long ref_partition(int dim, int const* mapLo, int const* mapHi, long np, int con
         return partitionF<2>(mapLo, mapHi, np, ic0, ic1, predLo, predHi, order);
     if (dim == 3)
         return partitionF<3>(mapLo, mapHi, np, ic0, ic1, predLo, predHi, order);
+    return -1;
+}
+
+// the reference's Splitter<dim, order, nbr> applied to one particle already on the fine grid
+// (the input of PatternDispatcher::operator()): its nbr refined particles, in dispatch order
+long ref_split_one(int dim, int order, int nbr, int const* ic, double const* de, double const* v,
+                   double w, double q, int* oic, double* ode, double* ov, double* ow, double* oq)
+{
+    long r = -1;
+    auto run = [&](auto dimc, auto splitter) {
+        constexpr std::size_t D = decltype(dimc)::value;
+        constexpr auto N        = decltype(splitter)::nbRefinedPart;
+        Particle<D> p{};
+        p.weight = w;
+        p.charge = q;
+        for (std::size_t d = 0; d < D; ++d)
+        {
+            p.iCell[d] = ic[d];
+            p.delta[d] = de[d];
+        }
+        for (std::size_t d = 0; d < 3; ++d)
+            p.v[d] = v[d];
+        std::array<Particle<D>, N> out{};
+        splitter(p, out);
+        for (std::size_t k = 0; k < N; ++k)
+        {
+            for (std::size_t d = 0; d < D; ++d)
+            {
+                oic[k * D + d] = out[k].iCell[d];
+                ode[k * D + d] = out[k].delta[d];
+            }
+            for (std::size_t d = 0; d < 3; ++d)
+                ov[k * 3 + d] = out[k].v[d];
+            ow[k] = out[k].weight;
+            oq[k] = out[k].charge;
+        }
+        r = static_cast<long>(N);
+    };
+    using PHARE::amr::Splitter;
+    using PHARE::core::DimConst;
+    using PHARE::core::InterpConst;
+    using PHARE::core::RefinedParticlesConst;
+    using D1 = std::integral_constant<std::size_t, 1>;
+    using D2 = std::integral_constant<std::size_t, 2>;
+    using D3 = std::integral_constant<std::size_t, 3>;
+    switch (dim * 100 + order * 10 + nbr)
+    {
+        case 112: run(D1{}, Splitter<DimConst<1>, InterpConst<1>, RefinedParticlesConst<2>>{}); break;
+        case 113: run(D1{}, Splitter<DimConst<1>, InterpConst<1>, RefinedParticlesConst<3>>{}); break;
+        case 122: run(D1{}, Splitter<DimConst<1>, InterpConst<2>, RefinedParticlesConst<2>>{}); break;
+        case 123: run(D1{}, Splitter<DimConst<1>, InterpConst<2>, RefinedParticlesConst<3>>{}); break;
+        case 124: run(D1{}, Splitter<DimConst<1>, InterpConst<2>, RefinedParticlesConst<4>>{}); break;
+        case 132: run(D1{}, Splitter<DimConst<1>, InterpConst<3>, RefinedParticlesConst<2>>{}); break;
+        case 133: run(D1{}, Splitter<DimConst<1>, InterpConst<3>, RefinedParticlesConst<3>>{}); break;
+        case 134: run(D1{}, Splitter<DimConst<1>, InterpConst<3>, RefinedParticlesConst<4>>{}); break;
+        case 135: run(D1{}, Splitter<DimConst<1>, InterpConst<3>, RefinedParticlesConst<5>>{}); break;
+        case 214: run(D2{}, Splitter<DimConst<2>, InterpConst<1>, RefinedParticlesConst<4>>{}); break;
+        case 224: run(D2{}, Splitter<DimConst<2>, InterpConst<2>, RefinedParticlesConst<4>>{}); break;
+        case 234: run(D2{}, Splitter<DimConst<2>, InterpConst<3>, RefinedParticlesConst<4>>{}); break;
+        case 316: run(D3{}, Splitter<DimConst<3>, InterpConst<1>, RefinedParticlesConst<6>>{}); break;
+        case 326: run(D3{}, Splitter<DimConst<3>, InterpConst<2>, RefinedParticlesConst<6>>{}); break;
+        case 336: run(D3{}, Splitter<DimConst<3>, InterpConst<3>, RefinedParticlesConst<6>>{}); break;
+        default: break;
+    }
+    return r;
+}
+
+// the reference's Splitter::maxCellDistanceFromSplit of (dim, order) (splitter.hpp:116-120)
+int ref_split_max_cell_distance(int order)
+{
+    using PHARE::amr::ASplitter;
+    using PHARE::core::DimConst;
+    using PHARE::core::InterpConst;
+    using PHARE::core::RefinedParticlesConst;
+    switch (order)
+    {
+        case 1: return ASplitter<DimConst<1>, InterpConst<1>, RefinedParticlesConst<2>>::maxCellDistanceFromSplit();
+        case 2: return ASplitter<DimConst<1>, InterpConst<2>, RefinedParticlesConst<2>>::maxCellDistanceFromSplit();
+        case 3: return ASplitter<DimConst<1>, InterpConst<3>, RefinedParticlesConst<2>>::maxCellDistanceFromSplit();
+    }
     return -1;
 }
 

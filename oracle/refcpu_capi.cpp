@@ -4,6 +4,7 @@
 //
 // Field ids and particle-array kinds use the same numbering as include/phx.h so that tests
 // can address both sides with one id; they are redefined here on purpose (no shared header).
+#include "refcpu_amr.hpp"
 #include "refcpu_solver.hpp"
 
 #include <memory>
@@ -842,4 +843,45 @@ extern "C" int ora_boris_trajectory(int dim, long nsteps, double dt, double cons
     else
         s_trajectory<3>(nsteps, dt, dx, mass, charge, E, B, ic0, de0, v0, out);
     return 0;
+}
+
+// the coarse-to-fine particle split (refcpu_amr.hpp): coarse particles (SoA, n) -> refined
+// particles kept in the destination boxes, written to the out arrays (capacity cap); returns the
+// count of refined particles, or -1 when the out arrays are too small or the splitter unknown
+extern "C" long ora_split(int dim, int order, int nbr, long n, int const* ic, double const* de,
+                          double const* v, double const* w, double const* q, int nb, int const* lo,
+                          int const* hi, long cap, int* oic, double* ode, double* ov, double* ow,
+                          double* oq)
+{
+    if (refcpu::splitTable(dim, order, nbr).w.empty())
+        return -1;
+    std::vector<refcpu::SplitPart> in(static_cast<size_t>(n)), out;
+    for (long i = 0; i < n; ++i)
+    {
+        auto& p = in[static_cast<size_t>(i)];
+        for (int d = 0; d < 3; ++d)
+        {
+            p.ic[d] = d < dim ? ic[i * dim + d] : 0;
+            p.de[d] = d < dim ? de[i * dim + d] : 0.;
+            p.v[d]  = v[i * 3 + d];
+        }
+        p.w = w[i];
+        p.q = q[i];
+    }
+    refcpu::splitParticles(dim, order, nbr, in, nb, lo, hi, out);
+    if (static_cast<long>(out.size()) > cap)
+        return -1;
+    for (size_t k = 0; k < out.size(); ++k)
+    {
+        for (int d = 0; d < dim; ++d)
+        {
+            oic[k * dim + d] = out[k].ic[d];
+            ode[k * dim + d] = out[k].de[d];
+        }
+        for (int d = 0; d < 3; ++d)
+            ov[k * 3 + d] = out[k].v[d];
+        ow[k] = out[k].w;
+        oq[k] = out[k].q;
+    }
+    return static_cast<long>(out.size());
 }

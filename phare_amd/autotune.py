@@ -4,7 +4,7 @@ For one workload (dimension, interpolation order, particles per cell, precision 
 the level on the GPU and searches the tunable shapes by coordinate descent: each parameter in
 turn takes every value of its candidate list while the others keep their best values so far,
 each candidate timed over whole predictor-corrector steps (`phx_advance`, the sorts included at
-their interval), two passes. The winners go into tuning_table.json beside this file, keyed by
+their interval) of a fresh level from the workload's initial state, two passes. The winners go into tuning_table.json beside this file, keyed by
 (dim, order, precision, particles per cell); phx.Level applies the matching entry before its
 first initialisation, and C++ callers pass the same values to phx_set_tuning.
 
@@ -65,13 +65,23 @@ def time_steps(lvl, wl, warmup, steps):
     return 1e3 * (time.perf_counter() - t0) / steps
 
 
-def tune(wl, precision, warmup, steps, passes=2, log=print):
-    prec = phx.PRECISION_FP32 if precision == "fp32" else phx.PRECISION_FP64
+def trial(wl, prec, tuning, warmup, steps):
+    """ms per PPC step of the workload under one tuning, from its initial state (each candidate
+    runs the same physics)"""
     lvl = phx.Level(wl.dim, wl.order, wl.ncells, wl.dx, masses=wl.masses, Te=wl.Te,
                     capacity=wl.n_particles, precision=prec, apply_tuning_table=False)
+    if tuning is not None:
+        lvl.set_tuning(**tuning)
     wl.setup(lvl, phx.F_B)
-    best = lvl.get_tuning()
-    best_ms = time_steps(lvl, wl, warmup, steps)
+    got = lvl.get_tuning()
+    ms = time_steps(lvl, wl, warmup, steps)
+    lvl.close()
+    return got, ms
+
+
+def tune(wl, precision, warmup, steps, passes=2, log=print):
+    prec = phx.PRECISION_FP32 if precision == "fp32" else phx.PRECISION_FP64
+    best, best_ms = trial(wl, prec, None, warmup, steps)
     log(f"[autotune] start {best} -> {best_ms:.4f} ms/step")
     trials = [dict(tuning=dict(best), ms_step=best_ms)]
     space = dict(SPACE[wl.dim])
@@ -84,14 +94,11 @@ def tune(wl, precision, warmup, steps, passes=2, log=print):
                     continue
                 cand = dict(best)
                 cand[key] = v
-                lvl.set_tuning(**cand)
-                ms = time_steps(lvl, wl, warmup, steps)
+                _, ms = trial(wl, prec, cand, warmup, steps)
                 trials.append(dict(tuning=dict(cand), ms_step=ms))
                 log(f"[autotune] {key}={v}: {ms:.4f} ms/step")
                 if ms < best_ms:
                     best, best_ms = cand, ms
-            lvl.set_tuning(**best)
-    lvl.close()
     return best, best_ms, trials
 
 

@@ -18,7 +18,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libphx.so")
 
-SOURCES = ["phx_particles.hip", "phx_push32.hip", "phx_fields.hip", "phx_capi.cpp"]
+SOURCES = ["phx_particles.hip", "phx_push32.hip", "phx_fields.hip", "phx_amr.hip", "phx_capi.cpp"]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 

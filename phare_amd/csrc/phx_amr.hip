@@ -1,0 +1,341 @@
+// phx_amr.hip -- coarse-to-fine kernels of the AMR level coupling for gfx950: the particle split
+// (ParticlesRefineOperator::refine_, src/amr/data/particles/refine/particles_data_split.hpp:157-260,
+// with the split patterns of src/amr/data/particles/refine/split_{1,2,3}d.hpp dispatched as in
+// splitter.hpp:53-104) and the linear time interpolation of fields
+// (src/amr/data/field/time_interpolate/field_linear_time_interpolate.hpp).
+//
+// The split runs one thread per coarse particle: the particle on the fine grid (toFineGrid,
+// particles_data_split.hpp:30-45), the candidates whose fine cell lies in a split box (a
+// destination box grown by ceil((order + 1) / 2) cells), and each of its refined particles that
+// lands in a destination box is appended to the destination array. The appends are ordered:
+// coarse particle order, then pattern order, through a count pass, a scan of the workgroup
+// counts and a write pass with the same in-workgroup prefix.
+#include "phx_kernels.hpp"
+
+namespace phx
+{
+namespace
+{
+constexpr int BLOCK = 256;
+
+__device__ __forceinline__ bool inAnyBox(int const* c, SplitSpec const& S, int D, bool grown)
+{
+    for (int b = 0; b < S.nBoxes; ++b)
+    {
+        bool in = true;
+        for (int d = 0; d < D; ++d)
+        {
+            int const lo = S.lo[b][d] - (grown ? S.grow : 0);
+            int const hi = S.hi[b][d] + (grown ? S.grow : 0);
+            in           = in && c[d] >= lo && c[d] <= hi;
+        }
+        if (in)
+            return true;
+    }
+    return false;
+}
+
+// the coarse particle i on the fine grid (toFineGrid): cell * 2 + int(2 delta), the fraction as
+// delta; false for an empty slot or a particle whose fine cell is in no split box
+template<int D>
+__device__ __forceinline__ bool fineParent(PView const& src, unsigned long long i, SplitSpec const& S,
+                                           int* cell, double* delta)
+{
+    if (src.ic[0][i] == DEAD_CELL)
+        return false;
+#pragma unroll
+    for (int d = 0; d < D; ++d)
+    {
+        double const fd = src.de[d][i] * static_cast<double>(S.ratio);
+        int const fi    = static_cast<int>(fd);
+        cell[d]         = src.ic[d][i] * S.ratio + fi;
+        delta[d]        = fd - fi;
+    }
+    return inAnyBox(cell, S, D, true);
+}
+
+// refined particle k of a parent on the fine grid (PatternDispatcher::dispatch, splitter.hpp:
+// 71-103): delta + the pattern's (single precision) offset, its floor moved into the cell
+template<int D>
+__device__ __forceinline__ void refined(SplitSpec const& S, int k, int const* pc, double const* pd,
+                                        int* cell, double* delta)
+{
+#pragma unroll
+    for (int d = 0; d < D; ++d)
+    {
+        double x            = pd[d] + static_cast<double>(S.delta[k][d]);
+        double const integra = floor(x);
+        x -= integra;
+        delta[d] = x;
+        cell[d]  = pc[d] + static_cast<int>(integra);
+    }
+}
+
+template<int D>
+__device__ __forceinline__ unsigned int keptCount(PView const& src, unsigned long long i,
+                                                  unsigned long long n, SplitSpec const& S)
+{
+    if (i >= n)
+        return 0u;
+    int pc[D];
+    double pd[D];
+    if (!fineParent<D>(src, i, S, pc, pd))
+        return 0u;
+    unsigned int k = 0;
+    for (int r = 0; r < S.nRefined; ++r)
+    {
+        int c[D];
+        double dl[D];
+        refined<D>(S, r, pc, pd, c, dl);
+        k += inAnyBox(c, S, D, false) ? 1u : 0u;
+    }
+    return k;
+}
+
+template<int D>
+__global__ __launch_bounds__(BLOCK) void k_split_count(PView src, unsigned long long n, SplitSpec S,
+                                                       unsigned int* __restrict__ blockCounts)
+{
+    __shared__ unsigned int s;
+    if (threadIdx.x == 0)
+        s = 0;
+    __syncthreads();
+    unsigned long long const i = static_cast<unsigned long long>(blockIdx.x) * BLOCK + threadIdx.x;
+    unsigned int const k       = keptCount<D>(src, i, n, S);
+    if (k)
+        atomicAdd(&s, k);
+    __syncthreads();
+    if (threadIdx.x == 0)
+        blockCounts[blockIdx.x] = s;
+}
+
+// exclusive scan of the workgroup counts in one workgroup of 1024; the total at *total
+__global__ __launch_bounds__(1024) void k_split_scan(unsigned int* __restrict__ blockCounts,
+                                                     unsigned int nblocks,
+                                                     unsigned long long* __restrict__ total)
+{
+    __shared__ unsigned long long part[1024];
+    unsigned int const per = (nblocks + 1023) / 1024;
+    unsigned int const b0  = threadIdx.x * per;
+    unsigned long long sum = 0;
+    for (unsigned int b = b0; b < b0 + per && b < nblocks; ++b)
+        sum += blockCounts[b];
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    for (unsigned int off = 1; off < 1024; off <<= 1)
+    {
+        unsigned long long const v = threadIdx.x >= off ? part[threadIdx.x - off] : 0ull;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    unsigned long long run = threadIdx.x ? part[threadIdx.x - 1] : 0ull;
+    for (unsigned int b = b0; b < b0 + per && b < nblocks; ++b)
+    {
+        unsigned int const c = blockCounts[b];
+        blockCounts[b]        = static_cast<unsigned int>(run);
+        run += c;
+    }
+    if (threadIdx.x == 1023)
+        *total = part[1023];
+}
+
+// the refined particles, at dstOffset + the workgroup's offset + the thread's in-workgroup prefix
+template<int D>
+__global__ __launch_bounds__(BLOCK) void k_split_write(PView src, unsigned long long n, SplitSpec S,
+                                                       unsigned int const* __restrict__ offsets,
+                                                       PView dst, unsigned long long dstOffset,
+                                                       double weightFactor)
+{
+    __shared__ unsigned int scan[BLOCK];
+    unsigned long long const i = static_cast<unsigned long long>(blockIdx.x) * BLOCK + threadIdx.x;
+    unsigned int const k       = keptCount<D>(src, i, n, S);
+    scan[threadIdx.x]          = k;
+    __syncthreads();
+    for (int off = 1; off < BLOCK; off <<= 1)
+    {
+        unsigned int const v = threadIdx.x >= off ? scan[threadIdx.x - off] : 0u;
+        __syncthreads();
+        scan[threadIdx.x] += v;
+        __syncthreads();
+    }
+    if (k == 0)
+        return;
+    unsigned long long j = dstOffset + offsets[blockIdx.x] + scan[threadIdx.x] - k;
+    int pc[D];
+    double pd[D];
+    fineParent<D>(src, i, S, pc, pd);
+    double const w = src.w[i], q = src.q[i];
+    double const vx = src.v[0][i], vy = src.v[1][i], vz = src.v[2][i];
+    for (int r = 0; r < S.nRefined; ++r)
+    {
+        int c[D];
+        double dl[D];
+        refined<D>(S, r, pc, pd, c, dl);
+        if (!inAnyBox(c, S, D, false))
+            continue;
+#pragma unroll
+        for (int d = 0; d < D; ++d)
+        {
+            dst.ic[d][j] = c[d];
+            dst.de[d][j] = dl[d];
+        }
+        dst.v[0][j] = vx;
+        dst.v[1][j] = vy;
+        dst.v[2][j] = vz;
+        // particle.weight * weight * power[dimension - 1] (splitter.hpp:91)
+        dst.w[j] = w * static_cast<double>(S.weight[r]) * weightFactor;
+        dst.q[j] = q;
+        ++j;
+    }
+}
+
+// FieldLinearTimeInterpolate (field_linear_time_interpolate.hpp): dst = (1 - alpha) old + alpha
+// new on the nodes of the given local box
+__global__ __launch_bounds__(BLOCK) void k_time_interp(double const* __restrict__ a,
+                                                       double const* __restrict__ b,
+                                                       double* __restrict__ out, int s1, int s2,
+                                                       int lo0, int lo1, int lo2, int n0, int n1,
+                                                       int n2, double alpha)
+{
+    long const t = static_cast<long>(blockIdx.x) * BLOCK + threadIdx.x;
+    long const total = static_cast<long>(n0) * n1 * n2;
+    if (t >= total)
+        return;
+    int const k = static_cast<int>(t % n2);
+    int const j = static_cast<int>((t / n2) % n1);
+    int const i = static_cast<int>(t / (static_cast<long>(n1) * n2));
+    long const x = (static_cast<long>(lo0 + i) * s1 + (lo1 + j)) * s2 + (lo2 + k);
+    out[x]       = (1. - alpha) * a[x] + alpha * b[x];
+}
+} // namespace
+
+// The split patterns (src/amr/data/particles/refine/split_{1,2,3}d.hpp): refined particle
+// offsets from the parent (in fine cells) and weights (fractions of the parent's weight before the
+// factor ratio^dim), in the order PatternDispatcher places them. Patterns: black = the parent's
+// position; pink = the 2 D offsets +-delta along each axis (1D: +d, -d; 2D: (0,-d), (-d,0),
+// (+d,0), (0,+d); 3D: +d along x, z, y, then -d along x, z, y); purple (2D) = the 4 corners
+// (-d,-d), (-d,+d), (+d,-d), (+d,+d). The default refined particle numbers of PHARE (2 in 1D,
+// 4 in 2D, 6 in 3D) and every 1D variant.
+bool splitPattern(int dim, int order, int nbr, SplitSpec& S)
+{
+    S.nRefined = 0;
+    auto put = [&S](float w, float dx, float dy, float dz) {
+        S.weight[S.nRefined]   = w;
+        S.delta[S.nRefined][0] = dx;
+        S.delta[S.nRefined][1] = dy;
+        S.delta[S.nRefined][2] = dz;
+        ++S.nRefined;
+    };
+    auto black = [&](float w) { put(w, 0.f, 0.f, 0.f); };
+    auto pink1 = [&](float w, float d) {
+        put(w, d, 0.f, 0.f);
+        put(w, -d, 0.f, 0.f);
+    };
+    auto pink2 = [&](float w, float d) {
+        put(w, 0.f, -d, 0.f);
+        put(w, -d, 0.f, 0.f);
+        put(w, d, 0.f, 0.f);
+        put(w, 0.f, d, 0.f);
+    };
+    auto purple2 = [&](float w, float d) {
+        put(w, -d, -d, 0.f);
+        put(w, -d, d, 0.f);
+        put(w, d, -d, 0.f);
+        put(w, d, d, 0.f);
+    };
+    auto pink3 = [&](float w, float d) {
+        for (int i = 0; i < 2; ++i)
+        {
+            float const m = d * (i % 2 ? -1.f : 1.f);
+            put(w, m, 0.f, 0.f);
+            put(w, 0.f, 0.f, m);
+            put(w, 0.f, m, 0.f);
+        }
+    };
+    switch (dim * 100 + order * 10 + nbr)
+    {
+        case 112: pink1(0.5f, 0.551569f); break;
+        case 113:
+            black(0.5f);
+            pink1(0.25f, 1.f);
+            break;
+        case 122: pink1(0.5f, 0.663959f); break;
+        case 123:
+            black(0.468137f);
+            pink1(0.265931f, 1.112033f);
+            break;
+        case 124:
+            pink1(0.375f, 0.5f);
+            pink1(0.125f, 1.5f);
+            break;
+        case 132: pink1(0.5f, 0.752399f); break;
+        case 133:
+            black(0.473943f);
+            pink1(0.263028f, 1.275922f);
+            break;
+        case 134:
+            pink1(0.364766f, 0.542949f);
+            pink1(0.135234f, 1.664886f);
+            break;
+        case 135:
+            black(0.375f);
+            pink1(0.25f, 1.f);
+            pink1(0.0625f, 2.f);
+            break;
+        case 214: purple2(0.25f, 0.571783f); break;
+        case 224: pink2(0.25f, 0.683734f); break;
+        case 234: purple2(0.25f, 0.776459f); break;
+        case 316: pink3(.166666f, .966431f); break;
+        case 326: pink3(.166666f, 1.149658f); break;
+        case 336: pink3(.166666f, 1.312622f); break;
+        default: return false;
+    }
+    return true;
+}
+
+unsigned long long launchSplitCount(PView const& src, unsigned long long n, SplitSpec const& S, int dim,
+                                    unsigned int* blockCounts, unsigned long long* total, hipStream_t s)
+{
+    unsigned int const nb = static_cast<unsigned int>((n + BLOCK - 1) / BLOCK);
+    if (nb == 0)
+        return 0;
+    switch (dim)
+    {
+        case 1: k_split_count<1><<<nb, BLOCK, 0, s>>>(src, n, S, blockCounts); break;
+        case 2: k_split_count<2><<<nb, BLOCK, 0, s>>>(src, n, S, blockCounts); break;
+        case 3: k_split_count<3><<<nb, BLOCK, 0, s>>>(src, n, S, blockCounts); break;
+    }
+    k_split_scan<<<1, 1024, 0, s>>>(blockCounts, nb, total);
+    return nb;
+}
+
+void launchSplitWrite(PView const& src, unsigned long long n, SplitSpec const& S, int dim,
+                      unsigned int const* offsets, PView const& dst, unsigned long long dstOffset,
+                      hipStream_t s)
+{
+    unsigned int const nb = static_cast<unsigned int>((n + BLOCK - 1) / BLOCK);
+    if (nb == 0)
+        return;
+    double const factor = static_cast<double>(dim == 1 ? S.ratio : (dim == 2 ? S.ratio * S.ratio
+                                                                              : S.ratio * S.ratio * S.ratio));
+    switch (dim)
+    {
+        case 1: k_split_write<1><<<nb, BLOCK, 0, s>>>(src, n, S, offsets, dst, dstOffset, factor); break;
+        case 2: k_split_write<2><<<nb, BLOCK, 0, s>>>(src, n, S, offsets, dst, dstOffset, factor); break;
+        case 3: k_split_write<3><<<nb, BLOCK, 0, s>>>(src, n, S, offsets, dst, dstOffset, factor); break;
+    }
+}
+
+void launchTimeInterp(double const* a, double const* b, double* out, int const* shape, int const* lo,
+                      int const* n, double alpha, hipStream_t s)
+{
+    long const total = static_cast<long>(n[0]) * n[1] * n[2];
+    if (total <= 0)
+        return;
+    unsigned int const nb = static_cast<unsigned int>((total + BLOCK - 1) / BLOCK);
+    k_time_interp<<<nb, BLOCK, 0, s>>>(a, b, out, shape[1], shape[2], lo[0], lo[1], lo[2], n[0], n[1],
+                                       n[2], alpha);
+}
+
+} // namespace phx

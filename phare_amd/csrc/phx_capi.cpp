@@ -89,7 +89,7 @@ struct Pop
     double charge = 0.; // the particles' charge (mixed-precision family: one per population)
     bool hasCharge = false;
     bool uniformCharge = true; // every particle uploaded so far has charge `charge`
-    Store arr[3]; // domain, patch ghost, level ghost
+    Store arr[5]; // domain, patch ghost, level ghost, level ghost old, level ghost new
     double* mom[5] = {};
     SegTables seg{}; // tables of the segmented domain array (items sized for itemsAlloc items)
     unsigned int itemsAlloc = 0;
@@ -770,7 +770,7 @@ phx_status validPop(phx_ctx* c, int pop, int kind)
 {
     if (!c)
         return PHX_ERR_ARG;
-    if (pop < 0 || pop >= c->cfg.n_pop || kind < 0 || kind > 2)
+    if (pop < 0 || pop >= c->cfg.n_pop || kind < 0 || kind > PHX_LEVEL_GHOST_NEW)
         return fail(c, PHX_ERR_ARG, "bad population or particle array kind");
     return PHX_OK;
 }
@@ -1443,7 +1443,7 @@ phx_status createLevel(phx_config const* cfg, phx_patch const* patch, int const*
         c->pops[p].mass = cfg->mass[p];
         for (int m = 0; m < 5 && s == PHX_OK; ++m)
             s = alloc(&c->pops[p].mom[m], QMOM);
-        for (int k = 0; k < 3 && s == PHX_OK; ++k)
+        for (int k = 0; k < 5 && s == PHX_OK; ++k)
             s = storeAlloc(c, c->pops[p].arr[k], k == 0 ? std::max<size_t>(particle_capacity, 1024) : 1024);
     }
     c->tuning = defaultTuning(D);
@@ -2181,6 +2181,129 @@ phx_status phx_deposit(phx_ctx* c, int pop, int kind, double coef)
     launchDeposit(A, c->pops[pop].arr[kind].view(), c->stream);
     PHX_TRY(checkLaunch(c, "phx_deposit"));
     return checkDeviceError(c);
+}
+
+phx_status phx_refine_particles(phx_ctx* fine, phx_ctx const* coarse, int pop, int dest_kind,
+                                int n_boxes, int const* box_lower, int const* box_upper,
+                                int refined_particle_nbr)
+{
+    if (!fine || !coarse || fine == coarse)
+        return fine ? fail(fine, PHX_ERR_ARG, "phx_refine_particles: two different contexts") : PHX_ERR_ARG;
+    PHX_TRY(validPop(fine, pop, dest_kind));
+    int const D = fine->cfg.dim, O = fine->cfg.interp_order;
+    if (coarse->cfg.dim != D || coarse->cfg.interp_order != O || pop >= coarse->cfg.n_pop)
+        return fail(fine, PHX_ERR_ARG, "phx_refine_particles: coarse and fine levels differ in "
+                                       "dimension, order or populations");
+    if (mixed(fine) || mixed(coarse))
+        return fail(fine, PHX_ERR_STATE, "phx_refine_particles: the split runs on FP64 levels");
+    if (dest_kind == PHX_PATCH_GHOST)
+        return fail(fine, PHX_ERR_ARG, "phx_refine_particles: the split fills the domain or the "
+                                       "level ghost arrays");
+    if (n_boxes < 1 || n_boxes > MAX_SPLIT_BOXES || !box_lower || !box_upper)
+        return fail(fine, PHX_ERR_ARG, "phx_refine_particles: 1.." + std::to_string(MAX_SPLIT_BOXES)
+                                           + " destination boxes");
+    SplitSpec S{};
+    int const nbr = refined_particle_nbr ? refined_particle_nbr : (D == 1 ? 2 : (D == 2 ? 4 : 6));
+    if (!splitPattern(D, O, nbr, S))
+        return fail(fine, PHX_ERR_ARG, "phx_refine_particles: no split pattern for dim "
+                                           + std::to_string(D) + ", order " + std::to_string(O)
+                                           + ", " + std::to_string(nbr) + " refined particles");
+    S.nBoxes = n_boxes;
+    S.ratio  = 2;
+    // Splitter::maxCellDistanceFromSplit (splitter.hpp:116-120): ceil((order + 1) / 2)
+    S.grow = (O + 2) / 2;
+    for (int b = 0; b < n_boxes; ++b)
+        for (int d = 0; d < 3; ++d)
+        {
+            S.lo[b][d] = d < D ? box_lower[3 * b + d] : 0;
+            S.hi[b][d] = d < D ? box_upper[3 * b + d] : 0;
+        }
+    Store const& src = coarse->pops[pop].arr[PHX_DOMAIN];
+    Store& dst       = fine->pops[pop].arr[dest_kind];
+    if (src.n == 0)
+        return PHX_OK;
+    // the coarse level's arrays are read on the fine level's stream
+    PHX_HIP(fine, hipStreamSynchronize(coarse->stream));
+    size_t const nb = (src.n + 255) / 256 + 1;
+    if (nb > fine->blockCap)
+    {
+        devFree(fine, fine->blockCnt);
+        fine->blockCnt = nullptr;
+        fine->blockCap = 0;
+        PHX_TRY(devAlloc(fine, (void**)&fine->blockCnt, nb * 2 * 4));
+        fine->blockCap = nb * 2;
+    }
+    launchSplitCount(src.view(), src.n, S, D, fine->blockCnt, fine->tot, fine->stream);
+    PHX_TRY(checkLaunch(fine, "phx_refine_particles (count)"));
+    unsigned long long got = 0;
+    PHX_HIP(fine, hipStreamSynchronize(fine->stream));
+    PHX_HIP(fine, hipMemcpy(&got, fine->tot, 8, hipMemcpyDeviceToHost));
+    if (got == 0)
+        return PHX_OK;
+    PHX_TRY(storeReserve(fine, dst, dst.n + got));
+    launchSplitWrite(src.view(), src.n, S, D, fine->blockCnt, dst.view(), dst.n, fine->stream);
+    PHX_TRY(checkLaunch(fine, "phx_refine_particles (write)"));
+    dst.n += got;
+    if (dest_kind == PHX_DOMAIN)
+        dst.seg = false;
+    return PHX_OK;
+}
+
+phx_status phx_level_ghost_last_step(phx_ctx* c)
+{
+    if (!c)
+        return PHX_ERR_ARG;
+    PHX_HIP(c, hipStreamSynchronize(c->stream));
+    for (int p = 0; p < c->cfg.n_pop; ++p)
+    {
+        Pop& P = c->pops[p];
+        // std::swap(levelGhostParticlesNew, levelGhostParticlesOld); New.clear(); levelGhost = Old
+        std::swap(P.arr[PHX_LEVEL_GHOST_NEW], P.arr[PHX_LEVEL_GHOST_OLD]);
+        P.arr[PHX_LEVEL_GHOST_NEW].n = 0;
+        Store const& old = P.arr[PHX_LEVEL_GHOST_OLD];
+        Store& lg        = P.arr[PHX_LEVEL_GHOST];
+        lg.n             = 0;
+        PHX_TRY(storeReserve(c, lg, old.n));
+        if (old.n)
+        {
+            launchSegCopy(old.view(), lg.view(), old.n, c->cfg.dim, c->stream);
+            PHX_TRY(checkLaunch(c, "phx_level_ghost_last_step"));
+        }
+        lg.n   = old.n;
+        lg.seg = false;
+    }
+    return PHX_OK;
+}
+
+phx_status phx_time_interpolate(phx_ctx* dst, int dst_field, phx_ctx const* old_ctx, int old_field,
+                                phx_ctx const* new_ctx, int new_field, int const lower[3],
+                                int const upper[3], double alpha)
+{
+    if (!dst || !old_ctx || !new_ctx || !lower || !upper)
+        return dst ? fail(dst, PHX_ERR_ARG, "phx_time_interpolate: null argument") : PHX_ERR_ARG;
+    double *pd, *po, *pn;
+    int qd, qo, qn;
+    if (!fieldOf(dst, dst_field, pd, qd) || !fieldOf(old_ctx, old_field, po, qo)
+        || !fieldOf(new_ctx, new_field, pn, qn))
+        return fail(dst, PHX_ERR_ARG, "phx_time_interpolate: bad field id");
+    int shape[3], lo[3], n[3];
+    for (int d = 0; d < 3; ++d)
+    {
+        shape[d] = dst->G.shape[qd][d];
+        if (old_ctx->G.shape[qo][d] != shape[d] || new_ctx->G.shape[qn][d] != shape[d])
+            return fail(dst, PHX_ERR_ARG, "phx_time_interpolate: the three fields differ in shape");
+        lo[d] = d < dst->cfg.dim ? lower[d] : 0;
+        int const hi = d < dst->cfg.dim ? upper[d] : 0;
+        if (lo[d] < 0 || hi >= shape[d] || hi < lo[d])
+            return fail(dst, PHX_ERR_ARG, "phx_time_interpolate: the box leaves the field");
+        n[d] = hi - lo[d] + 1;
+    }
+    if (old_ctx != dst)
+        PHX_HIP(dst, hipStreamSynchronize(old_ctx->stream));
+    if (new_ctx != dst && new_ctx != old_ctx)
+        PHX_HIP(dst, hipStreamSynchronize(new_ctx->stream));
+    launchTimeInterp(po, pn, pd, shape, lo, n, alpha, dst->stream);
+    return checkLaunch(dst, "phx_time_interpolate");
 }
 
 phx_status phx_update_ions(phx_ctx* c)

@@ -225,6 +225,38 @@ void launchCompact(PView const& src, PView const& dst, uint8_t const* status, un
                    uint8_t want, unsigned int* blockCounts, unsigned long long* total,
                    unsigned long long dstOffset, int dim, Wrap const& wrap, hipStream_t s);
 
+// ---- coarse-to-fine coupling (phx_amr.hip) ------------------------------------------------------
+constexpr int MAX_SPLIT_BOXES = 32;
+constexpr int MAX_REFINED     = 27;
+// one split: destination boxes (fine AMR cells, inclusive, disjoint), the split-box growth
+// ceil((order + 1) / 2), the refinement ratio and the pattern of refined particles (weight and
+// offset of each, in single precision as the reference's split tables hold them)
+struct SplitSpec
+{
+    int nBoxes;
+    int lo[MAX_SPLIT_BOXES][3], hi[MAX_SPLIT_BOXES][3];
+    int grow;
+    int ratio;
+    int nRefined;
+    float weight[MAX_REFINED];
+    float delta[MAX_REFINED][3];
+};
+// the split pattern of (dim, order, refined particles) into S (nRefined, weight, delta); false
+// when the combination is not supported
+bool splitPattern(int dim, int order, int nbr, SplitSpec& S);
+// count pass: the refined particles of src (n slots) kept in the destination boxes, per
+// workgroup, scanned in place (blockCounts: one word per 256 slots), the total at *total; returns
+// the workgroup count
+unsigned long long launchSplitCount(PView const& src, unsigned long long n, SplitSpec const& S, int dim,
+                                    unsigned int* blockCounts, unsigned long long* total, hipStream_t s);
+// write pass: the refined particles appended to dst at dstOffset in coarse-particle order
+void launchSplitWrite(PView const& src, unsigned long long n, SplitSpec const& S, int dim,
+                      unsigned int const* offsets, PView const& dst, unsigned long long dstOffset,
+                      hipStream_t s);
+// out = (1 - alpha) a + alpha b over the local box lo .. lo + n - 1 of arrays of the given shape
+void launchTimeInterp(double const* a, double const* b, double* out, int const* shape, int const* lo,
+                      int const* n, double alpha, hipStream_t s);
+
 // ---- fields ----------------------------------------------------------------------------------
 struct VecPtr
 {

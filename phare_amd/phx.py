@@ -13,7 +13,7 @@ LIB_PATH = os.path.join(HERE, "libphx.so")
 PHX_OK = 0
 PRECISION_FP64, PRECISION_FP32 = 0, 1
 MODE_DOMAIN_ONLY, MODE_ALL = 1, 2
-DOMAIN, PATCH_GHOST, LEVEL_GHOST = 0, 1, 2
+DOMAIN, PATCH_GHOST, LEVEL_GHOST, LEVEL_GHOST_OLD, LEVEL_GHOST_NEW = 0, 1, 2, 3, 4
 EM, EM_PRED, EM_AVG = 0, 1, 2
 F_E, F_B, F_EPRED, F_BPRED, F_EAVG, F_BAVG = 0, 3, 6, 9, 12, 15
 F_J, F_VE, F_PE, F_RHO, F_MASSD, F_VI, F_POP0 = 18, 21, 24, 25, 26, 27, 32
@@ -110,6 +110,10 @@ def lib():
         L.phx_gather.argtypes = [vp, C.c_int, C.c_int, C.c_int, _dp, _dp]
         L.phx_gather_noalloc.argtypes = [vp, C.c_int, C.c_int, C.c_int, _dp, _dp]
         L.phx_deposit.argtypes = [vp, C.c_int, C.c_int, C.c_double]
+        L.phx_refine_particles.argtypes = [vp, vp, C.c_int, C.c_int, C.c_int, _ip, _ip, C.c_int]
+        L.phx_level_ghost_last_step.argtypes = [vp]
+        L.phx_time_interpolate.argtypes = [vp, C.c_int, vp, C.c_int, vp, C.c_int, _ip, _ip,
+                                           C.c_double]
         L.phx_get_tuning.argtypes = [vp, C.POINTER(Tuning)]
         L.phx_set_tuning.argtypes = [vp, C.POINTER(Tuning)]
         _lib = L
@@ -128,7 +132,8 @@ EXPORTED = ["phx_create", "phx_destroy", "phx_last_error", "phx_synchronize", "p
             "phx_create_level", "phx_patch_box", "phx_patch_nonlevel_ghost_boxes",
             "phx_patch_field_shape", "phx_set_patch_field", "phx_get_patch_field",
             "phx_get_patch_particles", "phx_add_particles", "phx_gather_noalloc",
-            "phx_get_tuning", "phx_set_tuning"]
+            "phx_get_tuning", "phx_set_tuning", "phx_refine_particles",
+            "phx_level_ghost_last_step", "phx_time_interpolate"]
 
 
 class Level:
@@ -413,6 +418,27 @@ class Level:
         t = Tuning((C.c_int * 3)(*cur["segment_shape"]), cur["rows_per_item"], cur["tile_row_pad"],
                    cur["sort_interval"], cur["tile_copies"], cur["deposit_window"])
         self._rc(lib().phx_set_tuning(self.h, C.byref(t)))
+
+    def refine_particles(self, coarse, boxes, pop=0, kind=LEVEL_GHOST, nbr=0):
+        """phx_refine_particles: split the domain particles of `coarse` (a Level) into array
+        `kind` of this (fine) level, keeping those in the fine AMR boxes [(lower, upper), ...]"""
+        lo = np.zeros(3 * len(boxes), np.int32)
+        hi = np.zeros(3 * len(boxes), np.int32)
+        for i, (a, b) in enumerate(boxes):
+            lo[3 * i:3 * i + self.dim] = a
+            hi[3 * i:3 * i + self.dim] = b
+        self._rc(lib().phx_refine_particles(self.h, coarse.h, pop, kind, len(boxes), lo, hi, nbr))
+
+    def level_ghost_last_step(self):
+        self._rc(lib().phx_level_ghost_last_step(self.h))
+
+    def time_interpolate(self, fid, old, old_fid, new, new_fid, lower, upper, alpha):
+        lo = np.zeros(3, np.int32)
+        hi = np.zeros(3, np.int32)
+        lo[:self.dim] = lower
+        hi[:self.dim] = upper
+        self._rc(lib().phx_time_interpolate(self.h, fid, old.h, old_fid, new.h, new_fid, lo, hi,
+                                            alpha))
 
     def deposit(self, pop=0, kind=DOMAIN, coef=1.0):
         self._rc(lib().phx_deposit(self.h, pop, kind, coef))

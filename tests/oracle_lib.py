@@ -98,6 +98,9 @@ def lib():
         L.ora_partition.argtypes = [C.c_int, _ip, _ip, C.c_long, _ip, _ip, _ip, _ip, _lp]
         L.ora_boris_trajectory.argtypes = [C.c_int, C.c_long, C.c_double, _dp, C.c_double, C.c_double,
                                            _dp, _dp, _ip, _dp, _dp, _dp]
+        L.ora_split.restype = C.c_long
+        L.ora_split.argtypes = [C.c_int, C.c_int, C.c_int, C.c_long, _ip, _dp, _dp, _dp, _dp, C.c_int,
+                                _ip, _ip, C.c_long, _ip, _dp, _dp, _dp, _dp]
         _lib = L
     return _lib
 
@@ -126,6 +129,10 @@ def ref():
                                           _dp, _dp, _dp]
         R.ref_partition.restype = C.c_long
         R.ref_partition.argtypes = [C.c_int, _ip, _ip, C.c_long, _ip, _ip, _ip, _ip, _lp]
+        R.ref_split_one.restype = C.c_long
+        R.ref_split_one.argtypes = [C.c_int, C.c_int, C.c_int, _ip, _dp, _dp, C.c_double, C.c_double,
+                                    _ip, _dp, _dp, _dp, _dp]
+        R.ref_split_max_cell_distance.argtypes = [C.c_int]
         _ref = R
     return _ref
 
@@ -305,3 +312,55 @@ def ref_load_maxwellian(ncells, lower, n, V, Vth, charge, ppc, seed, cutoff=1e-1
     assert 0 <= m <= cap
     return dict(icell=ic[:m * dim].reshape(m, dim), delta=de[:m * dim].reshape(m, dim),
                 v=v[:m * 3].reshape(m, 3), weight=w[:m], charge=q[:m])
+
+
+def split(dim, order, nbr, p, boxes):
+    """the oracle's coarse-to-fine split of particle set p (SoA dict) into the fine AMR boxes
+    [(lower, upper), ...]: the refined particles kept, as a SoA dict"""
+    n = len(p["weight"])
+    lo = np.zeros(3 * len(boxes), np.int32)
+    hi = np.zeros(3 * len(boxes), np.int32)
+    for i, (a, b) in enumerate(boxes):
+        lo[3 * i:3 * i + dim] = a
+        hi[3 * i:3 * i + dim] = b
+    cap = max(1, n * max(nbr, 6) * len(boxes))
+    out = dict(icell=np.zeros((cap, dim), np.int32), delta=np.zeros((cap, dim)), v=np.zeros((cap, 3)),
+               weight=np.zeros(cap), charge=np.zeros(cap))
+    k = lib().ora_split(dim, order, nbr, n, np.ascontiguousarray(p["icell"], np.int32).reshape(-1),
+                        np.ascontiguousarray(p["delta"], np.float64).reshape(-1),
+                        np.ascontiguousarray(p["v"], np.float64).reshape(-1),
+                        np.ascontiguousarray(p["weight"], np.float64),
+                        np.ascontiguousarray(p["charge"], np.float64), len(boxes), lo, hi, cap,
+                        out["icell"].reshape(-1), out["delta"].reshape(-1), out["v"].reshape(-1),
+                        out["weight"], out["charge"])
+    if k < 0:
+        raise ValueError(f"no split pattern for dim {dim} order {order} nbr {nbr}")
+    return {key: val[:k] for key, val in out.items()}
+
+
+def split_one(dim, order, nbr, icell, delta, v, w, q, use_ref=False):
+    """one particle already on the fine grid split by the oracle's tables (use_ref: by the
+    reference's Splitter compiled in oracle/_ref): the refined particles in dispatch order"""
+    ic = np.ascontiguousarray(icell, np.int32)
+    de = np.ascontiguousarray(delta, np.float64)
+    vv = np.ascontiguousarray(v, np.float64)
+    if use_ref:
+        cap = 32
+        out = dict(icell=np.zeros((cap, dim), np.int32), delta=np.zeros((cap, dim)),
+                   v=np.zeros((cap, 3)), weight=np.zeros(cap), charge=np.zeros(cap))
+        k = ref().ref_split_one(dim, order, nbr, ic, de, vv, float(w), float(q),
+                                out["icell"].reshape(-1), out["delta"].reshape(-1),
+                                out["v"].reshape(-1), out["weight"], out["charge"])
+        return {key: val[:k] for key, val in out.items()}
+    # the oracle splits coarse particles: for an even fine cell the coarse particle
+    # (cell / 2, delta / 2) maps back onto exactly this fine particle under toFineGrid (halving
+    # and doubling are exact in binary; an odd cell would need 1 + delta, which rounds), and one
+    # destination box covering everything keeps every refined particle
+    if np.any(ic % 2):
+        raise ValueError("split_one: the oracle path takes even fine cells")
+    cc = ic // 2
+    cd = de / 2.0
+    p = dict(icell=cc.reshape(1, dim), delta=cd.reshape(1, dim), v=vv.reshape(1, 3),
+             weight=np.array([w], float), charge=np.array([q], float))
+    big = (tuple([-10 ** 6] * dim), tuple([10 ** 6] * dim))
+    return split(dim, order, nbr, p, [big])
