@@ -77,6 +77,10 @@ enum {
 #define PHX_F_POP(pop, k) (PHX_F_POP0 + 8 * (pop) + (k))
 
 #define PHX_MAX_POP 4
+/* level synchronisation fields of SolverPPC (solver_ppc.hpp:242-290): Bold (B at the start of the
+   step, prepareStep) and fluxSumE (the time-integrated averaged E a finer level hands to the
+   coarser one's reflux), 3 components each */
+enum { PHX_F_BOLD_X = 64, PHX_F_FLUXSUM_X = 67 };
 
 typedef struct {
     int dim;          /* 1, 2, 3 */
@@ -235,6 +239,30 @@ phx_status phx_level_ghost_last_step(phx_ctx* ctx);
 phx_status phx_time_interpolate(phx_ctx* dst, int dst_field, const phx_ctx* old_ctx, int old_field,
                                 const phx_ctx* new_ctx, int new_field, const int lower[3],
                                 const int upper[3], double alpha);
+/* ---- level synchronisation (fine to coarse) ------------------------------------------------- */
+/* SolverPPC::prepareStep (solver_ppc.hpp:240-257): Bold = B */
+phx_status phx_prepare_step(phx_ctx* ctx);
+/* SolverPPC::resetFluxSum / accumulateFluxSum (solver_ppc.hpp:260-290): fluxSumE = 0;
+   fluxSumE += Eavg * coef (the multiphysics integrator's coef is 1 / ratio^2,
+   multiphysics_integrator.hpp:541-548) */
+phx_status phx_reset_flux_sum(phx_ctx* ctx);
+phx_status phx_accumulate_flux_sum(phx_ctx* ctx, double coef);
+/* SolverPPC::reflux (solver_ppc.hpp:293-308): B = Faraday(Bold, Eavg, dt) on the physical nodes,
+   then the magnetic ghost fill (periodic levels; others are the caller's messenger). The coarser
+   level's Eavg (its "refluxElectric") holds the finer level's coarsened fluxSumE where the finer
+   level covers it (phx_coarsen with PHX_COARSEN_ELECTRIC from PHX_F_FLUXSUM_X). */
+phx_status phx_reflux(phx_ctx* ctx, double dt);
+/* the coarsen operators of HybridHybridMessengerStrategy's synchronisation and reflux
+   (hybrid_hybrid_messenger_strategy.hpp:106-110, 890-905, 194-198), ratio 2: the coarse nodes
+   lower..upper (inclusive, coarse AMR node indices) of coarse_field take the values of fine_field
+   (the same quantity) of the finer level: PHX_COARSEN_MOMENTS injects the fine node 2c
+   (MomentsCoarsener, src/amr/data/field/coarsening/moments_coarsener.hpp:39-75; all-primal
+   quantities: ion charge density, bulk velocity, population moments), PHX_COARSEN_ELECTRIC
+   averages the two fine nodes 2c, 2c + 1 along the component's dual direction
+   (ElectricFieldCoarsener, electric_field_coarsener.hpp:38-140; E and fluxSumE) */
+enum { PHX_COARSEN_MOMENTS = 0, PHX_COARSEN_ELECTRIC = 1 };
+phx_status phx_coarsen(phx_ctx* coarse, int coarse_field, const phx_ctx* fine, int fine_field, int op,
+                       const int lower[3], const int upper[3]);
 /* IonUpdater::updateIons (ion_updater.hpp:111-116) */
 phx_status phx_update_ions(phx_ctx* ctx);
 /* same-level fills of HybridHybridMessengerStrategy (hybrid_hybrid_messenger_strategy.hpp):

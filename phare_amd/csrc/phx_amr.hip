@@ -208,7 +208,66 @@ __global__ __launch_bounds__(BLOCK) void k_time_interp(double const* __restrict_
     long const x = (static_cast<long>(lo0 + i) * s1 + (lo1 + j)) * s2 + (lo2 + k);
     out[x]       = (1. - alpha) * a[x] + alpha * b[x];
 }
+// the coarsening of one coarse node from the fine nodes under it (ratio 2): MomentsCoarsener
+// (src/amr/data/field/coarsening/moments_coarsener.hpp:39-75) injects the fine node 2c;
+// ElectricFieldCoarsener (electric_field_coarsener.hpp:38-140) averages the two fine nodes 2c,
+// 2c + 1 along the component's dual direction and injects along its primal ones. Coarse nodes
+// (i, j, k) run over the local box cLo .. cLo + n - 1 of the coarse array; fOff is the fine local
+// index of coarse local index 0 (2 x (coarse AMR origin) - fine AMR origin)
+__global__ __launch_bounds__(BLOCK) void k_coarsen(double const* __restrict__ fine, int fs1, int fs2,
+                                                   double* __restrict__ coarse, int cs1, int cs2,
+                                                   int clo0, int clo1, int clo2, int n0, int n1,
+                                                   int n2, int fo0, int fo1, int fo2, int dualDir,
+                                                   int dim)
+{
+    long const t     = static_cast<long>(blockIdx.x) * BLOCK + threadIdx.x;
+    long const total = static_cast<long>(n0) * n1 * n2;
+    if (t >= total)
+        return;
+    int const c2 = clo2 + static_cast<int>(t % n2);
+    int const c1 = clo1 + static_cast<int>((t / n2) % n1);
+    int const c0 = clo0 + static_cast<int>(t / (static_cast<long>(n1) * n2));
+    int const f0 = 2 * c0 + fo0;
+    int const f1 = dim > 1 ? 2 * c1 + fo1 : 0;
+    int const f2 = dim > 2 ? 2 * c2 + fo2 : 0;
+    long const fi = (static_cast<long>(f0) * fs1 + f1) * fs2 + f2;
+    double v      = fine[fi];
+    if (dualDir >= 0)
+    {
+        long const step = dualDir == 0 ? static_cast<long>(fs1) * fs2 : (dualDir == 1 ? fs2 : 1);
+        v               = 0.5 * (fine[fi] + fine[fi + step]);
+    }
+    coarse[(static_cast<long>(c0) * cs1 + c1) * cs2 + c2] = v;
+}
+
+// a += b * coef over n values (PlusEqualsProduct, src/core/utilities/types.hpp)
+__global__ __launch_bounds__(BLOCK) void k_axpy(double* __restrict__ a, double const* __restrict__ b,
+                                                double coef, long n)
+{
+    long const t = static_cast<long>(blockIdx.x) * BLOCK + threadIdx.x;
+    if (t < n)
+        a[t] += b[t] * coef;
+}
 } // namespace
+
+void launchCoarsen(double const* fine, int const* fshape, double* coarse, int const* cshape,
+                   int const* clo, int const* n, int const* fineOff, int dualDir, int dim, hipStream_t s)
+{
+    long const total = static_cast<long>(n[0]) * n[1] * n[2];
+    if (total <= 0)
+        return;
+    unsigned int const nb = static_cast<unsigned int>((total + BLOCK - 1) / BLOCK);
+    k_coarsen<<<nb, BLOCK, 0, s>>>(fine, fshape[1], fshape[2], coarse, cshape[1], cshape[2], clo[0],
+                                   clo[1], clo[2], n[0], n[1], n[2], fineOff[0], fineOff[1], fineOff[2],
+                                   dualDir, dim);
+}
+
+void launchAxpy(double* a, double const* b, double coef, long n, hipStream_t s)
+{
+    if (n <= 0)
+        return;
+    k_axpy<<<static_cast<unsigned int>((n + BLOCK - 1) / BLOCK), BLOCK, 0, s>>>(a, b, coef, n);
+}
 
 // The split patterns (src/amr/data/particles/refine/split_{1,2,3}d.hpp): refined particle
 // offsets from the parent (in fine cells) and weights (fractions of the parent's weight before the

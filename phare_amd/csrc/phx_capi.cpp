@@ -109,6 +109,8 @@ struct phx_ctx
     double* em[3][6] = {};    // E/B of each set, one block per set (em[set][0] is its base)
     float* em32[6]   = {};    // mixed-precision family: FP32 copy of the averaged set (gather)
     double* J[3]     = {};
+    double* Bold[3]    = {}; // SolverPPC::Bold_ (prepareStep)
+    double* fluxSum[3] = {}; // SolverPPC::fluxSumE_ (E-centered)
     double* Ve[3]    = {};
     double* Pe = nullptr, *rho = nullptr, *massD = nullptr, *tmp = nullptr;
     double* Vi[3] = {};
@@ -308,7 +310,7 @@ phx_status scratchReserve(phx_ctx* c, size_t n)
 // field id -> (pointer, quantity)
 bool fieldOf(phx_ctx const* c, int fid, double*& p, int& q)
 {
-    if (fid >= PHX_F_POP0)
+    if (fid >= PHX_F_POP0 && fid < PHX_F_POP0 + 8 * PHX_MAX_POP)
     {
         int pop = (fid - PHX_F_POP0) / 8, k = (fid - PHX_F_POP0) % 8;
         if (pop >= c->cfg.n_pop || k > 4)
@@ -328,6 +330,18 @@ bool fieldOf(phx_ctx const* c, int fid, double*& p, int& q)
     {
         p = c->J[fid - PHX_F_JX];
         q = QJX + (fid - PHX_F_JX);
+        return true;
+    }
+    if (fid >= PHX_F_BOLD_X && fid <= PHX_F_BOLD_X + 2)
+    {
+        p = c->Bold[fid - PHX_F_BOLD_X];
+        q = QBX + (fid - PHX_F_BOLD_X);
+        return true;
+    }
+    if (fid >= PHX_F_FLUXSUM_X && fid <= PHX_F_FLUXSUM_X + 2)
+    {
+        p = c->fluxSum[fid - PHX_F_FLUXSUM_X];
+        q = QEX + (fid - PHX_F_FLUXSUM_X);
         return true;
     }
     q = QMOM;
@@ -1418,7 +1432,11 @@ phx_status createLevel(phx_config const* cfg, phx_patch const* patch, int const*
     }
     for (int k = 0; k < 3 && s == PHX_OK; ++k)
     {
-        s = alloc(&c->J[k], QJX + k);
+        s = alloc(&c->Bold[k], QBX + k);
+        if (s == PHX_OK)
+            s = alloc(&c->fluxSum[k], QEX + k);
+        if (s == PHX_OK)
+            s = alloc(&c->J[k], QJX + k);
         if (s == PHX_OK)
             s = alloc(&c->Ve[k], QMOM);
         if (s == PHX_OK)
@@ -1469,6 +1487,11 @@ phx_status createLevel(phx_config const* cfg, phx_patch const* patch, int const*
     for (int set = 0; set < 3; ++set)
         for (int k = 0; k < 6; ++k)
             launchZero(c->em[set][k], G.size(k < 3 ? QEX + k : QBX + k - 3), c->stream);
+    for (int k = 0; k < 3; ++k)
+    {
+        launchZero(c->Bold[k], G.size(QBX + k), c->stream);
+        launchZero(c->fluxSum[k], G.size(QEX + k), c->stream);
+    }
     for (int k = 0; k < 3; ++k)
     {
         launchZero(c->J[k], G.size(QJX + k), c->stream);
@@ -2304,6 +2327,95 @@ phx_status phx_time_interpolate(phx_ctx* dst, int dst_field, phx_ctx const* old_
         PHX_HIP(dst, hipStreamSynchronize(new_ctx->stream));
     launchTimeInterp(po, pn, pd, shape, lo, n, alpha, dst->stream);
     return checkLaunch(dst, "phx_time_interpolate");
+}
+
+phx_status phx_prepare_step(phx_ctx* c)
+{
+    if (!c)
+        return PHX_ERR_ARG;
+    for (int k = 0; k < 3; ++k)
+        launchCopy(c->em[PHX_EM][3 + k], c->Bold[k], count(c, QBX + k), c->stream);
+    return checkLaunch(c, "phx_prepare_step");
+}
+
+phx_status phx_reset_flux_sum(phx_ctx* c)
+{
+    if (!c)
+        return PHX_ERR_ARG;
+    for (int k = 0; k < 3; ++k)
+        launchZero(c->fluxSum[k], count(c, QEX + k), c->stream);
+    return checkLaunch(c, "phx_reset_flux_sum");
+}
+
+phx_status phx_accumulate_flux_sum(phx_ctx* c, double coef)
+{
+    if (!c)
+        return PHX_ERR_ARG;
+    for (int k = 0; k < 3; ++k)
+        launchAxpy(c->fluxSum[k], c->em[PHX_EM_AVG][k], coef, count(c, QEX + k), c->stream);
+    return checkLaunch(c, "phx_accumulate_flux_sum");
+}
+
+phx_status phx_reflux(phx_ctx* c, double dt)
+{
+    if (!c)
+        return PHX_ERR_ARG;
+    launchFaraday(c->G, cvec(c->Bold), cvec(c->em[PHX_EM_AVG]), vec(c->em[PHX_EM] + 3), dt, c->stream);
+    fillGhosts(c, c->em[PHX_EM] + 3, QBX);
+    return checkLaunch(c, "phx_reflux");
+}
+
+phx_status phx_coarsen(phx_ctx* coarse, int coarse_field, phx_ctx const* fine, int fine_field, int op,
+                       int const lower[3], int const upper[3])
+{
+    if (!coarse || !fine || !lower || !upper)
+        return coarse ? fail(coarse, PHX_ERR_ARG, "phx_coarsen: null argument") : PHX_ERR_ARG;
+    if (op != PHX_COARSEN_MOMENTS && op != PHX_COARSEN_ELECTRIC)
+        return fail(coarse, PHX_ERR_ARG, "phx_coarsen: bad operator");
+    double *pc, *pf;
+    int qc, qf;
+    if (!fieldOf(coarse, coarse_field, pc, qc) || !fieldOf(fine, fine_field, pf, qf) || qc != qf)
+        return fail(coarse, PHX_ERR_ARG, "phx_coarsen: bad field ids (one quantity on both levels)");
+    int const D = coarse->cfg.dim;
+    if (fine->cfg.dim != D)
+        return fail(coarse, PHX_ERR_ARG, "phx_coarsen: the levels differ in dimension");
+    int dualDir = -1;
+    for (int d = 0; d < D; ++d)
+        if (centering(qc, d) == CD)
+        {
+            if (op == PHX_COARSEN_MOMENTS || dualDir >= 0)
+                return fail(coarse, PHX_ERR_ARG,
+                            op == PHX_COARSEN_MOMENTS ? "phx_coarsen: the moments operator takes "
+                                                        "all-primal quantities"
+                                                      : "phx_coarsen: the electric operator takes "
+                                                        "E-centered quantities");
+            dualDir = d;
+        }
+    int cshape[3], fshape[3], clo[3], n[3], fo[3];
+    for (int d = 0; d < 3; ++d)
+    {
+        cshape[d] = coarse->G.shape[qc][d];
+        fshape[d] = fine->G.shape[qf][d];
+        if (d >= D)
+        {
+            clo[d] = 0;
+            n[d]   = 1;
+            fo[d]  = 0;
+            continue;
+        }
+        int const cOrg = coarse->G.lower[d] - coarse->G.g; // AMR index of local index 0
+        int const fOrg = fine->G.lower[d] - fine->G.g;
+        clo[d]         = lower[d] - cOrg;
+        int const chi  = upper[d] - cOrg;
+        n[d]           = chi - clo[d] + 1;
+        fo[d]          = 2 * cOrg - fOrg;
+        int const fLast = 2 * chi + fo[d] + (dualDir == d ? 1 : 0);
+        if (clo[d] < 0 || chi >= cshape[d] || n[d] < 1 || 2 * clo[d] + fo[d] < 0 || fLast >= fshape[d])
+            return fail(coarse, PHX_ERR_ARG, "phx_coarsen: the box leaves the coarse or the fine field");
+    }
+    PHX_HIP(coarse, hipStreamSynchronize(fine->stream));
+    launchCoarsen(pf, fshape, pc, cshape, clo, n, fo, dualDir, D, coarse->stream);
+    return checkLaunch(coarse, "phx_coarsen");
 }
 
 phx_status phx_update_ions(phx_ctx* c)

@@ -257,6 +257,15 @@ void launchSplitWrite(PView const& src, unsigned long long n, SplitSpec const& S
 void launchTimeInterp(double const* a, double const* b, double* out, int const* shape, int const* lo,
                       int const* n, double alpha, hipStream_t s);
 
+// coarse node values from the fine level (ratio 2) over the coarse local box clo .. clo + n - 1:
+// dualDir < 0 injects the fine node 2c (MomentsCoarsener; the primal directions of E), dualDir = d
+// averages the fine nodes 2c and 2c + 1 along d (ElectricFieldCoarsener); fineOff: the fine local
+// index of coarse local index 0 per direction
+void launchCoarsen(double const* fine, int const* fshape, double* coarse, int const* cshape,
+                   int const* clo, int const* n, int const* fineOff, int dualDir, int dim, hipStream_t s);
+// a += b * coef over n values
+void launchAxpy(double* a, double const* b, double coef, long n, hipStream_t s);
+
 // ---- fields ----------------------------------------------------------------------------------
 struct VecPtr
 {

@@ -17,6 +17,8 @@ DOMAIN, PATCH_GHOST, LEVEL_GHOST, LEVEL_GHOST_OLD, LEVEL_GHOST_NEW = 0, 1, 2, 3,
 EM, EM_PRED, EM_AVG = 0, 1, 2
 F_E, F_B, F_EPRED, F_BPRED, F_EAVG, F_BAVG = 0, 3, 6, 9, 12, 15
 F_J, F_VE, F_PE, F_RHO, F_MASSD, F_VI, F_POP0 = 18, 21, 24, 25, 26, 27, 32
+F_BOLD, F_FLUXSUM = 64, 67
+COARSEN_MOMENTS, COARSEN_ELECTRIC = 0, 1
 MAX_POP = 4
 
 
@@ -114,6 +116,11 @@ def lib():
         L.phx_level_ghost_last_step.argtypes = [vp]
         L.phx_time_interpolate.argtypes = [vp, C.c_int, vp, C.c_int, vp, C.c_int, _ip, _ip,
                                            C.c_double]
+        L.phx_prepare_step.argtypes = [vp]
+        L.phx_reset_flux_sum.argtypes = [vp]
+        L.phx_accumulate_flux_sum.argtypes = [vp, C.c_double]
+        L.phx_reflux.argtypes = [vp, C.c_double]
+        L.phx_coarsen.argtypes = [vp, C.c_int, vp, C.c_int, C.c_int, _ip, _ip]
         L.phx_get_tuning.argtypes = [vp, C.POINTER(Tuning)]
         L.phx_set_tuning.argtypes = [vp, C.POINTER(Tuning)]
         _lib = L
@@ -133,7 +140,8 @@ EXPORTED = ["phx_create", "phx_destroy", "phx_last_error", "phx_synchronize", "p
             "phx_patch_field_shape", "phx_set_patch_field", "phx_get_patch_field",
             "phx_get_patch_particles", "phx_add_particles", "phx_gather_noalloc",
             "phx_get_tuning", "phx_set_tuning", "phx_refine_particles",
-            "phx_level_ghost_last_step", "phx_time_interpolate"]
+            "phx_level_ghost_last_step", "phx_time_interpolate", "phx_prepare_step",
+            "phx_reset_flux_sum", "phx_accumulate_flux_sum", "phx_reflux", "phx_coarsen"]
 
 
 class Level:
@@ -439,6 +447,27 @@ class Level:
         hi[:self.dim] = upper
         self._rc(lib().phx_time_interpolate(self.h, fid, old.h, old_fid, new.h, new_fid, lo, hi,
                                             alpha))
+
+    def prepare_step(self):
+        self._rc(lib().phx_prepare_step(self.h))
+
+    def reset_flux_sum(self):
+        self._rc(lib().phx_reset_flux_sum(self.h))
+
+    def accumulate_flux_sum(self, coef):
+        self._rc(lib().phx_accumulate_flux_sum(self.h, coef))
+
+    def reflux(self, dt):
+        self._rc(lib().phx_reflux(self.h, dt))
+
+    def coarsen(self, fid, fine, fine_fid, op, lower, upper):
+        """phx_coarsen: this (coarse) level's field fid over the coarse AMR nodes lower..upper
+        from the finer level's field fine_fid"""
+        lo = np.zeros(3, np.int32)
+        hi = np.zeros(3, np.int32)
+        lo[:self.dim] = lower
+        hi[:self.dim] = upper
+        self._rc(lib().phx_coarsen(self.h, fid, fine.h, fine_fid, op, lo, hi))
 
     def deposit(self, pop=0, kind=DOMAIN, coef=1.0):
         self._rc(lib().phx_deposit(self.h, pop, kind, coef))

@@ -364,3 +364,27 @@ def split_one(dim, order, nbr, icell, delta, v, w, q, use_ref=False):
              weight=np.array([w], float), charge=np.array([q], float))
     big = (tuple([-10 ** 6] * dim), tuple([10 ** 6] * dim))
     return split(dim, order, nbr, p, [big])
+
+
+def coarsen(fine, fine_origin, coarse, coarse_origin, lower, upper, dual_dir):
+    """the fine-to-coarse coarsening of one field, ratio 2, restated from the reference's
+    MomentsCoarsener (src/amr/data/field/coarsening/moments_coarsener.hpp:39-75: the coarse node c
+    takes the fine node 2c) and ElectricFieldCoarsener (electric_field_coarsener.hpp:38-140: along
+    the component's dual direction the mean 0.5 * (f[2c] + f[2c + 1]), along the primal ones the
+    fine node 2c). Arrays in local node indices; *_origin: the AMR index of local index 0 per
+    direction (lower - ghost width); lower/upper: coarse AMR nodes (inclusive). Returns a copy of
+    `coarse` with the box overwritten."""
+    out = np.array(coarse, dtype=np.float64, copy=True)
+    dim = out.ndim
+    ranges = [range(lower[d], upper[d] + 1) for d in range(dim)]
+    import itertools
+
+    for c in itertools.product(*ranges):
+        f = tuple(2 * c[d] - fine_origin[d] for d in range(dim))
+        cl = tuple(c[d] - coarse_origin[d] for d in range(dim))
+        if dual_dir < 0:
+            out[cl] = fine[f]
+        else:
+            f1 = tuple(f[d] + (1 if d == dual_dir else 0) for d in range(dim))
+            out[cl] = 0.5 * (fine[f] + fine[f1])
+    return out
