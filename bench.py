@@ -37,9 +37,9 @@ HBM_PEAK_GBS = 8000.0
 
 # HBM bytes per push dispatch (mode all) from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of
 # this script, FETCH_SIZE scaled x2 and WRITE_SIZE x1 by a 1 GiB device copy's readings; keyed by
-# workload and precision family (k_push_seg / k_push_seg32), measured for the default Harris
-# workload only
-MEASURED_TRAFFIC = {("harris2d", "fp64"): 401.1e6, ("harris2d", "fp32"): 220.9e6}
+# workload and precision family (k_push_seg / k_push_seg32 in 2D, k_push on the 3D staged path)
+MEASURED_TRAFFIC = {("harris2d", "fp64"): 401.1e6, ("harris2d", "fp32"): 220.9e6,
+                    ("uniform3d", "fp64"): 22.45e9}
 
 
 def _hip():

@@ -263,6 +263,30 @@ phx_status phx_reflux(phx_ctx* ctx, double dt);
 enum { PHX_COARSEN_MOMENTS = 0, PHX_COARSEN_ELECTRIC = 1 };
 phx_status phx_coarsen(phx_ctx* coarse, int coarse_field, const phx_ctx* fine, int fine_field, int op,
                        const int lower[3], const int upper[3]);
+/* the refine operators of HybridHybridMessengerStrategy (hybrid_hybrid_messenger_strategy.hpp:
+   87-93, 1130-1136; FieldRefineOperator::refine, src/amr/data/field/refine/
+   field_refine_operator.hpp:75-108), ratio 2: the fine nodes lower..upper (inclusive, fine AMR
+   indices of the quantity's own nodes, the reference's intersection box) of fine_field take values
+   from coarse_field (the same quantity) of the coarser level.
+   PHX_REFINE_LINEAR: DefaultFieldRefiner (field_refiner.hpp:31-166), linear interpolation
+   (LinearWeighter, linear_weighter.cpp), only where the fine node holds NaN (the moment ghosts);
+   PHX_REFINE_LINEAR_OVERWRITE: FieldMomentsRefiner (field_moments_refiner.hpp), the same on every
+   node; PHX_REFINE_ELECTRIC: ElectricFieldRefiner (electric_field_refiner.hpp), E and J, NaN nodes
+   only; PHX_REFINE_MAGNETIC: MagneticFieldRefiner (magnetic_field_refiner.hpp), the fine faces on
+   coarse faces take the coarse face value, NaN nodes only; PHX_REFINE_MAGNETIC_INIT:
+   MagneticFieldInitRefiner (magnetic_field_init_refiner.hpp), the same on every node. The new
+   faces between coarse faces are phx_refine_magnetic_new_faces' work. */
+enum { PHX_REFINE_LINEAR = 0, PHX_REFINE_LINEAR_OVERWRITE = 1, PHX_REFINE_ELECTRIC = 2,
+       PHX_REFINE_MAGNETIC = 3, PHX_REFINE_MAGNETIC_INIT = 4 };
+phx_status phx_refine_field(phx_ctx* fine, int fine_field, const phx_ctx* coarse, int coarse_field,
+                            int op, const int lower[3], const int upper[3]);
+/* MagneticRefinePatchStrategy::postprocessRefine (src/amr/data/field/refine/
+   magnetic_refine_patch_strategy.hpp:67-125): in the fine AMR cell box cell_lower..cell_upper the
+   new faces of the three components of the B field whose x component is b_field_x (PHX_F_B of a
+   set, or PHX_F_BOLD_X) are set from the shared faces (Toth & Roe 2002, divergence-free); every
+   value read must lie inside the arrays (one node below the box, two above) */
+phx_status phx_refine_magnetic_new_faces(phx_ctx* fine, int b_field_x, const int cell_lower[3],
+                                         const int cell_upper[3]);
 /* IonUpdater::updateIons (ion_updater.hpp:111-116) */
 phx_status phx_update_ions(phx_ctx* ctx);
 /* same-level fills of HybridHybridMessengerStrategy (hybrid_hybrid_messenger_strategy.hpp):

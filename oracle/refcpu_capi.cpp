@@ -885,3 +885,30 @@ extern "C" long ora_split(int dim, int order, int nbr, long n, int const* ic, do
     }
     return static_cast<long>(out.size());
 }
+
+// the coarse-to-fine field refinement (refcpu_amr.hpp): fine and coarse arrays in local indices
+// with their shapes (x, y, z; unused directions 1) and AMR origins (AMR index of local index 0);
+// lo/hi: the fine AMR box of the quantity's nodes; primal: its centering per direction
+extern "C" int ora_refine_field(int dim, int op, int const* primal, double* fine, int const* fshape,
+                                int const* forg, double const* coarse, int const* cshape,
+                                int const* corg, int const* lo, int const* hi)
+{
+    refcpu::AmrField F{fine, {fshape[0], fshape[1], fshape[2]}, {forg[0], forg[1], forg[2]}};
+    refcpu::AmrField C{const_cast<double*>(coarse), {cshape[0], cshape[1], cshape[2]},
+                       {corg[0], corg[1], corg[2]}};
+    refcpu::refineField(dim, op, primal, F, C, lo, hi);
+    return 0;
+}
+
+// MagneticRefinePatchStrategy::postprocessRefine over the fine cell box lo..hi: the three B
+// components (shapes sx, sy, sz; one AMR origin) updated in place
+extern "C" int ora_magnetic_new_faces(int dim, double* bx, int const* sx, double* by, int const* sy,
+                                      double* bz, int const* sz, int const* org, double const* dx,
+                                      int const* lo, int const* hi)
+{
+    refcpu::AmrField X{bx, {sx[0], sx[1], sx[2]}, {org[0], org[1], org[2]}};
+    refcpu::AmrField Y{by, {sy[0], sy[1], sy[2]}, {org[0], org[1], org[2]}};
+    refcpu::AmrField Z{bz, {sz[0], sz[1], sz[2]}, {org[0], org[1], org[2]}};
+    refcpu::magneticNewFaces(dim, X, Y, Z, dx, lo, hi);
+    return 0;
+}

@@ -248,6 +248,269 @@ __global__ __launch_bounds__(BLOCK) void k_axpy(double* __restrict__ a, double c
     if (t < n)
         a[t] += b[t] * coef;
 }
+
+// ---- coarse-to-fine field refinement (ratio 2) -------------------------------------------------
+// floor(i / 2) for either sign: the coarse cell or node under fine index i (toCoarseIndex,
+// src/amr/resources_manager/amr_utils.hpp:129-135)
+__device__ __forceinline__ int halfFloor(int i) { return (i - (i < 0 ? 1 : 0)) / 2; }
+__device__ __forceinline__ bool isOdd(int i) { return (i & 1) != 0; }
+
+struct Grid3
+{
+    double const* p;
+    int s1, s2;
+    __device__ __forceinline__ double operator()(int i, int j, int k) const
+    {
+        return p[(static_cast<long>(i) * s1 + j) * s2 + k];
+    }
+};
+
+// linear interpolation weights along one direction for fine index f (FieldRefineIndexesAndWeights
+// with LinearWeighter, field_linear_refine.hpp:60-116, linear_weighter.cpp:6-53): the coarse node
+// to start from and the weights of it and of its right neighbour. A primal fine node on a coarse
+// node takes it whole (the neighbour with weight 0), one between two coarse nodes their mean; a
+// dual fine node lies a quarter of a coarse cell from one coarse dual node and three quarters from
+// the other.
+__device__ __forceinline__ int linearStart(int f, bool primal, double& w0, double& w1)
+{
+    bool const odd = isOdd(f);
+    if (primal)
+    {
+        w0 = odd ? 0.5 : 1.0;
+        w1 = odd ? 0.5 : 0.0;
+        return halfFloor(f);
+    }
+    w0 = odd ? 0.75 : 0.25;
+    w1 = odd ? 0.25 : 0.75;
+    return halfFloor(f - 1);
+}
+
+// DefaultFieldRefiner / FieldMomentsRefiner (field_refiner.hpp:58-161, field_moments_refiner.hpp:
+// 56-160): the tensor-product linear interpolation, summed innermost direction first, every weight
+// applied (a zero weight still multiplies its node)
+template<int D>
+__device__ __forceinline__ double linearValue(RefineArgs const& A, Grid3 const& C, int const* f)
+{
+    int s[3]        = {0, 0, 0};
+    double w[3][2]  = {{1.0, 0.0}, {1.0, 0.0}, {1.0, 0.0}};
+#pragma unroll
+    for (int d = 0; d < D; ++d)
+        s[d] = linearStart(f[d], A.primal[d] != 0, w[d][0], w[d][1]) - A.cOrg[d];
+    double v = 0.;
+    if constexpr (D == 1)
+    {
+        for (int a = 0; a < 2; ++a)
+            v += C(s[0] + a, 0, 0) * w[0][a];
+    }
+    else if constexpr (D == 2)
+    {
+        for (int a = 0; a < 2; ++a)
+        {
+            double y = 0.;
+            for (int b = 0; b < 2; ++b)
+                y += C(s[0] + a, s[1] + b, 0) * w[1][b];
+            v += y * w[0][a];
+        }
+    }
+    else
+    {
+        for (int a = 0; a < 2; ++a)
+        {
+            double y = 0.;
+            for (int b = 0; b < 2; ++b)
+            {
+                double z = 0.;
+                for (int c = 0; c < 2; ++c)
+                    z += C(s[0] + a, s[1] + b, s[2] + c) * w[2][c];
+                y += z * w[1][b];
+            }
+            v += y * w[0][a];
+        }
+    }
+    return v;
+}
+
+// ElectricFieldRefiner (electric_field_refiner.hpp:40-345): along the component's dual direction
+// both fine edges take the coarse edge they halve; along its primal directions a fine edge on a
+// coarse face takes the coarse value, one between two coarse faces their mean (the mean of four
+// when between faces in both primal directions). 1D copies the coarse value for every component.
+template<int D>
+__device__ __forceinline__ double electricValue(RefineArgs const& A, Grid3 const& C, int const* f)
+{
+    int c[3] = {0, 0, 0};
+#pragma unroll
+    for (int d = 0; d < D; ++d)
+        c[d] = halfFloor(f[d]) - A.cOrg[d];
+    if constexpr (D == 1)
+        return C(c[0], 0, 0);
+    int pd[2] = {0, 0};
+    int np    = 0;
+    for (int d = 0; d < D; ++d)
+        if (A.primal[d] && np < 2)
+            pd[np++] = d;
+    // the coarse value shifted by one node along the first (da) / second (db) primal direction
+    auto at = [&](int da, int db) {
+        int i[3] = {c[0], c[1], c[2]};
+        i[pd[0]] += da;
+        i[pd[1]] += db;
+        return C(i[0], i[1], i[2]);
+    };
+    if (np == 1)
+        return isOdd(f[pd[0]]) ? 0.5 * (at(0, 0) + at(1, 0)) : at(0, 0);
+    bool const oa = isOdd(f[pd[0]]);
+    bool const ob = isOdd(f[pd[1]]);
+    if (!oa && !ob)
+        return at(0, 0);
+    if (!oa)
+        return 0.5 * (at(0, 0) + at(0, 1));
+    if (!ob)
+        return 0.5 * (at(0, 0) + at(1, 0));
+    if (D == 3 && pd[0] == 1) // Ex in 3D: the pairs along y summed, then halved, z-pair by z-pair
+        return 0.25 * (at(0, 0) + at(1, 0)) + 0.25 * (at(0, 1) + at(1, 1));
+    return 0.25 * (at(0, 0) + at(1, 0) + at(0, 1) + at(1, 1));
+}
+
+template<int D>
+__global__ __launch_bounds__(BLOCK) void k_refine_field(RefineArgs A)
+{
+    long const t     = static_cast<long>(blockIdx.x) * BLOCK + threadIdx.x;
+    long const total = static_cast<long>(A.n[0]) * A.n[1] * A.n[2];
+    if (t >= total)
+        return;
+    int f[3];
+    f[2] = A.lo[2] + static_cast<int>(t % A.n[2]);
+    f[1] = A.lo[1] + static_cast<int>((t / A.n[2]) % A.n[1]);
+    f[0] = A.lo[0] + static_cast<int>(t / (static_cast<long>(A.n[1]) * A.n[2]));
+    long const fi = (static_cast<long>(f[0] - A.fOrg[0]) * A.fs1 + (f[1] - A.fOrg[1])) * A.fs2
+                    + (f[2] - A.fOrg[2]);
+    bool const nanOnly = A.op != REFINE_LINEAR_OVERWRITE && A.op != REFINE_MAGNETIC_INIT;
+    if (nanOnly && !isnan(A.fine[fi]))
+        return;
+    Grid3 const C{A.coarse, A.cs1, A.cs2};
+    double v;
+    if (A.op == REFINE_LINEAR || A.op == REFINE_LINEAR_OVERWRITE)
+        v = linearValue<D>(A, C, f);
+    else if (A.op == REFINE_ELECTRIC)
+        v = electricValue<D>(A, C, f);
+    else
+    {
+        // MagneticFieldRefiner / MagneticFieldInitRefiner (magnetic_field_refiner.hpp:50-194):
+        // the fine faces on a coarse face take its value; the new faces between two coarse faces
+        // (odd index along the component's primal direction) are left to the patch strategy
+        int c[3] = {0, 0, 0};
+#pragma unroll
+        for (int d = 0; d < D; ++d)
+        {
+            if (A.primal[d] && isOdd(f[d]))
+                return;
+            c[d] = halfFloor(f[d]) - A.cOrg[d];
+        }
+        v = C(c[0], c[1], c[2]);
+    }
+    A.fine[fi] = v;
+}
+
+// index along one direction of a B value around a new face (MagneticRefinePatchStrategy's
+// p_plus / p_minus along the value's primal direction, d_plus / d_minus along its dual ones,
+// magnetic_refine_patch_strategy.hpp:372-376): i - o on the minus side, i + 2 - o (primal) or
+// i + 1 - o (dual) on the plus side
+__device__ __forceinline__ int faceSide(int i, int o, bool primal, int plus)
+{
+    return plus ? i + (primal ? 2 : 1) - o : i - o;
+}
+
+// MagneticRefinePatchStrategy::postprocessRefine (magnetic_refine_patch_strategy.hpp:67-360): the
+// new fine faces of component n (odd AMR index along n), the divergence-free reconstruction of
+// Toth & Roe (2002) from the shared faces: the mean of the two faces along n, plus the cross terms
+// of the other components (2D: one group of 4; 3D: two first-order groups of 8 and two third-order
+// groups of 8 weighted by the mesh sizes)
+template<int D>
+__global__ __launch_bounds__(BLOCK) void k_mag_new_faces(NewFacesArgs A, int n, int e0, int e1, int e2)
+{
+    long const t     = static_cast<long>(blockIdx.x) * BLOCK + threadIdx.x;
+    long const total = static_cast<long>(e0) * e1 * e2;
+    if (t >= total)
+        return;
+    int f[3];
+    f[2] = A.lo[2] + static_cast<int>(t % e2);
+    f[1] = A.lo[1] + static_cast<int>((t / e2) % e1);
+    f[0] = A.lo[0] + static_cast<int>(t / (static_cast<long>(e1) * e2));
+    if (!isOdd(f[n]))
+        return;
+    int loc[3] = {0, 0, 0}, off[3] = {0, 0, 0};
+#pragma unroll
+    for (int d = 0; d < D; ++d)
+    {
+        loc[d] = f[d] - A.org[d];
+        off[d] = (d == n || isOdd(f[d])) ? 1 : 0;
+    }
+    auto at = [&](int comp, int i, int j, int k) {
+        return A.b[comp][(static_cast<long>(i) * A.s1[comp] + j) * A.s2[comp] + k];
+    };
+    // component comp on the side (sx, sy, sz) of the face (0 minus, 1 plus per direction)
+    auto val = [&](int comp, int sx, int sy, int sz) {
+        int const s[3] = {sx, sy, sz};
+        int i[3]       = {0, 0, 0};
+#pragma unroll
+        for (int d = 0; d < D; ++d)
+            i[d] = faceSide(loc[d], off[d], d == comp, s[d]);
+        return at(comp, i[0], i[1], i[2]);
+    };
+    int im[3] = {loc[0], loc[1], loc[2]}, ip[3] = {loc[0], loc[1], loc[2]};
+    im[n] -= 1;
+    ip[n] += 1;
+    double v = 0.5 * (at(n, im[0], im[1], im[2]) + at(n, ip[0], ip[1], ip[2]));
+    if constexpr (D == 2)
+    {
+        double s;
+        if (n == 0) // Bx from By, x-side outer
+            s = val(1, 0, 0, 0) - val(1, 0, 1, 0) - val(1, 1, 0, 0) + val(1, 1, 1, 0);
+        else // By from Bx, y-side outer
+            s = val(0, 0, 0, 0) - val(0, 1, 0, 0) - val(0, 0, 1, 0) + val(0, 1, 1, 0);
+        v = v + 0.25 * s;
+    }
+    else if constexpr (D == 3)
+    {
+        // first-order group of component comp: z-side slowest, then x, then y; a value counts
+        // positive where its sides along n and along comp agree
+        auto first = [&](int comp) {
+            double s = val(comp, 0, 0, 0);
+            for (int k = 1; k < 8; ++k)
+            {
+                int const sd[3] = {(k >> 1) & 1, k & 1, (k >> 2) & 1};
+                double const x  = val(comp, sd[0], sd[1], sd[2]);
+                s               = sd[n] == sd[comp] ? s + x : s - x;
+            }
+            return s;
+        };
+        // third-order group: all plus, one minus (x, y, z), two minus (y z, x z, x y), all minus;
+        // sign (-1)^(number of minus sides)
+        auto third = [&](int comp) {
+            constexpr int ORD[8][3] = {{1, 1, 1}, {0, 1, 1}, {1, 0, 1}, {1, 1, 0},
+                                       {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {0, 0, 0}};
+            double s = val(comp, 1, 1, 1);
+            for (int k = 1; k < 8; ++k)
+            {
+                double const x  = val(comp, ORD[k][0], ORD[k][1], ORD[k][2]);
+                int const minus = 3 - ORD[k][0] - ORD[k][1] - ORD[k][2];
+                s               = (minus & 1) ? s - x : s + x;
+            }
+            return s;
+        };
+        // weight of the third-order group of comp: +-1/8 Dt^2 / (Dn^2 + Dt^2), t the direction
+        // that is neither n nor comp, minus where the face is on the even side along t
+        auto weight = [&](int comp) {
+            int const tt = 3 - n - comp;
+            double const sgn = off[tt] ? 1.0 : -1.0;
+            return 0.125 * sgn * A.dx[tt] * A.dx[tt] / (A.dx[n] * A.dx[n] + A.dx[tt] * A.dx[tt]);
+        };
+        int const lo = n == 0 ? 1 : 0;
+        int const hi = n == 2 ? 1 : 2;
+        int const c1 = (n + 1) % 3, c2 = (n + 2) % 3;
+        v = v + 0.125 * first(lo) + 0.125 * first(hi) + weight(c1) * third(c1) + weight(c2) * third(c2);
+    }
+    A.b[n][(static_cast<long>(loc[0]) * A.s1[n] + loc[1]) * A.s2[n] + loc[2]] = v;
+}
 } // namespace
 
 void launchCoarsen(double const* fine, int const* fshape, double* coarse, int const* cshape,
@@ -260,6 +523,40 @@ void launchCoarsen(double const* fine, int const* fshape, double* coarse, int co
     k_coarsen<<<nb, BLOCK, 0, s>>>(fine, fshape[1], fshape[2], coarse, cshape[1], cshape[2], clo[0],
                                    clo[1], clo[2], n[0], n[1], n[2], fineOff[0], fineOff[1], fineOff[2],
                                    dualDir, dim);
+}
+
+void launchRefineField(RefineArgs const& A, hipStream_t s)
+{
+    long const total = static_cast<long>(A.n[0]) * A.n[1] * A.n[2];
+    if (total <= 0)
+        return;
+    unsigned int const nb = static_cast<unsigned int>((total + BLOCK - 1) / BLOCK);
+    switch (A.dim)
+    {
+        case 1: k_refine_field<1><<<nb, BLOCK, 0, s>>>(A); break;
+        case 2: k_refine_field<2><<<nb, BLOCK, 0, s>>>(A); break;
+        case 3: k_refine_field<3><<<nb, BLOCK, 0, s>>>(A); break;
+    }
+}
+
+void launchMagneticNewFaces(NewFacesArgs const& A, hipStream_t s)
+{
+    for (int n = 0; n < A.dim; ++n)
+    {
+        int e[3] = {1, 1, 1};
+        for (int d = 0; d < A.dim; ++d)
+            e[d] = A.hi[d] - A.lo[d] + 1 + (d == n ? 1 : 0);
+        long const total = static_cast<long>(e[0]) * e[1] * e[2];
+        if (total <= 0)
+            continue;
+        unsigned int const nb = static_cast<unsigned int>((total + BLOCK - 1) / BLOCK);
+        switch (A.dim)
+        {
+            case 1: k_mag_new_faces<1><<<nb, BLOCK, 0, s>>>(A, n, e[0], e[1], e[2]); break;
+            case 2: k_mag_new_faces<2><<<nb, BLOCK, 0, s>>>(A, n, e[0], e[1], e[2]); break;
+            case 3: k_mag_new_faces<3><<<nb, BLOCK, 0, s>>>(A, n, e[0], e[1], e[2]); break;
+        }
+    }
 }
 
 void launchAxpy(double* a, double const* b, double coef, long n, hipStream_t s)

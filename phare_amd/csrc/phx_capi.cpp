@@ -2418,6 +2418,131 @@ phx_status phx_coarsen(phx_ctx* coarse, int coarse_field, phx_ctx const* fine, i
     return checkLaunch(coarse, "phx_coarsen");
 }
 
+namespace
+{
+int floorHalf(int i) { return (i - (i < 0 ? 1 : 0)) / 2; }
+} // namespace
+
+phx_status phx_refine_field(phx_ctx* fine, int fine_field, phx_ctx const* coarse, int coarse_field,
+                            int op, int const lower[3], int const upper[3])
+{
+    if (!fine || !coarse || !lower || !upper)
+        return fine ? fail(fine, PHX_ERR_ARG, "phx_refine_field: null argument") : PHX_ERR_ARG;
+    if (op < PHX_REFINE_LINEAR || op > PHX_REFINE_MAGNETIC_INIT)
+        return fail(fine, PHX_ERR_ARG, "phx_refine_field: bad operator");
+    double *pf, *pc;
+    int qf, qc;
+    if (!fieldOf(fine, fine_field, pf, qf) || !fieldOf(coarse, coarse_field, pc, qc) || qf != qc)
+        return fail(fine, PHX_ERR_ARG, "phx_refine_field: bad field ids (one quantity on both levels)");
+    int const D = fine->cfg.dim;
+    if (coarse->cfg.dim != D)
+        return fail(fine, PHX_ERR_ARG, "phx_refine_field: the levels differ in dimension");
+    bool const isE = qf >= QEX && qf <= QJZ;
+    bool const isB = qf >= QBX && qf <= QBZ;
+    if (op == PHX_REFINE_ELECTRIC && !isE)
+        return fail(fine, PHX_ERR_ARG, "phx_refine_field: the electric operator takes E-centered "
+                                       "quantities (E, J)");
+    if ((op == PHX_REFINE_MAGNETIC || op == PHX_REFINE_MAGNETIC_INIT) && !isB)
+        return fail(fine, PHX_ERR_ARG, "phx_refine_field: the magnetic operators take B");
+    RefineArgs A{};
+    A.fine   = pf;
+    A.coarse = pc;
+    A.fs1    = fine->G.shape[qf][1];
+    A.fs2    = fine->G.shape[qf][2];
+    A.cs1    = coarse->G.shape[qc][1];
+    A.cs2    = coarse->G.shape[qc][2];
+    A.op     = op;
+    A.dim    = D;
+    for (int d = 0; d < 3; ++d)
+    {
+        A.primal[d] = centering(qf, d) == CP ? 1 : 0;
+        if (d >= D)
+        {
+            A.fOrg[d] = A.cOrg[d] = A.lo[d] = 0;
+            A.n[d]                         = 1;
+            continue;
+        }
+        A.fOrg[d] = fine->G.lower[d] - fine->G.g;
+        A.cOrg[d] = coarse->G.lower[d] - coarse->G.g;
+        A.lo[d]   = lower[d];
+        A.n[d]    = upper[d] - lower[d] + 1;
+        if (A.n[d] < 1 || lower[d] - A.fOrg[d] < 0 || upper[d] - A.fOrg[d] >= fine->G.shape[qf][d])
+            return fail(fine, PHX_ERR_ARG, "phx_refine_field: the box leaves the fine field");
+        // the coarse nodes the box reads
+        bool const p = A.primal[d] != 0;
+        int cmin, cmax;
+        if (op == PHX_REFINE_LINEAR || op == PHX_REFINE_LINEAR_OVERWRITE)
+        {
+            cmin = p ? floorHalf(lower[d]) : floorHalf(lower[d] - 1);
+            cmax = (p ? floorHalf(upper[d]) : floorHalf(upper[d] - 1)) + 1;
+        }
+        else
+        {
+            cmin = floorHalf(lower[d]);
+            cmax = op == PHX_REFINE_ELECTRIC && p && D >= 2 ? floorHalf(upper[d] + 1)
+                                                           : floorHalf(upper[d]);
+        }
+        if (cmin - A.cOrg[d] < 0 || cmax - A.cOrg[d] >= coarse->G.shape[qc][d])
+            return fail(fine, PHX_ERR_ARG, "phx_refine_field: the box reads beyond the coarse field");
+    }
+    if (coarse != fine)
+        PHX_HIP(fine, hipStreamSynchronize(coarse->stream));
+    launchRefineField(A, fine->stream);
+    return checkLaunch(fine, "phx_refine_field");
+}
+
+phx_status phx_refine_magnetic_new_faces(phx_ctx* fine, int b_field_x, int const cell_lower[3],
+                                         int const cell_upper[3])
+{
+    if (!fine || !cell_lower || !cell_upper)
+        return fine ? fail(fine, PHX_ERR_ARG, "phx_refine_magnetic_new_faces: null argument")
+                    : PHX_ERR_ARG;
+    NewFacesArgs A{};
+    for (int k = 0; k < 3; ++k)
+    {
+        int q;
+        if (!fieldOf(fine, b_field_x + k, A.b[k], q) || q != QBX + k)
+            return fail(fine, PHX_ERR_ARG,
+                        "phx_refine_magnetic_new_faces: b_field_x must name the x component of a "
+                        "magnetic field (B of a set, or Bold)");
+        A.s1[k] = fine->G.shape[QBX + k][1];
+        A.s2[k] = fine->G.shape[QBX + k][2];
+    }
+    int const D = fine->cfg.dim;
+    A.dim       = D;
+    for (int d = 0; d < 3; ++d)
+    {
+        A.dx[d] = fine->G.dx[d];
+        if (d >= D)
+        {
+            A.org[d] = A.lo[d] = A.hi[d] = 0;
+            continue;
+        }
+        A.org[d] = fine->G.lower[d] - fine->G.g;
+        A.lo[d]  = cell_lower[d];
+        A.hi[d]  = cell_upper[d];
+        if (A.hi[d] < A.lo[d])
+            return fail(fine, PHX_ERR_ARG, "phx_refine_magnetic_new_faces: empty box");
+    }
+    // every value a new face reads lies within [f - 1, f + 2] along each direction (f + 1 at
+    // most along the dual directions of the value read and along the face's normal)
+    for (int n = 0; n < D; ++n)
+        for (int a = 0; a < (D == 1 ? 1 : 3); ++a)
+        {
+            if (D == 2 && a == 2)
+                continue;
+            for (int d = 0; d < D; ++d)
+            {
+                int const need = (d == n || d == a) ? 2 : 1;
+                if (A.lo[d] - A.org[d] - 1 < 0 || A.hi[d] - A.org[d] + need >= fine->G.shape[QBX + a][d])
+                    return fail(fine, PHX_ERR_ARG, "phx_refine_magnetic_new_faces: the reconstruction "
+                                                   "stencil leaves the B arrays");
+            }
+        }
+    launchMagneticNewFaces(A, fine->stream);
+    return checkLaunch(fine, "phx_refine_magnetic_new_faces");
+}
+
 phx_status phx_update_ions(phx_ctx* c)
 {
     if (!c)

@@ -266,6 +266,41 @@ void launchCoarsen(double const* fine, int const* fshape, double* coarse, int co
 // a += b * coef over n values
 void launchAxpy(double* a, double const* b, double coef, long n, hipStream_t s);
 
+// coarse-to-fine refinement of one field (ratio 2) over the fine AMR box lo .. lo + n - 1 of the
+// quantity's own nodes. Arrays row-major (x slowest); org: the AMR index of local index 0 per
+// direction; primal[d]: the quantity's centering per direction
+enum RefineOp
+{
+    REFINE_LINEAR           = 0, // DefaultFieldRefiner: linear weights, NaN nodes only
+    REFINE_LINEAR_OVERWRITE = 1, // FieldMomentsRefiner: linear weights, every node
+    REFINE_ELECTRIC         = 2, // ElectricFieldRefiner (NaN nodes only)
+    REFINE_MAGNETIC         = 3, // MagneticFieldRefiner: shared faces, NaN nodes only
+    REFINE_MAGNETIC_INIT    = 4  // MagneticFieldInitRefiner: shared faces, every node
+};
+struct RefineArgs
+{
+    double* fine;
+    double const* coarse;
+    int fs1, fs2, cs1, cs2; // strides of the two slower directions' successors (shape[1], shape[2])
+    int fOrg[3], cOrg[3];
+    int lo[3], n[3];
+    int primal[3];
+    int op, dim;
+};
+void launchRefineField(RefineArgs const& A, hipStream_t s);
+// the new fine faces of B inside the fine AMR cell box lo .. hi (the faces with an odd index along
+// their normal), from the shared faces already refined (MagneticRefinePatchStrategy)
+struct NewFacesArgs
+{
+    double* b[3];
+    int s1[3], s2[3]; // shape[1], shape[2] of each component's array
+    int org[3];
+    int lo[3], hi[3];
+    double dx[3];
+    int dim;
+};
+void launchMagneticNewFaces(NewFacesArgs const& A, hipStream_t s);
+
 // ---- fields ----------------------------------------------------------------------------------
 struct VecPtr
 {

@@ -19,6 +19,7 @@ F_E, F_B, F_EPRED, F_BPRED, F_EAVG, F_BAVG = 0, 3, 6, 9, 12, 15
 F_J, F_VE, F_PE, F_RHO, F_MASSD, F_VI, F_POP0 = 18, 21, 24, 25, 26, 27, 32
 F_BOLD, F_FLUXSUM = 64, 67
 COARSEN_MOMENTS, COARSEN_ELECTRIC = 0, 1
+REFINE_LINEAR, REFINE_LINEAR_OVERWRITE, REFINE_ELECTRIC, REFINE_MAGNETIC, REFINE_MAGNETIC_INIT = range(5)
 MAX_POP = 4
 
 
@@ -121,6 +122,8 @@ def lib():
         L.phx_accumulate_flux_sum.argtypes = [vp, C.c_double]
         L.phx_reflux.argtypes = [vp, C.c_double]
         L.phx_coarsen.argtypes = [vp, C.c_int, vp, C.c_int, C.c_int, _ip, _ip]
+        L.phx_refine_field.argtypes = [vp, C.c_int, vp, C.c_int, C.c_int, _ip, _ip]
+        L.phx_refine_magnetic_new_faces.argtypes = [vp, C.c_int, _ip, _ip]
         L.phx_get_tuning.argtypes = [vp, C.POINTER(Tuning)]
         L.phx_set_tuning.argtypes = [vp, C.POINTER(Tuning)]
         _lib = L
@@ -141,7 +144,8 @@ EXPORTED = ["phx_create", "phx_destroy", "phx_last_error", "phx_synchronize", "p
             "phx_get_patch_particles", "phx_add_particles", "phx_gather_noalloc",
             "phx_get_tuning", "phx_set_tuning", "phx_refine_particles",
             "phx_level_ghost_last_step", "phx_time_interpolate", "phx_prepare_step",
-            "phx_reset_flux_sum", "phx_accumulate_flux_sum", "phx_reflux", "phx_coarsen"]
+            "phx_reset_flux_sum", "phx_accumulate_flux_sum", "phx_reflux", "phx_coarsen",
+            "phx_refine_field", "phx_refine_magnetic_new_faces"]
 
 
 class Level:
@@ -468,6 +472,23 @@ class Level:
         lo[:self.dim] = lower
         hi[:self.dim] = upper
         self._rc(lib().phx_coarsen(self.h, fid, fine.h, fine_fid, op, lo, hi))
+
+    def refine_field(self, fid, coarse, coarse_fid, op, lower, upper):
+        """phx_refine_field: this (fine) level's field fid over its fine AMR nodes lower..upper
+        from the coarser level's field coarse_fid"""
+        lo = np.zeros(3, np.int32)
+        hi = np.zeros(3, np.int32)
+        lo[:self.dim] = lower
+        hi[:self.dim] = upper
+        self._rc(lib().phx_refine_field(self.h, fid, coarse.h, coarse_fid, op, lo, hi))
+
+    def refine_magnetic_new_faces(self, b_fid, lower, upper):
+        """phx_refine_magnetic_new_faces over the fine AMR cell box lower..upper"""
+        lo = np.zeros(3, np.int32)
+        hi = np.zeros(3, np.int32)
+        lo[:self.dim] = lower
+        hi[:self.dim] = upper
+        self._rc(lib().phx_refine_magnetic_new_faces(self.h, b_fid, lo, hi))
 
     def deposit(self, pop=0, kind=DOMAIN, coef=1.0):
         self._rc(lib().phx_deposit(self.h, pop, kind, coef))

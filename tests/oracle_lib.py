@@ -98,6 +98,9 @@ def lib():
         L.ora_partition.argtypes = [C.c_int, _ip, _ip, C.c_long, _ip, _ip, _ip, _ip, _lp]
         L.ora_boris_trajectory.argtypes = [C.c_int, C.c_long, C.c_double, _dp, C.c_double, C.c_double,
                                            _dp, _dp, _ip, _dp, _dp, _dp]
+        L.ora_refine_field.argtypes = [C.c_int, C.c_int, _ip, _dp, _ip, _ip, _dp, _ip, _ip, _ip, _ip]
+        L.ora_magnetic_new_faces.argtypes = [C.c_int, _dp, _ip, _dp, _ip, _dp, _ip, _ip, _dp, _ip,
+                                             _ip]
         L.ora_split.restype = C.c_long
         L.ora_split.argtypes = [C.c_int, C.c_int, C.c_int, C.c_long, _ip, _dp, _dp, _dp, _dp, C.c_int,
                                 _ip, _ip, C.c_long, _ip, _dp, _dp, _dp, _dp]
@@ -387,4 +390,40 @@ def coarsen(fine, fine_origin, coarse, coarse_origin, lower, upper, dual_dir):
         else:
             f1 = tuple(f[d] + (1 if d == dual_dir else 0) for d in range(dim))
             out[cl] = 0.5 * (fine[f] + fine[f1])
+    return out
+
+
+# refine operators of the oracle (values of include/phx.h's PHX_REFINE_*)
+REFINE_LINEAR, REFINE_LINEAR_OVERWRITE, REFINE_ELECTRIC, REFINE_MAGNETIC, REFINE_MAGNETIC_INIT = range(5)
+
+
+def _shape3(a):
+    return _i3(list(a.shape) + [1] * (3 - a.ndim))
+
+
+def refine_field(op, primal, fine, fine_origin, coarse, coarse_origin, lower, upper):
+    """the coarse-to-fine refinement of one field, ratio 2 (oracle/refcpu_amr.hpp: the reference's
+    DefaultFieldRefiner, FieldMomentsRefiner, ElectricFieldRefiner, MagneticFieldRefiner and
+    MagneticFieldInitRefiner). Arrays in local indices; *_origin: the AMR index of local index 0;
+    primal: the quantity's centering flags; lower/upper: fine AMR nodes of the quantity
+    (inclusive). Returns a copy of `fine` with the box refined."""
+    out = np.array(fine, dtype=np.float64, copy=True, order="C")
+    dim = out.ndim
+    lib().ora_refine_field(dim, op, _i3([1 if p else 0 for p in primal]), out.reshape(-1),
+                           _shape3(out), _i3(fine_origin),
+                           np.ascontiguousarray(coarse, np.float64).reshape(-1), _shape3(coarse),
+                           _i3(coarse_origin), _i3(lower), _i3(upper))
+    return out
+
+
+def magnetic_new_faces(B, origin, dx, lower, upper):
+    """MagneticRefinePatchStrategy::postprocessRefine (oracle/refcpu_amr.hpp) over the fine cell
+    box lower..upper (AMR, inclusive): copies of the three B components with the new faces set"""
+    out = [np.array(b, dtype=np.float64, copy=True, order="C") for b in B]
+    dim = out[0].ndim
+    d = np.zeros(3)
+    d[:dim] = dx[:dim]
+    lib().ora_magnetic_new_faces(dim, out[0].reshape(-1), _shape3(out[0]), out[1].reshape(-1),
+                                 _shape3(out[1]), out[2].reshape(-1), _shape3(out[2]), _i3(origin),
+                                 d, _i3(lower), _i3(upper))
     return out
