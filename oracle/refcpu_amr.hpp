@@ -342,7 +342,16 @@ inline void magneticNewFaces(int dim, AmrField const& bx, AmrField const& by, Am
     auto dp = [](int i, int o) { return i + 1 - o; };
     auto dm = [](int i, int o) { return i - o; };
     auto odd = [](int i) { return i % 2 != 0; };
-    double const ijk[2] = {-1., 1.};
+    // ORA_PLANT_*: mistakes planted by the tests to show that their checks catch them (the
+    // defaults are the reference's values and leave every result bit-identical)
+#ifndef ORA_PLANT_IJK
+#define ORA_PLANT_IJK -1., 1.
+#endif
+#ifndef ORA_PLANT_CROSS
+#define ORA_PLANT_CROSS 1.
+#endif
+    double const ijk[2] = {ORA_PLANT_IJK};
+    double const cross  = ORA_PLANT_CROSS;
     double const Dx = meshSize[0], Dy = dim > 1 ? meshSize[1] : 0., Dz = dim > 2 ? meshSize[2] : 0.;
     // the field box of a component: the cell box with one more node along its primal direction
     auto box = [&](int comp, int d, int& a, int& b) {
@@ -373,7 +382,7 @@ inline void magneticNewFaces(int dim, AmrField const& bx, AmrField const& by, Am
                         {
                             int const xo = 1, yo = odd(iy) ? 1 : 0;
                             bx(ix, iy) = 0.5 * (bx(ix - 1, iy) + bx(ix + 1, iy))
-                                         + 0.25
+                                         + cross * 0.25
                                                * (by(dm(ix, xo), pm(iy, yo)) - by(dm(ix, xo), pp(iy, yo))
                                                   - by(dp(ix, xo), pm(iy, yo)) + by(dp(ix, xo), pp(iy, yo)));
                         }
@@ -381,7 +390,7 @@ inline void magneticNewFaces(int dim, AmrField const& bx, AmrField const& by, Am
                         {
                             int const xo = odd(ix) ? 1 : 0, yo = 1;
                             by(ix, iy) = 0.5 * (by(ix, iy - 1) + by(ix, iy + 1))
-                                         + 0.25
+                                         + cross * 0.25
                                                * (bx(pm(ix, xo), dm(iy, yo)) - bx(pp(ix, xo), dm(iy, yo))
                                                   - bx(pm(ix, xo), dp(iy, yo)) + bx(pp(ix, xo), dp(iy, yo)));
                         }
@@ -402,11 +411,11 @@ inline void magneticNewFaces(int dim, AmrField const& bx, AmrField const& by, Am
                     {
                         bx(ix, iy, iz)
                             = 0.5 * (bx(ix - 1, iy, iz) + bx(ix + 1, iy, iz))
-                              + 0.125
+                              + cross * 0.125
                                     * (V(by, 1, -1, -1, -1) - V(by, 1, -1, +1, -1) - V(by, 1, +1, -1, -1)
                                        + V(by, 1, +1, +1, -1) + V(by, 1, -1, -1, +1) - V(by, 1, -1, +1, +1)
                                        - V(by, 1, +1, -1, +1) + V(by, 1, +1, +1, +1))
-                              + 0.125
+                              + cross * 0.125
                                     * (V(bz, 2, -1, -1, -1) + V(bz, 2, -1, +1, -1) - V(bz, 2, +1, -1, -1)
                                        - V(bz, 2, +1, +1, -1) - V(bz, 2, -1, -1, +1) - V(bz, 2, -1, +1, +1)
                                        + V(bz, 2, +1, -1, +1) + V(bz, 2, +1, +1, +1))
@@ -423,11 +432,11 @@ inline void magneticNewFaces(int dim, AmrField const& bx, AmrField const& by, Am
                     {
                         by(ix, iy, iz)
                             = 0.5 * (by(ix, iy - 1, iz) + by(ix, iy + 1, iz))
-                              + 0.125
+                              + cross * 0.125
                                     * (V(bx, 0, -1, -1, -1) - V(bx, 0, -1, +1, -1) - V(bx, 0, +1, -1, -1)
                                        + V(bx, 0, +1, +1, -1) + V(bx, 0, -1, -1, +1) - V(bx, 0, -1, +1, +1)
                                        - V(bx, 0, +1, -1, +1) + V(bx, 0, +1, +1, +1))
-                              + 0.125
+                              + cross * 0.125
                                     * (V(bz, 2, -1, -1, -1) - V(bz, 2, -1, +1, -1) + V(bz, 2, +1, -1, -1)
                                        - V(bz, 2, +1, +1, -1) - V(bz, 2, -1, -1, +1) + V(bz, 2, -1, +1, +1)
                                        - V(bz, 2, +1, -1, +1) + V(bz, 2, +1, +1, +1))
@@ -444,11 +453,11 @@ inline void magneticNewFaces(int dim, AmrField const& bx, AmrField const& by, Am
                     {
                         bz(ix, iy, iz)
                             = 0.5 * (bz(ix, iy, iz - 1) + bz(ix, iy, iz + 1))
-                              + 0.125
+                              + cross * 0.125
                                     * (V(bx, 0, -1, -1, -1) + V(bx, 0, -1, +1, -1) - V(bx, 0, +1, -1, -1)
                                        - V(bx, 0, +1, +1, -1) - V(bx, 0, -1, -1, +1) - V(bx, 0, -1, +1, +1)
                                        + V(bx, 0, +1, -1, +1) + V(bx, 0, +1, +1, +1))
-                              + 0.125
+                              + cross * 0.125
                                     * (V(by, 1, -1, -1, -1) - V(by, 1, -1, +1, -1) + V(by, 1, +1, -1, -1)
                                        - V(by, 1, +1, +1, -1) - V(by, 1, -1, -1, +1) + V(by, 1, -1, +1, +1)
                                        - V(by, 1, +1, -1, +1) + V(by, 1, +1, +1, +1))

@@ -416,14 +416,30 @@ def refine_field(op, primal, fine, fine_origin, coarse, coarse_origin, lower, up
     return out
 
 
-def magnetic_new_faces(B, origin, dx, lower, upper):
+def planted_oracle(defines, tmpdir):
+    """the oracle compiled with mistakes planted through its ORA_PLANT_* macros (defines: name ->
+    value), loaded from tmpdir; for tests that show their checks catch those mistakes"""
+    import subprocess
+
+    so = os.path.join(str(tmpdir), "liboracle_planted.so")
+    cmd = ["g++", "-std=c++20", "-O1", "-fPIC", "-ffp-contract=off", "-shared", "-o", so,
+           os.path.join(ROOT, "oracle", "refcpu_capi.cpp")]
+    cmd += [f"-D{k}={v}" for k, v in defines.items()]
+    subprocess.run(cmd, check=True)
+    L = C.CDLL(so)
+    L.ora_magnetic_new_faces.argtypes = [C.c_int, _dp, _ip, _dp, _ip, _dp, _ip, _ip, _dp, _ip, _ip]
+    return L
+
+
+def magnetic_new_faces(B, origin, dx, lower, upper, oracle=None):
     """MagneticRefinePatchStrategy::postprocessRefine (oracle/refcpu_amr.hpp) over the fine cell
-    box lower..upper (AMR, inclusive): copies of the three B components with the new faces set"""
+    box lower..upper (AMR, inclusive): copies of the three B components with the new faces set
+    (oracle: a planted_oracle handle instead of the oracle)"""
     out = [np.array(b, dtype=np.float64, copy=True, order="C") for b in B]
     dim = out[0].ndim
     d = np.zeros(3)
     d[:dim] = dx[:dim]
-    lib().ora_magnetic_new_faces(dim, out[0].reshape(-1), _shape3(out[0]), out[1].reshape(-1),
+    (oracle or lib()).ora_magnetic_new_faces(dim, out[0].reshape(-1), _shape3(out[0]), out[1].reshape(-1),
                                  _shape3(out[1]), out[2].reshape(-1), _shape3(out[2]), _i3(origin),
                                  d, _i3(lower), _i3(upper))
     return out

@@ -187,6 +187,44 @@ def _fine_div(B, lo_loc, hi_loc, h, dim):
     return div
 
 
+def _rebuilt_divergence(dim, h, oracle=None):
+    """max |div B| of the fine cells after the new faces of a divergence-free fine field (equal or
+    unequal mesh sizes h) were erased and rebuilt, and the field's scale"""
+    rng = np.random.default_rng(31 + dim)
+    cl, cn, fl, fn = _levels(dim)
+    hf = [x / 2 for x in h]
+    clo = [fl[d] + 2 for d in range(dim)]
+    chi = [fl[d] + fn[d] - 3 for d in range(dim)]
+    lo_loc = [clo[d] - fl[d] + G for d in range(dim)]
+    hi_loc = [chi[d] - fl[d] + G for d in range(dim)]
+    Bfine = _curl_B(dim, fn, rng, hf)
+    erased = [b.copy() for b in Bfine]
+    for c in range(dim):
+        for loc in itertools.product(*[range(s) for s in erased[c].shape]):
+            inside = all(lo_loc[d] <= loc[d] <= hi_loc[d] + (1 if d == c else 0) for d in range(dim))
+            if inside and (loc[c] - G + fl[c]) % 2:
+                erased[c][loc] = np.nan
+    rebuilt = O.magnetic_new_faces(erased, [a - G for a in fl], hf, clo, chi, oracle=oracle)
+    div = _fine_div(rebuilt, lo_loc, hi_loc, hf, dim)
+    return np.max(np.abs(div)), max(np.max(np.abs(b)) for b in Bfine) / min(hf)
+
+
+@pytest.mark.parametrize("plant", [{"ORA_PLANT_IJK": "1.,-1."}, {"ORA_PLANT_IJK": "0.,0."},
+                                   {"ORA_PLANT_CROSS": "0."}, {"ORA_PLANT_CROSS": "-1."}])
+def test_new_faces_check_catches_planted_mistakes(plant, tmp_path):
+    """the divergence check above fails for an oracle whose new-face reconstruction has the
+    third-order terms of the wrong sign or none, or the first-order cross terms dropped or
+    flipped (3D; 2D for the cross terms)"""
+    L = O.planted_oracle(plant, tmp_path)
+    dims = [3] if "ORA_PLANT_IJK" in plant else [2, 3]
+    for dim in dims:
+        h = (1.0,) * dim
+        good, scale = _rebuilt_divergence(dim, h)
+        bad, _ = _rebuilt_divergence(dim, h, oracle=L)
+        assert good <= 1e-13 * scale
+        assert bad > 1e-3 * scale, (plant, dim, bad)
+
+
 @pytest.mark.parametrize("dim,h", [(2, (1.0, 1.0)), (2, (1.0, 0.5)), (3, (1.0, 1.0, 1.0)),
                                    (3, (1.0, 0.75, 0.5))])
 def test_new_faces_keep_B_divergence_free(dim, h):
