@@ -30,7 +30,7 @@
 #define PHX_TILE_COPIES_2D 8
 #endif
 #ifndef PHX_TILE_COPIES_3D
-#define PHX_TILE_COPIES_3D 2
+#define PHX_TILE_COPIES_3D 1
 #endif
 // moment copies of the segmented push (its lanes own distinct cells)
 #ifndef PHX_SEG_COPIES
@@ -818,6 +818,48 @@ std::vector<size_t> chunkInterleave(size_t n, int dim, int order)
     return ord;
 }
 
+// 3D particles (cells ic[3 n], AMR) in brick order: by brick of BRICK^3 cells on the grid of
+// bricks from the patch's lower cell (row-major), then by cell inside the brick (row-major), the
+// caller's order kept among particles of one cell (a counting sort). Particles beyond the bricks
+// that cover the patch and its ghost layers go last, in the caller's order. With 64 particles per
+// cell every chunk of the linear push is one brick, and at each step the 64 lanes of a wavefront
+// hold particles of 64 different cells.
+std::vector<size_t> brickOrder(Geom const& G, int32_t const* ic, size_t n)
+{
+    int nb[3];
+    for (int d = 0; d < 3; ++d)
+        nb[d] = (G.n[d] + BRICK - 1) / BRICK + 2; // one brick of ghost cells on each side
+    size_t const bricks = static_cast<size_t>(nb[0]) * nb[1] * nb[2];
+    size_t const keys   = bricks * BRICK * BRICK * BRICK + 1; // the last key: beyond the bricks
+    auto key = [&](size_t i) -> size_t {
+        size_t b = 0, cib = 0;
+        for (int d = 0; d < 3; ++d)
+        {
+            int const r  = ic[i * 3 + d] - G.lower[d];
+            int const bd = (r >= 0 ? r / BRICK : -((BRICK - 1 - r) / BRICK)) + 1;
+            if (bd < 0 || bd >= nb[d])
+                return keys - 1;
+            b   = b * nb[d] + static_cast<size_t>(bd);
+            cib = cib * BRICK + static_cast<size_t>(r - (bd - 1) * BRICK);
+        }
+        return b * BRICK * BRICK * BRICK + cib;
+    };
+    std::vector<size_t> count(keys + 1, 0);
+    std::vector<uint32_t> k(n);
+    for (size_t i = 0; i < n; ++i)
+    {
+        size_t const kk = key(i);
+        k[i]            = static_cast<uint32_t>(kk);
+        ++count[kk + 1];
+    }
+    for (size_t j = 1; j <= keys; ++j)
+        count[j] += count[j - 1];
+    std::vector<size_t> ord(n);
+    for (size_t i = 0; i < n; ++i)
+        ord[count[k[i]]++] = i;
+    return ord;
+}
+
 // FP64 host values into a device array of the store's precision
 phx_status putReals(phx_ctx* c, Store const& s, void* dst, size_t offset, std::vector<double> const& src)
 {
@@ -881,9 +923,16 @@ phx_status upload(phx_ctx* c, Pop& P, Store& s, size_t n, int32_t const* ic, dou
     // the copies below run on the null stream, which does not wait for the context's
     // non-blocking stream: nothing queued there may still use the array
     PHX_HIP(c, hipStreamSynchronize(c->stream));
-    // appended particles start a fresh chunk alignment only when the array was empty
-    std::vector<size_t> const ord = offset == 0 ? chunkInterleave(n, D, c->cfg.interp_order)
-                                                : chunkInterleave(0, D, c->cfg.interp_order);
+    // appended particles start a fresh chunk alignment only when the array was empty; in 3D the
+    // particles of an empty array are put in brick order first (the linear push's brick tiles)
+    std::vector<size_t> ord = offset == 0 ? chunkInterleave(n, D, c->cfg.interp_order)
+                                          : chunkInterleave(0, D, c->cfg.interp_order);
+    if (offset == 0 && D == 3 && n)
+    {
+        std::vector<size_t> const bo = brickOrder(c->G, ic, n);
+        for (size_t i = 0; i < n; ++i)
+            ord[i] = bo[ord[i]];
+    }
     std::vector<size_t> identity;
     if (offset != 0)
     {

@@ -70,7 +70,12 @@ struct TileSpec
     int copies; // moment copies (power of two): lane l accumulates into copy l % copies
     int stride;
     unsigned int bytes; // dynamic LDS of the launch
+    int brick;          // > 0: the tile covers the brick (cube of `brick` cells, on the grid of
+                        // bricks from the patch's lower cell) of the chunk's first particle
 };
+// cells per side of the bricks of the linear 3D push: brick-ordered arrays put the particles of
+// one brick (BRICK^3 = 64 cells, one per lane of a wavefront) in one chunk
+constexpr int BRICK = 4;
 
 // Cell-segmented storage of a level's domain array: every patch of the level (a device patch
 // table, patches of uneven sizes) is cut into segments, boxes of at most 64 cells (shape `seg`),

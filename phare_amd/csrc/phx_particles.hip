@@ -23,7 +23,10 @@
 #include <mutex>
 
 #ifndef PHX_PPT_ND
-#define PHX_PPT_ND 2 // particles per thread of the linear 2D/3D push
+#define PHX_PPT_ND 2 // particles per thread of the linear 2D push
+#endif
+#ifndef PHX_PPT_3D
+#define PHX_PPT_3D 16 // particles per thread of the linear 3D push: a chunk of 4096 is one brick
 #endif
 
 namespace phx
@@ -47,7 +50,7 @@ template<int D, int O>
 struct Cfg
 {
     static constexpr bool ACC  = D == 1;
-    static constexpr int PPT   = ACC ? 8 : PHX_PPT_ND;
+    static constexpr int PPT   = ACC ? 8 : (D == 3 ? PHX_PPT_3D : PHX_PPT_ND);
     static constexpr int CHUNK = BLOCK * PPT;
     static constexpr int W     = O == 1 ? 2 : 4;  // window width per direction
     static constexpr int WB    = O == 1 ? 0 : 1;  // window base = local cell - WB
@@ -572,19 +575,27 @@ __global__ __launch_bounds__(BLOCK) void k_push(PushArgs A, PView in, PView out)
     zeroTile(lds, A.tile);
     // the tile is placed around the cell of the chunk's first particle (chunks of a cell-ordered
     // array hold neighbouring cells), with host-chosen margins for the chunk's extent, one cell
-    // of motion and the stencil; a dead first particle leaves the tile out of reach
+    // of motion and the stencil; brick tiles (3D, brick-ordered arrays) around the brick of that
+    // cell; a dead first particle leaves the tile out of reach
     TileView T;
     tileView(T, lds, A.tile);
     int const c0   = in.ic[0][base];
     bool const far = c0 == DEAD_CELL;
+    int const bk   = A.tile.brick;
 #pragma unroll
     for (int d = 0; d < D; ++d)
-        T.org[d] = far ? -0x20000000 : in.ic[d][base] - (G.lower[d] - G.g) - A.tile.lo[d];
+    {
+        int first = in.ic[d][base] - G.lower[d];
+        if (bk > 0)
+            first = (first >= 0 ? first / bk : -((bk - 1 - first) / bk)) * bk;
+        T.org[d] = far ? -0x20000000 : first + G.g - A.tile.lo[d];
+    }
     __syncthreads();
 
     Acc<D, O> acc;
     if constexpr (C::ACC)
         accZero(acc);
+#pragma unroll 1
     for (int k = 0; k < PPT; ++k)
     {
         unsigned long long const i = base + chunkOffset(t, k);
@@ -1277,6 +1288,24 @@ TileSpec tileShape(int dim, int order, double ppc, int copies)
     TileSpec T{};
     T.copies        = copies;
     int const slabs = 5 * copies;
+    if (dim == 3)
+    {
+        // brick tile: the chunk's brick of BRICK^3 cells with the margins below (stencil from
+        // cell - 1, one cell of motion) and above (stencil, one cell of motion)
+        TileSpec B = T;
+        B.brick    = BRICK;
+        for (int d = 0; d < 3; ++d)
+        {
+            B.lo[d]    = 2;
+            B.ext[d]   = BRICK + 2 + order + 2;
+            B.pitch[d] = B.ext[d];
+        }
+        B.nodes  = B.ext[0] * B.ext[1] * B.ext[2];
+        B.stride = (B.nodes + 15) / 16 * 16 + 1;
+        B.bytes  = static_cast<unsigned int>(8 * slabs * B.stride);
+        if (B.bytes <= TILE_BYTES_MAX)
+            return B;
+    }
     // nodes that fit the LDS budget with the bank padding of every slab
     int const cap   = static_cast<int>(TILE_BYTES_MAX / (8u * static_cast<unsigned>(slabs))) - 17;
     int const CHUNK = chunkSize(dim, order);

@@ -342,6 +342,21 @@ def test_advance_uniform3d_staged(order):
     g.close()
 
 
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_advance_uniform3d_bricks(order):
+    """64 particles per cell on 8^3 (order 1) or 12^3 cells (orders 2, 3: a periodic level needs
+    more cells than twice the ghost width): every chunk of the linear 3D push is one brick of
+    4x4x4 cells (the loader's particles put in brick order at upload), so the deposit goes through
+    the brick tiles with the lanes of a wavefront in 64 different cells; 3 PPC steps against the
+    oracle at 1e-12"""
+    _say(f"[gpu] advance uniform3d bricks order={order}")
+    n = 8 if order == 1 else 12
+    wl = W.uniform3d(ncells=(n, n, n), ppc=64, order=order)
+    g, o = _run_pair(wl, 3)
+    _check_state(g, o, wl)
+    g.close()
+
+
 def test_mixed_family_has_no_3d_path():
     """the mixed-precision family's push runs on the captured fused step only, which is off in 3D:
     a 3D mixed-precision level refuses to initialise"""
