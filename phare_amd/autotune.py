@@ -29,14 +29,14 @@ from phare_amd import workloads as W  # noqa: E402
 
 TABLE = os.path.join(HERE, "tuning_table.json")
 
-# candidate values per dimension; the segmented-storage shapes do not apply to the 3D staged path
-# (linear storage), whose tile copies do not apply to the segmented pushes of 1D/2D
+# candidate values per dimension (the 3D staged path pushes segmented storage with one workgroup
+# per segment: its segment shape and sort interval)
 SPACE = {
     1: {"rows_per_item": [8, 16, 32, 64, 128], "sort_interval": [10, 20, 30, 60]},
     2: {"segment_shape": [[2, 32, 1], [4, 16, 1], [1, 64, 1], [8, 8, 1]],
         "rows_per_item": [16, 24, 32, 48, 64], "tile_row_pad": [1, 8, 16, 32],
         "sort_interval": [15, 20, 30, 40, 60]},
-    3: {"tile_copies": [1, 2, 4, 8]},
+    3: {"segment_shape": [[4, 4, 4], [2, 4, 8], [2, 2, 16]], "sort_interval": [5, 10, 20, 30, 60]},
 }
 
 

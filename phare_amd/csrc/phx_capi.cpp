@@ -705,6 +705,36 @@ phx_status enqueueAdvanceFast(phx_ctx* c, double dt)
 // one GPU run of it having faulted without a cause shown.
 bool staged3d(phx_ctx const* c) { return c->cfg.dim == 3 && c->patch.periodic; }
 
+// the push (or, pushOn 0, the deposit) of population p's domain array on the staged 3D path:
+// cell-segmented storage (after a sort) with one workgroup per segment, otherwise the linear
+// brick-tiled kernels; FP64 or mixed precision
+void enqueuePushStaged3d(phx_ctx* c, int p, PushArgs& A)
+{
+    Pop const& P     = c->pops[p];
+    Store const& dom = P.arr[PHX_DOMAIN];
+    A.segOn          = 0;
+    if (dom.seg)
+    {
+        A.sg   = c->sg;
+        A.st   = P.seg;
+        A.tile = segwgTileShape(c->cfg.dim, c->cfg.interp_order, c->sg, mixed(c) ? 4 : 5);
+        if (mixed(c))
+            launchPush32SegWG(A, dom.view32(), c->stream);
+        else
+            launchPushSegWG(A, dom.view(), c->stream);
+        return;
+    }
+    if (mixed(c))
+    {
+        A.tile = brickTileShape32(c->cfg.interp_order);
+        launchPush32Linear(A, dom.view32(), c->stream);
+    }
+    else if (A.pushOn)
+        launchPush(A, dom.view(), dom.view(), c->stream);
+    else
+        launchDeposit(A, dom.view(), c->stream);
+}
+
 // SolverPPC::moveIons_ (solver_ppc.hpp:537-598) of a 3D periodic level: the push deposits into
 // the population moments and wraps leavers back through the periodic image in place
 // (fillIonGhostParticles), then fillFluxBorders/fillDensityBorders, updateIons, fillIonBorders
@@ -731,7 +761,7 @@ phx_status moveIonsStaged3d(phx_ctx* c, double dt, int mode)
             ev->pop  = p;
             PHX_HIP(c, hipEventRecord(ev->a, c->stream));
         }
-        launchPush(A, dom.view(), dom.view(), c->stream);
+        enqueuePushStaged3d(c, p, A);
         if (ev)
             PHX_HIP(c, hipEventRecord(ev->b, c->stream));
     }
@@ -1338,11 +1368,12 @@ phx_status allocSegTables(phx_ctx* c)
 
 // the launch shapes of this build before any tuning: the autotuner's winners on the 2D Harris
 // workload (segments of 2x32 cells, at most 32 rows per work item, tile rows padded to 16 nodes,
-// a sort every 30 PPC steps); runs of 64 cells in 1D, 2x4x8 blocks in 3D
+// a sort every 30 PPC steps); runs of 64 cells in 1D, 4x4x4 blocks in 3D (the staged 3D path's
+// one-workgroup-per-segment push gives each lane of a wavefront one cell of the block)
 phx_tuning defaultTuning(int dim)
 {
     phx_tuning t{};
-    static int const pref[3][3] = {{64, 1, 1}, {2, 32, 1}, {2, 4, 8}};
+    static int const pref[3][3] = {{64, 1, 1}, {2, 32, 1}, {4, 4, 4}};
     for (int d = 0; d < 3; ++d)
         t.segment_shape[d] = pref[dim - 1][d];
     t.rows_per_item = PHX_SEG_ROWS;
@@ -2713,8 +2744,9 @@ phx_status phx_move_ions(phx_ctx* c, double dt, int mode)
         return fail(c, PHX_ERR_ARG, "bad mode");
     if (fastPath(c) && staged3d(c))
     {
+        // the averaged set may have been set or computed outside the average kernel
         if (mixed(c))
-            return fail(c, PHX_ERR_STATE, "phx_move_ions: the mixed-precision family has no 3D path");
+            launchEmToF32(c->G, c->em[PHX_EM_AVG], c->em32, c->stream);
         PHX_TRY(moveIonsStaged3d(c, dt, mode));
         return checkDeviceError(c);
     }
@@ -2818,15 +2850,17 @@ phx_status phx_initialize(phx_ctx* c)
     if (!c)
         return PHX_ERR_ARG;
     fillGhosts(c, c->em[PHX_EM] + 3, QBX);
-    if (!staged3d(c))
-        PHX_TRY(sortDomain(c));
-    else if (mixed(c))
-        return fail(c, PHX_ERR_STATE, "phx_initialize: the mixed-precision family has no 3D path");
+    PHX_TRY(sortDomain(c));
     zeroMoments(c);
     for (int p = 0; p < c->cfg.n_pop; ++p)
     {
         PushArgs A = pushArgs(c, p, KIND_DOMAIN, PHX_MODE_ALL, PHX_EM, 0., true);
-        if (mixed(c))
+        if (staged3d(c))
+        {
+            A.pushOn = 0;
+            enqueuePushStaged3d(c, p, A);
+        }
+        else if (mixed(c))
         {
             A.pushOn = 0;
             launchPush32(A, c->pops[p].arr[PHX_DOMAIN].view32(), c->stream);
@@ -2870,12 +2904,12 @@ phx_status phx_advance(phx_ctx* c, double dt, int nsteps)
     }
     if (staged3d(c))
     {
-        if (mixed(c))
-            return fail(c, PHX_ERR_STATE,
-                        "phx_advance: the mixed-precision family has no 3D path (its push runs on "
-                        "the captured fused step, which is off in 3D)");
         for (int s = 0; s < nsteps; ++s)
+        {
             PHX_TRY(advanceStaged3d(c, dt));
+            if (c->sortInterval > 0 && ++c->stepsSinceSort >= c->sortInterval)
+                PHX_TRY(sortDomain(c));
+        }
         return checkDeviceError(c);
     }
     PHX_TRY(sortIfLinear(c));

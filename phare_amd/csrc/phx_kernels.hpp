@@ -209,6 +209,15 @@ void launchSegCopy(PView32 const& src, PView32 const& dst, unsigned long long n,
 // deposit only); the LDS tile of segTileShape32
 void launchPush32(PushArgs const& A, PView32 const& v, hipStream_t s);
 TileSpec segTileShape32(int dim, int order, SegGeom const& sg);
+// the same over brick-ordered linear FP32 storage (3D), in place; the LDS tile of brickTileShape32
+void launchPush32Linear(PushArgs const& A, PView32 const& v, hipStream_t s);
+TileSpec brickTileShape32(int order);
+// push + gather + deposit over cell-segmented storage with one workgroup per segment and all its
+// rows, in place (3D; the staged 3D path), FP64 and mixed precision; grid nsegs + 1; the LDS tile
+// of segwgTileShape (5 slabs FP64, 4 mixed precision)
+void launchPushSegWG(PushArgs const& A, PView const& in, hipStream_t s);
+void launchPush32SegWG(PushArgs const& A, PView32 const& v, hipStream_t s);
+TileSpec segwgTileShape(int dim, int order, SegGeom const& sg, int slabs);
 // BorisPusher::move in place over A.n slots with box-list selectors; status[i] gets the slot's
 // fate for the partition (0 both selectors, 1 first only, 2 neither, 3 empty slot)
 void launchMove(PushArgs const& A, PView const& v, SelBoxes const& first, SelBoxes const& second,

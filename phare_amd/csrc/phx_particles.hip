@@ -25,9 +25,6 @@
 #ifndef PHX_PPT_ND
 #define PHX_PPT_ND 2 // particles per thread of the linear 2D push
 #endif
-#ifndef PHX_PPT_3D
-#define PHX_PPT_3D 16 // particles per thread of the linear 3D push: a chunk of 4096 is one brick
-#endif
 
 namespace phx
 {
@@ -673,6 +670,42 @@ void k_push_seg(PushArgs A, PView in)
 }
 
 // ---------------------------------------------------------------------------------------------
+// fused push + deposit over cell-segmented storage, one workgroup per segment with all its rows
+// (the staged 3D path, segments of 4x4x4 cells): wavefront w takes rows w, w + 4, ..., so that
+// each instruction's 64 lanes walk 64 cells of the segment; the tile covers the segment with the
+// stencil and one cell of motion on each side; the overflow segment (number nsegs) deposits
+// straight into HBM
+// ---------------------------------------------------------------------------------------------
+template<int D, int O>
+__global__ __launch_bounds__(BLOCK) void k_push_segwg(PushArgs A, PView in)
+{
+    extern __shared__ double lds[];
+    if (A.G.stopped())
+        return;
+    Geom const& G          = A.G;
+    unsigned int const seg = blockIdx.x;
+    int const t            = threadIdx.x;
+    int const lane         = t & 63;
+    unsigned long long const off = A.st.off[seg];
+    unsigned int const R         = A.st.rows[seg];
+    zeroTile(lds, A.tile);
+    TileView T;
+    tileView(T, lds, A.tile);
+    pc::segTileOrigin<D>(G, A.sg, A.tile, seg, T.org);
+    __syncthreads();
+    Acc<D, O> acc;
+    if constexpr (Cfg<D, O>::ACC)
+        accZero(acc);
+#pragma unroll 1
+    for (unsigned int r = static_cast<unsigned int>(t >> 6); r < R; r += NW)
+        pushParticle<D, O>(A, in, in, off + static_cast<unsigned long long>(r) * LANES + lane, T, 0, acc);
+    if constexpr (Cfg<D, O>::ACC)
+        accFlush<D, O>(acc, G, T, 0, A.mom);
+    __syncthreads();
+    flushTile<D>(G, T, A.tile.nodes, A.mom);
+}
+
+// ---------------------------------------------------------------------------------------------
 // stable compaction: append the particles whose status equals `want` to dst at dstOffset
 // ---------------------------------------------------------------------------------------------
 __global__ __launch_bounds__(BLOCK) void k_count_status(uint8_t const* __restrict__ status,
@@ -1175,7 +1208,7 @@ SegPatch segPatchOf(int dim, int const* lo, int const* hi, int const* pref)
 {
     // the preferred segment shape `pref` (per dimension; by default a run of 64 cells in 1D, a
     // 2x32 block in 2D, 32 cells along the fastest dimension so that a row's gather loads stay
-    // in runs of consecutive nodes, and a 2x4x8 block in 3D: compact blocks keep the LDS tile with
+    // in runs of consecutive nodes, and a 4x4x4 block in 3D: compact blocks keep the LDS tile with
     // its margins small), clipped to the box; the cells left over go to the slower dimensions
     SegPatch P{};
     int room = LANES;
@@ -1393,6 +1426,49 @@ void launchCompact(PView const& src, PView const& dst, uint8_t const* status, un
     k_scan_counts<<<1, 1024, 0, s>>>(blockCounts, nb, total);
     k_scatter_status<<<nb, BLOCK, 0, s>>>(src, dst, status, n, want, blockCounts, dstOffset, dim,
                                           wrap);
+}
+
+// tile of the workgroup-per-segment push: the largest segment plus the stencil and one cell of
+// motion below (2 nodes) and above (order + 2 nodes), `slabs` moment slabs
+TileSpec segwgTileShape(int dim, int order, SegGeom const& sg, int slabs)
+{
+    TileSpec T{};
+    T.copies = 1;
+    for (int d = 0; d < 3; ++d)
+    {
+        T.lo[d]    = d < dim ? 2 : 0;
+        T.ext[d]   = d < dim ? sg.maxSeg[d] + 2 + order + 2 : 1;
+        T.pitch[d] = T.ext[d];
+    }
+    T.nodes  = T.ext[0] * T.ext[1] * T.ext[2];
+    T.stride = (T.nodes + 15) / 16 * 16;
+    T.bytes  = static_cast<unsigned int>(8 * slabs * T.stride);
+    if (T.bytes > TILE_BYTES_MAX)
+    {
+        // no room: every deposit goes straight to HBM
+        for (int d = 0; d < 3; ++d)
+        {
+            T.ext[d]   = 1;
+            T.pitch[d] = 1;
+        }
+        T.lo[0]  = -0x10000000;
+        T.nodes  = 1;
+        T.stride = 16;
+        T.bytes  = static_cast<unsigned int>(8 * slabs * T.stride);
+    }
+    return T;
+}
+
+void launchPushSegWG(PushArgs const& A, PView const& in, hipStream_t s)
+{
+    unsigned int const grid = static_cast<unsigned int>(A.sg.nsegs) + 1u;
+    switch (A.G.dim * 10 + A.G.order)
+    {
+        case 31: k_push_segwg<3, 1><<<grid, BLOCK, A.tile.bytes, s>>>(A, in); break;
+        case 32: k_push_segwg<3, 2><<<grid, BLOCK, A.tile.bytes, s>>>(A, in); break;
+        case 33: k_push_segwg<3, 3><<<grid, BLOCK, A.tile.bytes, s>>>(A, in); break;
+        default: break;
+    }
 }
 
 // ---------------------------------------------------------------------------------------------

@@ -646,6 +646,9 @@ namespace
 {
 // rows [rA, rB) of segment seg through the FP64 LDS tile: zero the tile, push the rows (wavefront
 // w takes rows rA + w, rA + w + NW, ...), flush the tile. Every work-item of the workgroup calls it.
+template<int D, int TB>
+__device__ __forceinline__ void flushTile32(PushArgs const& A, Tile32 const& T);
+
 template<int D, int O, bool WIN>
 __device__ __forceinline__ void pushPiece32(PushArgs const& A, PView32 const& v, double* lds32,
                                             unsigned int seg, unsigned int rA, unsigned int rB)
@@ -685,10 +688,17 @@ __device__ __forceinline__ void pushPiece32(PushArgs const& A, PView32 const& v,
     if constexpr (WIN)
         winFlush<D, O>(win, G, T, A.mom, A.charge);
     __syncthreads();
+    flushTile32<D, pc::SEG_BLOCK>(A, T);
+}
 
+template<int D, int TB>
+__device__ __forceinline__ void flushTile32(PushArgs const& A, Tile32 const& T)
+{
+    Geom const& G = A.G;
+    int const t   = threadIdx.x;
     // flush: one HBM atomic per touched node and moment
     int const e12 = T.pitch[1] * T.pitch[2];
-    for (int x = t; x < A.tile.nodes; x += pc::SEG_BLOCK)
+    for (int x = t; x < A.tile.nodes; x += TB)
     {
         int node = -1;
 #pragma unroll
@@ -757,6 +767,149 @@ static void launchPush32T(PushArgs const& A, PView32 const& v, hipStream_t s)
             return;
         }
     k_push_seg32<D, O, false><<<A.st.itemsCap, pc::SEG_BLOCK, A.tile.bytes, s>>>(A, v);
+}
+
+// ---------------------------------------------------------------------------------------------
+// fused push + deposit over brick-ordered linear FP32 storage (3D, the staged 3D path), in place:
+// chunks of 4096 particles, 16 per thread, the tile over the brick of the chunk's first particle
+// (the storage of the FP64 family's linear 3D push, in FP32)
+// ---------------------------------------------------------------------------------------------
+namespace
+{
+constexpr int LIN32_PPT = PHX_PPT_3D; // the FP64 linear 3D push's, so that both share the brick sort's interleave
+
+template<int D, int O>
+__global__ __launch_bounds__(BLOCK) void k_push32_lin(PushArgs A, PView32 v)
+{
+    extern __shared__ double lds32[];
+    if (A.G.stopped())
+        return;
+    Geom const& G = A.G;
+    int const t   = threadIdx.x;
+    unsigned long long const base = static_cast<unsigned long long>(blockIdx.x) * (BLOCK * LIN32_PPT);
+    for (int x = t; x < 4 * A.tile.stride; x += BLOCK)
+        lds32[x] = 0.;
+    Tile32 T;
+    T.lds    = lds32;
+    T.stride = A.tile.stride;
+    int const bk = A.tile.brick;
+    bool const far = v.ic[0][base] == DEAD_CELL;
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+    {
+        T.ext[d]   = A.tile.ext[d];
+        T.pitch[d] = A.tile.pitch[d];
+        T.org[d]   = 0;
+    }
+#pragma unroll
+    for (int d = 0; d < D; ++d)
+    {
+        int first = v.ic[d][base] - G.lower[d];
+        first     = (first >= 0 ? first / bk : -((bk - 1 - first) / bk)) * bk;
+        T.org[d]  = far ? -0x20000000 : first + G.g - A.tile.lo[d];
+    }
+    Push32 K;
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+        K.hdtdx[d] = static_cast<float>(A.hdtdx[d]);
+    K.coef1 = static_cast<float>(A.charge * A.dto2m);
+    K.coef  = static_cast<float>(A.coef);
+    __syncthreads();
+    Win32<D, O> win;
+#pragma unroll 1
+    for (int k = 0; k < LIN32_PPT; ++k)
+    {
+        unsigned long long const i = base + static_cast<unsigned long long>(k) * BLOCK + t;
+        if (i < A.n)
+            pushSlot32<D, O, false>(A, K, v, i, T, win);
+    }
+    __syncthreads();
+    flushTile32<D, BLOCK>(A, T);
+}
+} // namespace
+
+// the same over cell-segmented FP32 storage, one workgroup per segment with all its rows (the
+// staged 3D path's layout after its sorts; the FP64 family's k_push_segwg in FP32)
+template<int D, int O>
+__global__ __launch_bounds__(BLOCK) void k_push32_segwg(PushArgs A, PView32 v)
+{
+    extern __shared__ double lds32[];
+    if (A.G.stopped())
+        return;
+    Geom const& G          = A.G;
+    unsigned int const seg = blockIdx.x;
+    int const t            = threadIdx.x;
+    int const lane         = t & 63;
+    unsigned long long const off = A.st.off[seg];
+    unsigned int const R         = A.st.rows[seg];
+    for (int x = t; x < 4 * A.tile.stride; x += BLOCK)
+        lds32[x] = 0.;
+    Tile32 T;
+    T.lds    = lds32;
+    T.stride = A.tile.stride;
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+    {
+        T.ext[d]   = A.tile.ext[d];
+        T.pitch[d] = A.tile.pitch[d];
+        T.org[d]   = 0;
+    }
+    pc::segTileOrigin<D>(G, A.sg, A.tile, seg, T.org);
+    Push32 K;
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+        K.hdtdx[d] = static_cast<float>(A.hdtdx[d]);
+    K.coef1 = static_cast<float>(A.charge * A.dto2m);
+    K.coef  = static_cast<float>(A.coef);
+    __syncthreads();
+    Win32<D, O> win;
+#pragma unroll 1
+    for (unsigned int r = static_cast<unsigned int>(t >> 6); r < R; r += NW)
+        pushSlot32<D, O, false>(A, K, v, off + static_cast<unsigned long long>(r) * LANES + lane, T, win);
+    __syncthreads();
+    flushTile32<D, BLOCK>(A, T);
+}
+
+void launchPush32SegWG(PushArgs const& A, PView32 const& v, hipStream_t s)
+{
+    unsigned int const grid = static_cast<unsigned int>(A.sg.nsegs) + 1u;
+    switch (A.G.dim * 10 + A.G.order)
+    {
+        case 31: k_push32_segwg<3, 1><<<grid, BLOCK, A.tile.bytes, s>>>(A, v); break;
+        case 32: k_push32_segwg<3, 2><<<grid, BLOCK, A.tile.bytes, s>>>(A, v); break;
+        case 33: k_push32_segwg<3, 3><<<grid, BLOCK, A.tile.bytes, s>>>(A, v); break;
+        default: break;
+    }
+}
+
+TileSpec brickTileShape32(int order)
+{
+    TileSpec T{};
+    T.copies = 1;
+    T.brick  = BRICK;
+    for (int d = 0; d < 3; ++d)
+    {
+        T.lo[d]    = 2;
+        T.ext[d]   = BRICK + 2 + order + 2;
+        T.pitch[d] = T.ext[d];
+    }
+    T.nodes  = T.ext[0] * T.ext[1] * T.ext[2];
+    T.stride = (T.nodes + 15) / 16 * 16;
+    T.bytes  = static_cast<unsigned int>(8 * 4 * T.stride);
+    return T;
+}
+
+void launchPush32Linear(PushArgs const& A, PView32 const& v, hipStream_t s)
+{
+    if (A.n == 0 || A.G.dim != 3)
+        return;
+    unsigned int const grid = static_cast<unsigned int>((A.n + BLOCK * LIN32_PPT - 1) / (BLOCK * LIN32_PPT));
+    switch (A.G.order)
+    {
+        case 1: k_push32_lin<3, 1><<<grid, BLOCK, A.tile.bytes, s>>>(A, v); break;
+        case 2: k_push32_lin<3, 2><<<grid, BLOCK, A.tile.bytes, s>>>(A, v); break;
+        case 3: k_push32_lin<3, 3><<<grid, BLOCK, A.tile.bytes, s>>>(A, v); break;
+    }
 }
 
 void launchPush32(PushArgs const& A, PView32 const& v, hipStream_t s)

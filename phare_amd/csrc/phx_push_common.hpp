@@ -10,6 +10,9 @@
 #ifndef PHX_TILE_SPREAD
 #define PHX_TILE_SPREAD 1
 #endif
+#ifndef PHX_PPT_3D
+#define PHX_PPT_3D 16 // particles per thread of the linear 3D pushes: a chunk of 4096 is one brick
+#endif
 #ifndef PHX_SEG_WAVES
 #define PHX_SEG_WAVES 4 // waves per SIMD the register allocation of the segmented push targets
 #endif

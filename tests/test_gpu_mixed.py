@@ -30,6 +30,11 @@ WORKLOADS = {
     "harris2d-o1": lambda: W.harris2d(ncells=(32, 24), ppc=12, order=1),
     "harris2d-o2": lambda: W.harris2d(ncells=(32, 24), ppc=12, order=2),
     "harris2d-o3": lambda: W.harris2d(ncells=(32, 24), ppc=12, order=3),
+    # 3D: the staged path's brick-ordered linear FP32 storage, bricks aligned with the push's
+    # chunks (64 ppc on 8^3 cells) and not (8 and 16 ppc on uneven boxes)
+    "uniform3d-o1": lambda: W.uniform3d(ncells=(8, 8, 8), ppc=64, order=1),
+    "uniform3d-o2": lambda: W.uniform3d(ncells=(12, 11, 10), ppc=8, order=2),
+    "uniform3d-o3": lambda: W.uniform3d(ncells=(12, 12, 12), ppc=16, order=3),
 }
 
 
@@ -91,12 +96,12 @@ def test_mixed_one_step(name):
     g.close()
 
 
-@pytest.mark.parametrize("name", ["uniform1d", "harris2d-o2"])
+@pytest.mark.parametrize("name", ["uniform1d", "harris2d-o2", "uniform3d-o1"])
 def test_mixed_twenty_steps(name):
     _say(f"[gpu] mixed 20 PPC steps {name}")
     wl = WORKLOADS[name]()
     g, o = _pair(wl)
-    g.set_sort_interval(7)
+    g.set_sort_interval(7)  # 3D: the device brick re-sort of the FP32 storage
     g.advance(wl.dt, 20)
     o.advance(wl.dt, 20)
     e = _moment_errors(g, o)

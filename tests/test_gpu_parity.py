@@ -357,16 +357,27 @@ def test_advance_uniform3d_bricks(order):
     g.close()
 
 
-def test_mixed_family_has_no_3d_path():
-    """the mixed-precision family's push runs on the captured fused step only, which is off in 3D:
-    a 3D mixed-precision level refuses to initialise"""
-    _say("[gpu] mixed family refuses 3D")
-    g = phx.Level(3, 1, [9, 10, 11], [0.2] * 3, precision=phx.PRECISION_FP32)
-    with pytest.raises(phx.PhxError) as e:
-        g.initialize()
-    assert e.value.status == 6 and "3D" in str(e.value)
+def test_advance_uniform3d_sorted_every_step():
+    """the staged 3D path on cell-segmented storage (one workgroup per 4x4x4-cell segment), sorted
+    again after every PPC step with a time step that moves particles across cells: the run stays
+    equal to the oracle's at 1e-12 and the storage stays grouped by segment"""
+    _say("[gpu] advance uniform3d sorted every step")
+    wl = W.uniform3d(ncells=(8, 8, 8), ppc=64, order=1)
+    wl.dt = 0.03
+    g = phx.Level(wl.dim, wl.order, wl.ncells, wl.dx, masses=wl.masses, Te=wl.Te,
+                  capacity=wl.n_particles, apply_tuning_table=False)
+    o = O.Sim(wl.dim, wl.order, wl.ncells, wl.dx, masses=wl.masses, Te=wl.Te)
+    g.set_sort_interval(1)
+    wl.setup(g, phx.F_B)
+    wl.setup(o, O.F_B)
+    for _ in range(4):
+        g.advance(wl.dt, 1)
+        o.advance(wl.dt, 1)
+        g.sync()
+        q = g.get_particles()
+        assert np.all(np.diff(_segment_of(q["icell"], wl.ncells)) >= 0)
+    _check_state(g, o, wl)
     g.close()
-
 
 def test_advance_harris2d_full_size_three_steps():
     """the full 2D Harris workload (256x128 cells, 100 ppc, 3.28 M particles, order 2) over 3 PPC
@@ -407,10 +418,10 @@ def test_move_ions_general_path_matches_fast_path():
 
 def _segment_of(icell, ncells):
     """segment index of each particle's cell: the domain box is cut into segments of at most 64
-    cells (a run of 64 cells in 1D, a 2x32 block in 2D, a 2x4x8 block in 3D, clipped to the box
+    cells (a run of 64 cells in 1D, a 2x32 block in 2D, a 4x4x4 block in 3D, clipped to the box
     with the cells left over given to the slower dimensions), numbered row-major"""
     dim = len(ncells)
-    pref = [[64], [1, 32], [2, 4, 8]][dim - 1]
+    pref = [[64], [1, 32], [4, 4, 4]][dim - 1]
     seg, room = [1] * dim, 64
     for d in range(dim - 1, -1, -1):
         want = room if d == 0 else min(pref[d], room)
