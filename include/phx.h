@@ -280,11 +280,18 @@ enum { PHX_REFINE_LINEAR = 0, PHX_REFINE_LINEAR_OVERWRITE = 1, PHX_REFINE_ELECTR
        PHX_REFINE_MAGNETIC = 3, PHX_REFINE_MAGNETIC_INIT = 4 };
 phx_status phx_refine_field(phx_ctx* fine, int fine_field, const phx_ctx* coarse, int coarse_field,
                             int op, const int lower[3], const int upper[3]);
+/* setNaNsOnFieldGhosts (hybrid_hybrid_messenger_strategy.hpp:959-994) for a one-patch context:
+   every node of the field outside its physical nodes becomes NaN, so that the NaN-only refine
+   operators of a level-ghost fill (phx_refine_field) complete exactly those nodes */
+phx_status phx_set_ghosts_nan(phx_ctx* ctx, int field_id);
 /* MagneticRefinePatchStrategy::postprocessRefine (src/amr/data/field/refine/
    magnetic_refine_patch_strategy.hpp:67-125): in the fine AMR cell box cell_lower..cell_upper the
    new faces of the three components of the B field whose x component is b_field_x (PHX_F_B of a
-   set, or PHX_F_BOLD_X) are set from the shared faces (Toth & Roe 2002, divergence-free); every
-   value read must lie inside the arrays (one node below the box, two above) */
+   set, or PHX_F_BOLD_X) are set from the shared faces (Toth & Roe 2002, divergence-free). The
+   box's cells must lie in the arrays. At their edge (no counterpart in the reference, which
+   refines into a scratch region one stencil wider), a face whose cross terms would read beyond the
+   arrays takes the mean of its two neighbours along the normal, and a face without both neighbours
+   is left unchanged. */
 phx_status phx_refine_magnetic_new_faces(phx_ctx* fine, int b_field_x, const int cell_lower[3],
                                          const int cell_upper[3]);
 /* IonUpdater::updateIons (ion_updater.hpp:111-116) */

@@ -371,6 +371,49 @@ inline void magneticNewFaces(int dim, AmrField const& bx, AmrField const& by, Am
                     int const idx[3] = {ix, iy, iz};
                     if (!odd(idx[comp]))
                         continue;
+                    // at the edge of the arrays (this build's rule; the reference refines into
+                    // a scratch region one stencil wider): without both neighbours along the
+                    // normal the face is left, with cross terms reading beyond the arrays it takes
+                    // the neighbours' mean
+                    AmrField const& own = comp == 0 ? bx : (comp == 1 ? by : bz);
+                    auto inF = [&](AmrField const& F, int i, int j, int k) {
+                        int const a[3] = {i, j, k};
+                        for (int d = 0; d < 3; ++d)
+                            if (a[d] - F.org[d] < 0 || a[d] - F.org[d] >= F.shape[d])
+                                return false;
+                        return true;
+                    };
+                    int lo3[3] = {ix, iy, iz}, hi3[3] = {ix, iy, iz};
+                    lo3[comp] -= 1;
+                    hi3[comp] += 1;
+                    if (!inF(own, lo3[0], lo3[1], lo3[2]) || !inF(own, hi3[0], hi3[1], hi3[2]))
+                        continue;
+                    bool crossIn = true;
+                    for (int c = 0; c < dim && dim > 1; ++c)
+                    {
+                        if (c == comp)
+                            continue;
+                        AmrField const& F = c == 0 ? bx : (c == 1 ? by : bz);
+                        for (int sx = 0; sx < 2; ++sx)
+                            for (int sy = 0; sy < (dim > 1 ? 2 : 1); ++sy)
+                                for (int sz = 0; sz < (dim > 2 ? 2 : 1); ++sz)
+                                {
+                                    int const sd[3] = {sx, sy, sz};
+                                    int r[3]        = {0, 0, 0};
+                                    for (int d = 0; d < dim; ++d)
+                                    {
+                                        int const o = d == comp ? 1 : (odd(idx[d]) ? 1 : 0);
+                                        r[d] = sd[d] ? (d == c ? pp(idx[d], o) : dp(idx[d], o))
+                                                     : (d == c ? pm(idx[d], o) : dm(idx[d], o));
+                                    }
+                                    crossIn = crossIn && inF(F, r[0], r[1], r[2]);
+                                }
+                    }
+                    if (dim > 1 && !crossIn)
+                    {
+                        own(ix, iy, iz) = 0.5 * (own(lo3[0], lo3[1], lo3[2]) + own(hi3[0], hi3[1], hi3[2]));
+                        continue;
+                    }
                     if (dim == 1)
                     {
                         bx(ix) = 0.5 * (bx(ix - 1) + bx(ix + 1));

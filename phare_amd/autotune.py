@@ -36,7 +36,7 @@ SPACE = {
     2: {"segment_shape": [[2, 32, 1], [4, 16, 1], [1, 64, 1], [8, 8, 1]],
         "rows_per_item": [16, 24, 32, 48, 64], "tile_row_pad": [1, 8, 16, 32],
         "sort_interval": [15, 20, 30, 40, 60]},
-    3: {"segment_shape": [[4, 4, 4], [2, 4, 8], [2, 2, 16]], "sort_interval": [5, 10, 20, 30, 60]},
+    3: {"segment_shape": [[4, 4, 4], [2, 4, 8]], "sort_interval": [5, 10, 15, 30]},
 }
 
 

@@ -459,7 +459,26 @@ __global__ __launch_bounds__(BLOCK) void k_mag_new_faces(NewFacesArgs A, int n, 
     int im[3] = {loc[0], loc[1], loc[2]}, ip[3] = {loc[0], loc[1], loc[2]};
     im[n] -= 1;
     ip[n] += 1;
+    // at the edge of the arrays: no neighbour along the normal -> left as it is; cross terms that
+    // would read beyond the arrays -> the mean of the two neighbours only
+    if (im[n] < 0 || ip[n] >= A.shape[n][n])
+        return;
+    bool cross = true;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+    {
+        if (c == n || (D == 2 && c == 2) || D == 1)
+            continue;
+#pragma unroll
+        for (int d = 0; d < D; ++d)
+            cross = cross && loc[d] - off[d] >= 0 && loc[d] + (d == c ? 2 : 1) - off[d] < A.shape[c][d];
+    }
     double v = 0.5 * (at(n, im[0], im[1], im[2]) + at(n, ip[0], ip[1], ip[2]));
+    if (!cross)
+    {
+        A.b[n][(static_cast<long>(loc[0]) * A.s1[n] + loc[1]) * A.s2[n] + loc[2]] = v;
+        return;
+    }
     if constexpr (D == 2)
     {
         double s;

@@ -2571,6 +2571,18 @@ phx_status phx_refine_field(phx_ctx* fine, int fine_field, phx_ctx const* coarse
     return checkLaunch(fine, "phx_refine_field");
 }
 
+phx_status phx_set_ghosts_nan(phx_ctx* ctx, int field_id)
+{
+    if (!ctx)
+        return PHX_ERR_ARG;
+    double* p;
+    int q;
+    if (!fieldOf(ctx, field_id, p, q))
+        return fail(ctx, PHX_ERR_ARG, "phx_set_ghosts_nan: bad field id");
+    launchNanGhosts(ctx->G, q, p, ctx->stream);
+    return checkLaunch(ctx, "phx_set_ghosts_nan");
+}
+
 phx_status phx_refine_magnetic_new_faces(phx_ctx* fine, int b_field_x, int const cell_lower[3],
                                          int const cell_upper[3])
 {
@@ -2587,6 +2599,8 @@ phx_status phx_refine_magnetic_new_faces(phx_ctx* fine, int b_field_x, int const
                         "magnetic field (B of a set, or Bold)");
         A.s1[k] = fine->G.shape[QBX + k][1];
         A.s2[k] = fine->G.shape[QBX + k][2];
+        for (int d = 0; d < 3; ++d)
+            A.shape[k][d] = fine->G.shape[QBX + k][d];
     }
     int const D = fine->cfg.dim;
     A.dim       = D;
@@ -2604,21 +2618,10 @@ phx_status phx_refine_magnetic_new_faces(phx_ctx* fine, int b_field_x, int const
         if (A.hi[d] < A.lo[d])
             return fail(fine, PHX_ERR_ARG, "phx_refine_magnetic_new_faces: empty box");
     }
-    // every value a new face reads lies within [f - 1, f + 2] along each direction (f + 1 at
-    // most along the dual directions of the value read and along the face's normal)
-    for (int n = 0; n < D; ++n)
-        for (int a = 0; a < (D == 1 ? 1 : 3); ++a)
-        {
-            if (D == 2 && a == 2)
-                continue;
-            for (int d = 0; d < D; ++d)
-            {
-                int const need = (d == n || d == a) ? 2 : 1;
-                if (A.lo[d] - A.org[d] - 1 < 0 || A.hi[d] - A.org[d] + need >= fine->G.shape[QBX + a][d])
-                    return fail(fine, PHX_ERR_ARG, "phx_refine_magnetic_new_faces: the reconstruction "
-                                                   "stencil leaves the B arrays");
-            }
-        }
+    // the box's cells within the arrays (faces whose stencil leaves them take the edge rule)
+    for (int d = 0; d < D; ++d)
+        if (A.lo[d] - A.org[d] < 0 || A.hi[d] - A.org[d] >= fine->G.shape[QBX + (d + 1) % 3][d])
+            return fail(fine, PHX_ERR_ARG, "phx_refine_magnetic_new_faces: the box leaves the B arrays");
     launchMagneticNewFaces(A, fine->stream);
     return checkLaunch(fine, "phx_refine_magnetic_new_faces");
 }

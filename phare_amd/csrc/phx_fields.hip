@@ -565,6 +565,25 @@ __global__ __launch_bounds__(FB) void k_fill_ghosts(Geom G0, int q, double* f)
         f[G.idx(q, i, j, k)] = f[G.idx(q, src[0], src[1], src[2])];
 }
 
+// setNaNsOnFieldGhosts of a single patch (hybrid_hybrid_messenger_strategy.hpp:959-994): every
+// node outside the quantity's physical nodes becomes NaN (what the NaN-only refine operators of a
+// level-ghost fill then complete)
+template<int DIM>
+__global__ __launch_bounds__(FB) void k_nan_ghosts(Geom G0, int q, double* f)
+{
+    GeomD<DIM> const G(G0);
+    long const t = static_cast<long>(blockIdx.x) * FB + threadIdx.x;
+    int i, j, k;
+    if (!nodeOf(G, q, t, i, j, k))
+        return;
+    int const c[3] = {i, j, k};
+    bool ghost     = false;
+    for (int d = 0; d < G.dim; ++d)
+        ghost = ghost || c[d] < G.g || c[d] > G.physEnd(q, d);
+    if (ghost)
+        f[G.idx(q, i, j, k)] = __builtin_nan("");
+}
+
 // image k of {-1,0,1}^dim (lexicographic, zero excluded) overlaps the all-primal ghost box on
 // [max(0, k*N), min(end, end + k*N)]; node n receives orig[n - k*N]
 template<int DIM, bool MAX>
@@ -1215,6 +1234,15 @@ void launchZero(double* p, long count, hipStream_t s) { k_zero<<<gridFor(count),
 void launchCopy(double const* src, double* dst, long count, hipStream_t s)
 {
     k_copy<<<gridFor(count), FB, 0, s>>>(src, dst, count);
+}
+void launchNanGhosts(Geom const& G, int q, double* f, hipStream_t s)
+{
+    switch (G.dim)
+    {
+        case 1: k_nan_ghosts<1><<<gridFor(G.size(q)), FB, 0, s>>>(G, q, f); break;
+        case 2: k_nan_ghosts<2><<<gridFor(G.size(q)), FB, 0, s>>>(G, q, f); break;
+        case 3: k_nan_ghosts<3><<<gridFor(G.size(q)), FB, 0, s>>>(G, q, f); break;
+    }
 }
 void launchFillGhosts(Geom const& G, int q, double* f, hipStream_t s)
 {

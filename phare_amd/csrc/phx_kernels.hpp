@@ -303,11 +303,14 @@ struct RefineArgs
 };
 void launchRefineField(RefineArgs const& A, hipStream_t s);
 // the new fine faces of B inside the fine AMR cell box lo .. hi (the faces with an odd index along
-// their normal), from the shared faces already refined (MagneticRefinePatchStrategy)
+// their normal), from the shared faces already refined (MagneticRefinePatchStrategy); at the edge
+// of the arrays, a face whose cross terms would read beyond them takes the mean of its two
+// neighbours along the normal, and one without both neighbours is left as it is
 struct NewFacesArgs
 {
     double* b[3];
     int s1[3], s2[3]; // shape[1], shape[2] of each component's array
+    int shape[3][3];  // each component's array shape (the reads' bounds)
     int org[3];
     int lo[3], hi[3];
     double dx[3];
@@ -339,6 +342,8 @@ void launchOhmAverage(Geom const& G, double eta, double nu, int hyper, int level
                       CVecPtr Ve, double const* Pe, CVecPtr Bp, CVecPtr J, VecPtr Ep, CVecPtr E,
                       CVecPtr B, VecPtr Ea, VecPtr Ba, hipStream_t s, float* const* mirror = nullptr);
 void launchFillGhosts(Geom const& G, int q, double* f, hipStream_t s);
+// every node of f (quantity q) outside the physical nodes set to NaN
+void launchNanGhosts(Geom const& G, int q, double* f, hipStream_t s);
 void launchBorderSum(Geom const& G, double const* orig, double* out, hipStream_t s);
 void launchBorderMax(Geom const& G, double const* orig, double* inout, hipStream_t s);
 void launchZero(double* p, long count, hipStream_t s);

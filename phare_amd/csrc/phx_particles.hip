@@ -683,7 +683,7 @@ __global__ __launch_bounds__(BLOCK) void k_push_segwg(PushArgs A, PView in)
     if (A.G.stopped())
         return;
     Geom const& G          = A.G;
-    unsigned int const seg = blockIdx.x;
+    unsigned int const seg = pc::xcdItem(blockIdx.x, gridDim.x);
     int const t            = threadIdx.x;
     int const lane         = t & 63;
     unsigned long long const off = A.st.off[seg];

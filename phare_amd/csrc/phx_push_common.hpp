@@ -72,6 +72,18 @@ __device__ __forceinline__ void ktEnd(KTime* kt, unsigned int working)
     atomicExch(&kt->done, 0u);
 }
 
+// XCD-aware order of a grid's workgroups: the hardware deals workgroup b to XCD b % 8, so that
+// XCD x gets workgroups x, x + 8, ...; the work item of b is chosen so that each XCD takes one
+// contiguous run of items (neighbouring segments share their field halos and moment nodes in that
+// XCD's L2). A bijection of [0, n).
+__device__ __forceinline__ unsigned int xcdItem(unsigned int b, unsigned int n)
+{
+    constexpr unsigned int XCD = 8;
+    unsigned int const q = n / XCD, r = n % XCD;
+    unsigned int const x = b % XCD, i = b / XCD;
+    return x * q + (x < r ? x : r) + i;
+}
+
 // rows [r0, r1) of work item j of a segment of R rows: the segment's ceil(R / rowsPerItem)
 // items share its rows evenly (at most rowsPerItem each)
 __device__ __forceinline__ void itemRows(unsigned int j, unsigned int R, unsigned int rowsPerItem,

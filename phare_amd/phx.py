@@ -124,6 +124,7 @@ def lib():
         L.phx_coarsen.argtypes = [vp, C.c_int, vp, C.c_int, C.c_int, _ip, _ip]
         L.phx_refine_field.argtypes = [vp, C.c_int, vp, C.c_int, C.c_int, _ip, _ip]
         L.phx_refine_magnetic_new_faces.argtypes = [vp, C.c_int, _ip, _ip]
+        L.phx_set_ghosts_nan.argtypes = [vp, C.c_int]
         L.phx_get_tuning.argtypes = [vp, C.POINTER(Tuning)]
         L.phx_set_tuning.argtypes = [vp, C.POINTER(Tuning)]
         _lib = L
@@ -145,7 +146,7 @@ EXPORTED = ["phx_create", "phx_destroy", "phx_last_error", "phx_synchronize", "p
             "phx_get_tuning", "phx_set_tuning", "phx_refine_particles",
             "phx_level_ghost_last_step", "phx_time_interpolate", "phx_prepare_step",
             "phx_reset_flux_sum", "phx_accumulate_flux_sum", "phx_reflux", "phx_coarsen",
-            "phx_refine_field", "phx_refine_magnetic_new_faces"]
+            "phx_refine_field", "phx_refine_magnetic_new_faces", "phx_set_ghosts_nan"]
 
 
 class Level:
@@ -190,6 +191,8 @@ class Level:
         self.h = h
         self.npop = len(masses)
         self.ncells = list(ncells)
+        self.dx, self.lower, self.capacity = list(dx), lower, int(capacity)
+        self.masses, self.Te, self.eta, self.nu = tuple(masses), Te, eta, nu
         self.precision = precision
         self._tuning_pending = apply_tuning_table
 
@@ -489,6 +492,10 @@ class Level:
         lo[:self.dim] = lower
         hi[:self.dim] = upper
         self._rc(lib().phx_refine_magnetic_new_faces(self.h, b_fid, lo, hi))
+
+    def set_ghosts_nan(self, fid):
+        """phx_set_ghosts_nan: NaN on every node of field fid outside its physical nodes"""
+        self._rc(lib().phx_set_ghosts_nan(self.h, fid))
 
     def deposit(self, pop=0, kind=DOMAIN, coef=1.0):
         self._rc(lib().phx_deposit(self.h, pop, kind, coef))

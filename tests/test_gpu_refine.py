@@ -123,8 +123,17 @@ def test_magnetic_new_faces_match_oracle(dim, order):
         want = O.magnetic_new_faces(B, forg, hf, lo2, hi2)
         for k in range(3):
             np.testing.assert_array_equal(fine.get(base + k), want[k], err_msg=f"B{k} inner")
+        # every cell of the arrays: the faces at their edge take the edge rule
+        for k in range(3):
+            fine.set(base + k, B[k])
+        lo3 = list(forg)
+        hi3 = [flo[d] + fn[d] + g - 1 for d in range(dim)]
+        fine.refine_magnetic_new_faces(base, lo3, hi3)
+        want = O.magnetic_new_faces(B, forg, hf, lo3, hi3)
+        for k in range(3):
+            np.testing.assert_array_equal(fine.get(base + k), want[k], err_msg=f"B{k} edge")
     with pytest.raises(phx.PhxError):
-        fine.refine_magnetic_new_faces(phx.F_B, list(forg), [a + 4 for a in forg])
+        fine.refine_magnetic_new_faces(phx.F_B, [a - 1 for a in forg], [a + 4 for a in forg])
     with pytest.raises(phx.PhxError):
         fine.refine_magnetic_new_faces(phx.F_E, [a + 1 for a in forg], [a + 4 for a in forg])
     coarse.close()
@@ -181,4 +190,29 @@ def test_refined_B_is_divergence_free(dim):
     scale = max(np.nanmax(np.abs(x)) for x in Bf) / min(hf)
     assert np.max(np.abs(div)) <= 1e-13 * scale
     coarse.close()
+    fine.close()
+
+
+@pytest.mark.parametrize("dim,order", [(2, 1), (3, 2)])
+def test_magnetic_new_faces_edge_rule_matches_oracle(dim, order):
+    """a patch starting at an odd AMR cell: over every cell of its arrays the faces at the edge
+    take the edge rule (neighbour mean, or left), the others the full reconstruction, as the
+    oracle does"""
+    _say(f"[gpu] magnetic new faces edge rule dim={dim} order={order}")
+    rng = np.random.default_rng(700 + dim)
+    fn = [[14], [14, 10], [10, 8, 6]][dim - 1]
+    flo = [5] * dim
+    hf = [0.2, 0.15, 0.125][:dim]
+    fine = phx.Level(dim, order, fn, hf, lower=flo, periodic=False, apply_tuning_table=False)
+    g = [2, 4, 4][order - 1]
+    forg = [a - g for a in flo]
+    B = [rng.standard_normal(fine.shape(phx.F_B + k)) for k in range(3)]
+    for k in range(3):
+        fine.set(phx.F_B + k, B[k])
+    lo = list(forg)
+    hi = [flo[d] + fn[d] + g - 1 for d in range(dim)]
+    fine.refine_magnetic_new_faces(phx.F_B, lo, hi)
+    want = O.magnetic_new_faces(B, forg, hf, lo, hi)
+    for k in range(3):
+        np.testing.assert_array_equal(fine.get(phx.F_B + k), want[k], err_msg=f"B{k}")
     fine.close()

@@ -310,3 +310,32 @@ def test_new_faces_uniform_and_1d():
         else:
             assert out[0][loc] == B1[0][loc]
     assert np.array_equal(out[1], B1[1]) and np.array_equal(out[2], B1[2])
+
+
+def test_new_faces_edge_rule():
+    """at the edge of the arrays a new face whose cross terms would read beyond them takes the
+    mean of its two neighbours along the normal, and one without both neighbours is left (a box
+    aligned to even indices never needs the rule; here the arrays start at odd AMR indices): a
+    uniform B stays uniform, the first faces (no lower neighbour) keep their values, some faces
+    take the neighbour mean and the others the full reconstruction"""
+    dim = 2
+    fl, fn = [-3, -1], [14, 10]
+    org = [a - G for a in fl]
+    lo = list(org)
+    hi = [fl[d] + fn[d] + G - 1 for d in range(dim)]
+    val = [1.5, -0.25, 0.75]
+    Bf = [np.full(_shape(fn, [d == c for d in range(dim)]), val[c]) for c in range(3)]
+    Bn = O.magnetic_new_faces(Bf, org, [0.5, 0.5], lo, hi)
+    for c in range(3):
+        assert np.all(Bn[c] == val[c])
+    rng = np.random.default_rng(5)
+    Br = [rng.standard_normal(_shape(fn, [d == c for d in range(dim)])) for c in range(3)]
+    Bn = O.magnetic_new_faces(Br, org, [0.5, 0.5], lo, hi)
+    # Bx at local x 0 (AMR -5, odd: a new face without a lower neighbour) is left
+    assert np.array_equal(Bn[0][0, :], Br[0][0, :])
+    mean = 0.5 * (Br[0][:-2, :] + Br[0][2:, :])
+    new = Bn[0][1:-1, :]
+    odd_rows = [i for i in range(1, Br[0].shape[0] - 1) if (i + org[0]) % 2 != 0]
+    took_mean = [np.any(new[i - 1] == mean[i - 1]) for i in odd_rows]
+    full = [np.any(new[i - 1] != mean[i - 1]) for i in odd_rows]
+    assert any(took_mean) and any(full)
