@@ -57,8 +57,8 @@ def test_uniform_plasma_keeps_its_density_on_the_finer_patch(dim, order):
     ncells = [[64], [24, 20]][dim - 1]
     ppc = [200, 64][dim - 1]
     wl, coarse = _uniform(dim, order, ncells, ppc)
-    clo = [8, 6][:dim]
-    chi = [40, 13][:dim]
+    clo = [[8], [6, 5]][dim - 1]
+    chi = [[40], [17, 14]][dim - 1]
     h = H.TwoLevel(coarse, clo, chi)
     h.initialize()
     q0 = coarse.diagnostics()["charge"]
