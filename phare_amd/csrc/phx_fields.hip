@@ -595,7 +595,14 @@ __global__ __launch_bounds__(FB) void k_border(Geom G0, double const* orig, doub
     if (!nodeOf(G, QMOM, t, i, j, k))
         return;
     int const c[3] = {i, j, k};
-    int total      = 1;
+    // a node no periodic image overlaps (farther than the image shift from both ends in every
+    // direction) keeps its value: out already holds it (the callers copy out into orig first)
+    bool near = false;
+    for (int d = 0; d < G.dim; ++d)
+        near = near || c[d] >= G.n[d] || c[d] <= G.ghostEnd(QMOM, d) - G.n[d];
+    if (!near)
+        return;
+    int total = 1;
     for (int d = 0; d < G.dim; ++d)
         total *= 3;
     int const n = G.idx(QMOM, i, j, k);

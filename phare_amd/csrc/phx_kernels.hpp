@@ -344,6 +344,8 @@ void launchOhmAverage(Geom const& G, double eta, double nu, int hyper, int level
 void launchFillGhosts(Geom const& G, int q, double* f, hipStream_t s);
 // every node of f (quantity q) outside the physical nodes set to NaN
 void launchNanGhosts(Geom const& G, int q, double* f, hipStream_t s);
+// out = orig combined (sum / max) with its periodic images on the nodes they overlap; out must
+// already hold orig (the nodes no image overlaps are not written)
 void launchBorderSum(Geom const& G, double const* orig, double* out, hipStream_t s);
 void launchBorderMax(Geom const& G, double const* orig, double* inout, hipStream_t s);
 void launchZero(double* p, long count, hipStream_t s);
