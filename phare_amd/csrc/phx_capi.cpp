@@ -718,6 +718,7 @@ void enqueuePushStaged3d(phx_ctx* c, int p, PushArgs& A)
         A.sg   = c->sg;
         A.st   = P.seg;
         A.tile = segwgTileShape(c->cfg.dim, c->cfg.interp_order, c->sg, mixed(c) ? 4 : 5);
+        A.rhoFromN = !mixed(c) && P.hasCharge && P.uniformCharge ? 1 : 0;
         if (mixed(c))
             launchPush32SegWG(A, dom.view32(), c->stream);
         else

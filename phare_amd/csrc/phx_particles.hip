@@ -674,9 +674,10 @@ void k_push_seg(PushArgs A, PView in)
 // (the staged 3D path, segments of 4x4x4 cells): wavefront w takes rows w, w + 4, ..., so that
 // each instruction's 64 lanes walk 64 cells of the segment; the tile covers the segment with the
 // stencil and one cell of motion on each side; the overflow segment (number nsegs) deposits
-// straight into HBM
+// straight into HBM. RN: the population has one charge, the tile leaves the charge density out
+// and its flush takes the charge times n
 // ---------------------------------------------------------------------------------------------
-template<int D, int O>
+template<int D, int O, bool RN>
 __global__ __launch_bounds__(BLOCK) void k_push_segwg(PushArgs A, PView in)
 {
     extern __shared__ double lds[];
@@ -698,11 +699,11 @@ __global__ __launch_bounds__(BLOCK) void k_push_segwg(PushArgs A, PView in)
         accZero(acc);
 #pragma unroll 1
     for (unsigned int r = static_cast<unsigned int>(t >> 6); r < R; r += NW)
-        pushParticle<D, O>(A, in, in, off + static_cast<unsigned long long>(r) * LANES + lane, T, 0, acc);
+        pushParticle<D, O, RN>(A, in, in, off + static_cast<unsigned long long>(r) * LANES + lane, T, 0, acc);
     if constexpr (Cfg<D, O>::ACC)
         accFlush<D, O>(acc, G, T, 0, A.mom);
     __syncthreads();
-    flushTile<D>(G, T, A.tile.nodes, A.mom);
+    flushTile<D, RN>(G, T, A.tile.nodes, A.mom, A.charge);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1464,9 +1465,12 @@ void launchPushSegWG(PushArgs const& A, PView const& in, hipStream_t s)
     unsigned int const grid = static_cast<unsigned int>(A.sg.nsegs) + 1u;
     switch (A.G.dim * 10 + A.G.order)
     {
-        case 31: k_push_segwg<3, 1><<<grid, BLOCK, A.tile.bytes, s>>>(A, in); break;
-        case 32: k_push_segwg<3, 2><<<grid, BLOCK, A.tile.bytes, s>>>(A, in); break;
-        case 33: k_push_segwg<3, 3><<<grid, BLOCK, A.tile.bytes, s>>>(A, in); break;
+        case 31: A.rhoFromN ? k_push_segwg<3, 1, true><<<grid, BLOCK, A.tile.bytes, s>>>(A, in)
+                            : k_push_segwg<3, 1, false><<<grid, BLOCK, A.tile.bytes, s>>>(A, in); break;
+        case 32: A.rhoFromN ? k_push_segwg<3, 2, true><<<grid, BLOCK, A.tile.bytes, s>>>(A, in)
+                            : k_push_segwg<3, 2, false><<<grid, BLOCK, A.tile.bytes, s>>>(A, in); break;
+        case 33: A.rhoFromN ? k_push_segwg<3, 3, true><<<grid, BLOCK, A.tile.bytes, s>>>(A, in)
+                            : k_push_segwg<3, 3, false><<<grid, BLOCK, A.tile.bytes, s>>>(A, in); break;
         default: break;
     }
 }
