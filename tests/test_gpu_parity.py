@@ -379,6 +379,35 @@ def test_advance_uniform3d_sorted_every_step():
     _check_state(g, o, wl)
     g.close()
 
+@pytest.mark.parametrize("charges", ["one", "mixed"])
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_advance_3d_segmented_one_and_mixed_charges(order, charges):
+    """the staged 3D path's segment push for a population of one charge (the LDS tile leaves the
+    charge density out, the flush takes q n) and of mixed charges (deposited particle by
+    particle): 4 PPC steps sorted every 2 equal the oracle's at 1e-12, identical cells"""
+    _say(f"[gpu] advance 3d segmented order {order} charges {charges}")
+    wl = W.uniform3d(ncells=(12, 12, 12), ppc=16, order=order)
+    wl.dt = 0.02
+    rng = np.random.default_rng(5 + order)
+    g = phx.Level(3, order, wl.ncells, wl.dx, masses=wl.masses, Te=wl.Te, capacity=40000,
+                  apply_tuning_table=False)
+    g.set_sort_interval(2)
+    o = O.Sim(3, order, wl.ncells, wl.dx, masses=wl.masses, Te=wl.Te)
+    n = 16 * 12 ** 3
+    p = U.random_particles(rng, 3, n, wl.ncells, [0, 0, 0], 0, vscale=0.5)
+    p["charge"] = np.ones(n) if charges == "one" else np.where(rng.random(n) < 0.5, 1.0, 2.0)
+    for sim, fb in ((g, phx.F_B), (o, O.F_B)):
+        sim.set_particles(0, 0, p)
+        for c, b in enumerate(wl.b_arrays(lambda c: sim.shape(fb + c))):
+            sim.set(fb + c, b)
+        sim.initialize()
+    g.advance(wl.dt, 4)
+    o.advance(wl.dt, 4)
+    g.sync()
+    _check_state(g, o, wl)
+    g.close()
+
+
 def test_advance_harris2d_full_size_three_steps():
     """the full 2D Harris workload (256x128 cells, 100 ppc, 3.28 M particles, order 2) over 3 PPC
     steps against the oracle: identical cells, 1e-12 on fields, moments and particles"""
