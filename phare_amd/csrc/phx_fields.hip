@@ -584,24 +584,91 @@ __global__ __launch_bounds__(FB) void k_nan_ghosts(Geom G0, int q, double* f)
         f[G.idx(q, i, j, k)] = __builtin_nan("");
 }
 
+// the nodes of the all-primal ghost box (S nodes per direction) that a periodic image overlaps:
+// the m = S - N layers at each end of every direction, all S when the two ends meet (K nodes);
+// enumerated as the nodes near along direction 0, then those inside along 0 and near along 1,
+// then inside along 0 and 1 and near along 2
+struct Shell
+{
+    int S[3], m[3], K[3];
+    long count;
+};
+
+Shell shellOf(Geom const& G)
+{
+    Shell H{};
+    for (int d = 0; d < 3; ++d)
+    {
+        H.S[d] = d < G.dim ? G.shape[QMOM][d] : 1;
+        H.m[d] = d < G.dim ? H.S[d] - G.n[d] : 0;
+        H.K[d] = d < G.dim ? (2 * H.m[d] < H.S[d] ? 2 * H.m[d] : H.S[d]) : 0;
+    }
+    long const in0 = H.S[0] - H.K[0], in1 = H.S[1] - H.K[1];
+    H.count = static_cast<long>(H.K[0]) * H.S[1] * H.S[2];
+    if (G.dim > 1)
+        H.count += in0 * H.K[1] * H.S[2];
+    if (G.dim > 2)
+        H.count += in0 * in1 * H.K[2];
+    return H;
+}
+
+// coordinate of the u-th near node / v-th inner node along d
+__device__ __forceinline__ int shellNear(Shell const& H, int d, int u)
+{
+    return H.K[d] == H.S[d] || u < H.m[d] ? u : H.S[d] - H.K[d] + u;
+}
+
+template<int DIM>
+__device__ __forceinline__ bool shellNode(Shell const& H, long t, int* c)
+{
+    if (t >= H.count)
+        return false;
+    long const r0 = static_cast<long>(H.K[0]) * H.S[1] * H.S[2];
+    c[1] = c[2] = 0;
+    if (t < r0)
+    {
+        long const pl = static_cast<long>(H.S[1]) * H.S[2];
+        c[0]          = shellNear(H, 0, static_cast<int>(t / pl));
+        long const r  = t % pl;
+        c[1]          = static_cast<int>(r / H.S[2]);
+        c[2]          = static_cast<int>(r % H.S[2]);
+        return true;
+    }
+    t -= r0;
+    int const in0 = H.S[0] - H.K[0];
+    long const r1 = DIM > 1 ? static_cast<long>(in0) * H.K[1] * H.S[2] : 0;
+    if (t < r1)
+    {
+        long const pl = static_cast<long>(H.K[1]) * H.S[2];
+        c[0]          = H.m[0] + static_cast<int>(t / pl);
+        long const r  = t % pl;
+        c[1]          = shellNear(H, 1, static_cast<int>(r / H.S[2]));
+        c[2]          = static_cast<int>(r % H.S[2]);
+        return true;
+    }
+    t -= r1;
+    int const in1 = H.S[1] - H.K[1];
+    long const pl = static_cast<long>(in1) * H.K[2];
+    c[0]          = H.m[0] + static_cast<int>(t / pl);
+    long const r  = t % pl;
+    c[1]          = H.m[1] + static_cast<int>(r / H.K[2]);
+    c[2]          = shellNear(H, 2, static_cast<int>(r % H.K[2]));
+    return true;
+}
+
 // image k of {-1,0,1}^dim (lexicographic, zero excluded) overlaps the all-primal ghost box on
-// [max(0, k*N), min(end, end + k*N)]; node n receives orig[n - k*N]
+// [max(0, k*N), min(end, end + k*N)]; node n receives orig[n - k*N]. Launched over the shell of
+// nodes some image overlaps: the others keep their value (out already holds it, the callers
+// copy out into orig first)
 template<int DIM, bool MAX>
-__global__ __launch_bounds__(FB) void k_border(Geom G0, double const* orig, double* out)
+__global__ __launch_bounds__(FB) void k_border(Geom G0, Shell H, double const* orig, double* out)
 {
     GeomD<DIM> const G(G0);
     long const t = static_cast<long>(blockIdx.x) * FB + threadIdx.x;
-    int i, j, k;
-    if (!nodeOf(G, QMOM, t, i, j, k))
+    int c[3];
+    if (!shellNode<DIM>(H, t, c))
         return;
-    int const c[3] = {i, j, k};
-    // a node no periodic image overlaps (farther than the image shift from both ends in every
-    // direction) keeps its value: out already holds it (the callers copy out into orig first)
-    bool near = false;
-    for (int d = 0; d < G.dim; ++d)
-        near = near || c[d] >= G.n[d] || c[d] <= G.ghostEnd(QMOM, d) - G.n[d];
-    if (!near)
-        return;
+    int const i = c[0], j = c[1], k = c[2];
     int total = 1;
     for (int d = 0; d < G.dim; ++d)
         total *= 3;
@@ -1262,20 +1329,26 @@ void launchFillGhosts(Geom const& G, int q, double* f, hipStream_t s)
 }
 void launchBorderSum(Geom const& G, double const* orig, double* out, hipStream_t s)
 {
+    Shell const H = shellOf(G);
+    if (H.count == 0)
+        return;
     switch (G.dim)
     {
-        case 1: k_border<1, false><<<gridFor(G.size(QMOM)), FB, 0, s>>>(G, orig, out); break;
-        case 2: k_border<2, false><<<gridFor(G.size(QMOM)), FB, 0, s>>>(G, orig, out); break;
-        case 3: k_border<3, false><<<gridFor(G.size(QMOM)), FB, 0, s>>>(G, orig, out); break;
+        case 1: k_border<1, false><<<gridFor(H.count), FB, 0, s>>>(G, H, orig, out); break;
+        case 2: k_border<2, false><<<gridFor(H.count), FB, 0, s>>>(G, H, orig, out); break;
+        case 3: k_border<3, false><<<gridFor(H.count), FB, 0, s>>>(G, H, orig, out); break;
     }
 }
 void launchBorderMax(Geom const& G, double const* orig, double* inout, hipStream_t s)
 {
+    Shell const H = shellOf(G);
+    if (H.count == 0)
+        return;
     switch (G.dim)
     {
-        case 1: k_border<1, true><<<gridFor(G.size(QMOM)), FB, 0, s>>>(G, orig, inout); break;
-        case 2: k_border<2, true><<<gridFor(G.size(QMOM)), FB, 0, s>>>(G, orig, inout); break;
-        case 3: k_border<3, true><<<gridFor(G.size(QMOM)), FB, 0, s>>>(G, orig, inout); break;
+        case 1: k_border<1, true><<<gridFor(H.count), FB, 0, s>>>(G, H, orig, inout); break;
+        case 2: k_border<2, true><<<gridFor(H.count), FB, 0, s>>>(G, H, orig, inout); break;
+        case 3: k_border<3, true><<<gridFor(H.count), FB, 0, s>>>(G, H, orig, inout); break;
     }
 }
 void launchIonTotals(PopMoments const& P, double* rho, double* massDensity, VecPtr Vi, long count,
