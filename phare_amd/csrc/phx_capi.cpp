@@ -1071,7 +1071,8 @@ namespace
 // counting sort of every population's domain array into cell-segmented storage: scattered into
 // the population's sort buffer, then copied back, so that the domain array keeps its address and
 // the captured step its graph. The two swap when the array is linear (the first sort after an
-// upload) or too small for the sorted slots. Lost particles are dropped.
+// upload) or too small for the sorted slots, and always on the staged 3D path (no captured step).
+// Lost particles are dropped.
 phx_status sortDomain(phx_ctx* c)
 {
     for (int p = 0; p < c->cfg.n_pop; ++p)
@@ -1130,8 +1131,9 @@ phx_status sortDomain(phx_ctx* c)
             launchSegScatter(c->sg, dom.view(), buf.view(), dom.n, slots, c->segWork, c->sortRank,
                              c->sortPerm, P.seg, c->cfg.dim, c->stream);
         PHX_TRY(checkLaunch(c, "particle sort (scatter)"));
-        // a linear array (an upload, sized for it) moves into the buffer with its headroom
-        if (dom.seg && slots <= dom.cap)
+        // a linear array (an upload, sized for it) moves into the buffer with its headroom; the
+        // staged 3D path captures no step, so its two storages always swap
+        if (dom.seg && slots <= dom.cap && !staged3d(c))
         {
             if (mixed(c))
                 launchSegCopy(buf.view32(), dom.view32(), slots, c->cfg.dim, c->stream);
