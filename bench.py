@@ -121,8 +121,11 @@ def run_family(wl, precision, steps, warmup, drift_steps=0):
     prec = phx.PRECISION_FP32 if precision == "fp32" else phx.PRECISION_FP64
     lvl = phx.Level(wl.dim, wl.order, wl.ncells, wl.dx, masses=wl.masses, Te=wl.Te,
                     capacity=wl.n_particles, precision=prec)
+    t_setup = time.perf_counter()
     wl.setup(lvl, phx.F_B)
     diag0 = lvl.diagnostics()
+    print(f"[bench] {precision}: level loaded and initialised in {time.perf_counter() - t_setup:.1f} s",
+          file=sys.stderr, flush=True)
     # the pushes of the captured steps time themselves (first workgroup start to last workgroup
     # end, device wall clock); enabled before the warm-up so that the timed steps replay the
     # step captured with it, and reset right before the timed steps
@@ -130,12 +133,14 @@ def run_family(wl, precision, steps, warmup, drift_steps=0):
     lvl.advance(wl.dt, warmup)
     lvl.sync()
     device_sync()
+    print(f"[bench] {precision}: {warmup} warm-up steps done", file=sys.stderr, flush=True)
     lvl.set_push_timing(True)
     t0 = time.perf_counter()
     lvl.advance(wl.dt, steps)
     lvl.sync()
     device_sync()
     elapsed = time.perf_counter() - t0
+    print(f"[bench] {precision}: {steps} timed steps in {elapsed:.2f} s", file=sys.stderr, flush=True)
     ms_all, n_all = lvl.push_timing(phx.MODE_ALL)
     ms_dom, n_dom = lvl.push_timing(phx.MODE_DOMAIN_ONLY)
     assert n_all == steps and n_dom == steps, (n_all, n_dom, steps)
