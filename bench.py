@@ -40,7 +40,7 @@ HBM_PEAK_GBS = 8000.0
 # workload and precision family (k_push_seg / k_push_seg32 in 2D, k_push_segwg / k_push32_segwg on
 # the 3D staged path)
 MEASURED_TRAFFIC = {("harris2d", "fp64"): 401.1e6, ("harris2d", "fp32"): 220.9e6,
-                    ("uniform3d", "fp64"): 19.46e9, ("uniform3d", "fp32"): 11.03e9}
+                    ("uniform3d", "fp64"): 19.41e9, ("uniform3d", "fp32"): 11.03e9}
 
 
 def _hip():
