@@ -1343,6 +1343,7 @@ phx_status buildPatchTable(phx_ctx* c, int const* plo, int const* phi, int np)
     S.patch    = dP;
     S.owner    = dO;
     S.segPatch = dS;
+    S.first    = c->segPatch.front();
     return PHX_OK;
 }
 

@@ -105,6 +105,7 @@ struct SegGeom
     int maxSeg[3]; // largest segment extent over the patches (the launch's LDS tile)
     int rowsPerItem; // most segment rows per work item of the segmented push
     int rowPad;      // 2D LDS tile rows padded to a multiple of this many nodes
+    SegPatch first;               // patch 0 (a one-patch level reads it from the kernel arguments)
     SegPatch const* patch;        // [npatch] (device)
     int const* owner;             // [level box cells, row-major] patch of each cell, -1: none
     unsigned int const* segPatch; // [nsegs] patch of each segment

@@ -103,8 +103,9 @@ __device__ __forceinline__ void segTileOrigin(Geom const& G, SegGeom const& S, T
 {
     if (seg < static_cast<unsigned int>(S.nsegs))
     {
-        SegPatch const& P = S.patch[S.segPatch[seg]];
-        int rem = static_cast<int>(seg - P.segBase);
+        // one patch: its record comes with the kernel arguments, no dependent table reads
+        SegPatch const P = S.npatch == 1 ? S.first : S.patch[S.segPatch[seg]];
+        int rem          = static_cast<int>(seg - P.segBase);
         int sidx[3];
 #pragma unroll
         for (int d = D - 1; d >= 0; --d)
