@@ -84,6 +84,41 @@ def test_uniform_plasma_keeps_its_density_on_the_finer_patch(dim, order):
     coarse.close()
 
 
+@pytest.mark.parametrize("order", [1, 2])
+def test_uniform_plasma_keeps_its_density_on_a_3d_finer_patch(order):
+    """3D: a periodic 16^3 coarser level (the staged path) and a 16^3-cell finer patch over coarse
+    cells 4..11: the refined uniform plasma keeps its density on the patch (mean over the patch,
+    mean over its two outer node layers, every node within the particle noise), no NaN, and the
+    coarser level's charge is conserved, over coarse steps with subcycling"""
+    _say(f"[gpu] two-level uniform plasma 3D order={order}")
+    wl = W.uniform3d(ncells=(16, 16, 16), ppc=16, order=order)
+    coarse = phx.Level(3, order, wl.ncells, wl.dx, masses=wl.masses, Te=wl.Te,
+                       capacity=wl.n_particles, apply_tuning_table=False)
+    wl.setup(coarse, phx.F_B)
+    h = H.TwoLevel(coarse, [4, 4, 4], [11, 11, 11])
+    h.initialize()
+    q0 = coarse.diagnostics()["charge"]
+    fine = h.fine
+    g = H.GHOSTS[order]
+    for step in range(3):
+        rho = _physical(fine, phx.F_RHO)
+        assert not np.any(np.isnan(rho)), f"NaN density after {step} steps"
+        assert abs(rho.mean() - 1.0) < 0.03, (step, rho.mean())
+        assert np.abs(rho - 1.0).max() < 0.6, (step, np.abs(rho - 1.0).max())
+        edge = np.concatenate([rho[:2].ravel(), rho[-2:].ravel(), rho[:, :2].ravel(), rho[:, -2:].ravel(),
+                               rho[:, :, :2].ravel(), rho[:, :, -2:].ravel()])
+        assert abs(edge.mean() - 1.0) < 0.06, (step, edge.mean())
+        for fid in list(range(0, 6)) + [phx.F_J, phx.F_J + 1, phx.F_J + 2]:
+            a = fine.get(fid)
+            sl = tuple(slice(g, a.shape[d] - g) for d in range(3))
+            assert not np.any(np.isnan(a[sl])), (step, fid)
+        h.advance(wl.dt)
+    assert abs(coarse.diagnostics()["charge"] - q0) <= 1e-12 * abs(q0)
+    _say("  ok")
+    h.close()
+    coarse.close()
+
+
 def _div(B, dx, dim, g):
     """discrete divergence of a Yee B over the physical cells"""
     n = [B[1].shape[0] - 2 * g] if dim == 1 else [B[1].shape[0] - 2 * g, B[0].shape[1] - 2 * g]
