@@ -166,3 +166,67 @@ def test_divergence_free_b_on_both_levels():
         h.advance(wl.dt)
     h.close()
     coarse.close()
+
+
+def _div3(B, dx, g):
+    """discrete divergence of a 3D Yee B over the physical cells"""
+    n = [B[0].shape[0] - 2 * g - 1, B[1].shape[1] - 2 * g - 1, B[2].shape[2] - 2 * g - 1]
+    div = 0.0
+    for d in range(3):
+        a = [slice(g, g + n[k]) for k in range(3)]
+        b = list(a)
+        b[d] = slice(g + 1, g + n[d] + 1)
+        div = div + (B[d][tuple(b)] - B[d][tuple(a)]) / dx[d]
+    return div
+
+
+def test_divergence_free_b_on_both_levels_3d():
+    """3D: a coarse B made divergence-free as the discrete curl of A = (Ax(y, z), 0, Az(x, y)) on
+    the Yee edges, with a uniform plasma: the finer B from the init refiner and the 3D new faces is
+    divergence-free, and both levels keep div B at round-off over coarse steps with subcycling and
+    reflux"""
+    _say("[gpu] two-level div B 3D")
+    order = 1
+    wl = W.uniform3d(ncells=(16, 16, 16), ppc=8, order=order)
+    coarse = phx.Level(3, order, wl.ncells, wl.dx, masses=wl.masses, Te=wl.Te,
+                       capacity=wl.n_particles, apply_tuning_table=False)
+    wl.setup(coarse, phx.F_B)
+    g = H.GHOSTS[order]
+    dx = coarse.dx
+    L = [16 * h for h in dx]
+
+    def prim(d, i):
+        return (i - g) * dx[d]
+
+    def Ax(y, z):
+        return 0.04 * np.sin(2 * np.pi * y / L[1]) * np.cos(2 * np.pi * z / L[2])
+
+    def Az(x, y):
+        return 0.05 * np.cos(2 * np.pi * x / L[0]) * np.sin(2 * np.pi * y / L[1])
+
+    B = [coarse.get(phx.F_B + k) for k in range(3)]
+    # Bx (primal, dual, dual) = dAz/dy
+    i, j, k = np.meshgrid(*[np.arange(s) for s in B[0].shape], indexing="ij")
+    B[0] = B[0] + (Az(prim(0, i), prim(1, j + 1)) - Az(prim(0, i), prim(1, j))) / dx[1]
+    # By (dual, primal, dual) = dAx/dz - dAz/dx
+    i, j, k = np.meshgrid(*[np.arange(s) for s in B[1].shape], indexing="ij")
+    B[1] = B[1] + (Ax(prim(1, j), prim(2, k + 1)) - Ax(prim(1, j), prim(2, k))) / dx[2] \
+        - (Az(prim(0, i + 1), prim(1, j)) - Az(prim(0, i), prim(1, j))) / dx[0]
+    # Bz (dual, dual, primal) = -dAx/dy
+    i, j, k = np.meshgrid(*[np.arange(s) for s in B[2].shape], indexing="ij")
+    B[2] = B[2] - (Ax(prim(1, j + 1), prim(2, k)) - Ax(prim(1, j), prim(2, k))) / dx[1]
+    for c in range(3):
+        coarse.set(phx.F_B + c, B[c])
+    coarse.initialize()
+    h = H.TwoLevel(coarse, [4, 4, 4], [11, 11, 11])
+    h.initialize()
+    fine = h.fine
+    for step in range(3):
+        for lvl, name in ((coarse, "coarse"), (fine, "fine")):
+            Bl = [lvl.get(phx.F_B + c) for c in range(3)]
+            div = _div3(Bl, lvl.dx, g)
+            scale = max(np.abs(b).max() for b in Bl) / min(lvl.dx)
+            assert np.abs(div).max() <= 1e-12 * scale, (name, step, np.abs(div).max())
+        h.advance(wl.dt)
+    h.close()
+    coarse.close()
