@@ -1,6 +1,7 @@
 """The two-level subcycled hierarchy (phare_amd/hierarchy.py: MultiPhysicsIntegrator's advance and
 synchronisation over a periodic coarser level and one finer patch, the messenger's fills done by
-the refine/coarsen entry points). No oracle restates the reference's driver (SAMRAI schedules):
+the refine/coarsen entry points), in 1D, 2D and 3D (the 3D coarser level on the staged path). No
+oracle restates the reference's driver (SAMRAI schedules):
 these tests pin what the mathematics fixes ("parity unpinned" for the driver's sequence itself):
  - a uniform plasma refined into the finer patch keeps its density everywhere on the patch, up to
    the coarse-fine boundary (without the level ghost particles and their alpha-weighted moments
