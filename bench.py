@@ -282,8 +282,9 @@ def main():
         "charge_drift": f64["charge_drift"],
         "run": line["run"],
         "headline": "the north star's headline (batched FP32 vs plain FP64 on the 3D AMR "
-                    "two-population configuration) is not measurable in this build: no AMR "
-                    "hierarchy driver; mixed_fp32 below is the same comparison on this "
+                    "two-population configuration) is not measurable in this build: its "
+                    "hierarchy driver has two levels with one finer patch, and the finer level "
+                    "runs in FP64 only; mixed_fp32 below is the same comparison on this "
                     "single-patch 3D workload",
     }
     if not args.no_mixed:
