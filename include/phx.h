@@ -348,7 +348,8 @@ typedef struct {
                              most 64; clipped to each patch, the cells left over go to the slower
                              dimensions); 0 keeps the default of the dimension */
     int rows_per_item;    /* most segment rows per workgroup of the segmented push (1..4096) */
-    int tile_row_pad;     /* 2D LDS tile rows padded to a multiple of this many nodes (1: none) */
+    int tile_row_pad;     /* LDS tile pitches padded to a multiple of this many nodes (1: none):
+                             2D the rows; 3D (segment push) the pitches of dimensions 1 and 2 */
     int sort_interval;    /* PPC steps of phx_advance between two sorts (0: never) */
     int tile_copies;      /* moment copies of the LDS tile of the linear-storage push (1, 2, 4, 8;
                              lane l adds into copy l % copies) */

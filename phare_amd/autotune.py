@@ -9,7 +9,7 @@ their interval) of a fresh level from the workload's initial state, two passes. 
 first initialisation, and C++ callers pass the same values to phx_set_tuning.
 
 Tunables (their meaning is in include/phx.h): the segment shape of the cell-segmented storage,
-the most segment rows per workgroup of the segmented push, the row padding of the 2D LDS tile,
+the most segment rows per workgroup of the segmented push, the pitch padding of the LDS tile,
 the sort interval, the moment copies of the linear-storage push's tile (the 3D staged path), and
 for the mixed-precision family the register window of the deposit.
 
@@ -36,7 +36,8 @@ SPACE = {
     2: {"segment_shape": [[2, 32, 1], [4, 16, 1], [1, 64, 1], [8, 8, 1]],
         "rows_per_item": [16, 24, 32, 48, 64], "tile_row_pad": [1, 8, 16, 32],
         "sort_interval": [15, 20, 30, 40, 60]},
-    3: {"segment_shape": [[4, 4, 4], [2, 4, 8]], "sort_interval": [5, 10, 15, 30]},
+    3: {"segment_shape": [[4, 4, 4], [2, 4, 8]], "tile_row_pad": [1, 4],
+        "sort_interval": [5, 10, 15, 30]},
 }
 
 

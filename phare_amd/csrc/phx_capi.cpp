@@ -717,7 +717,8 @@ void enqueuePushStaged3d(phx_ctx* c, int p, PushArgs& A)
     {
         A.sg   = c->sg;
         A.st   = P.seg;
-        A.tile = segwgTileShape(c->cfg.dim, c->cfg.interp_order, c->sg, mixed(c) ? 4 : 5);
+        A.tile = segwgTileShape(c->cfg.dim, c->cfg.interp_order, c->sg, mixed(c) ? 4 : 5,
+                                c->tuning.tile_row_pad);
         A.rhoFromN = !mixed(c) && P.hasCharge && P.uniformCharge ? 1 : 0;
         if (mixed(c))
             launchPush32SegWG(A, dom.view32(), c->stream);
@@ -1374,14 +1375,18 @@ phx_status allocSegTables(phx_ctx* c)
 // workload (segments of 2x32 cells, at most 32 rows per work item, tile rows padded to 16 nodes,
 // a sort every 30 PPC steps); runs of 64 cells in 1D, 4x4x4 blocks in 3D (the staged 3D path's
 // one-workgroup-per-segment push gives each lane of a wavefront one cell of the block)
-phx_tuning defaultTuning(int dim)
+phx_tuning defaultTuning(int dim, int order)
 {
     phx_tuning t{};
     static int const pref[3][3] = {{64, 1, 1}, {2, 32, 1}, {4, 4, 4}};
     for (int d = 0; d < 3; ++d)
         t.segment_shape[d] = pref[dim - 1][d];
     t.rows_per_item = PHX_SEG_ROWS;
-    t.tile_row_pad  = PHX_TILE_ROW_PAD;
+    // 3D: the segment tile's pitches padded to a multiple of 4 nodes at orders 1 and 3 (every
+    // particle's stencil starts at a fixed node of its cell, so the 64 lanes of one LDS atomic
+    // fall on distinct bank pairs); unpadded at order 2 (starts shift with the delta at random,
+    // and the larger tile would halve the workgroups per CU)
+    t.tile_row_pad  = dim == 3 ? (order == 2 ? 1 : 4) : PHX_TILE_ROW_PAD;
     t.sort_interval = 30;
     t.tile_copies   = dim == 1 ? PHX_TILE_COPIES_1D : (dim == 2 ? PHX_TILE_COPIES_2D : PHX_TILE_COPIES_3D);
     t.deposit_window = 0;
@@ -1548,7 +1553,7 @@ phx_status createLevel(phx_config const* cfg, phx_patch const* patch, int const*
         for (int k = 0; k < 5 && s == PHX_OK; ++k)
             s = storeAlloc(c, c->pops[p].arr[k], k == 0 ? std::max<size_t>(particle_capacity, 1024) : 1024);
     }
-    c->tuning = defaultTuning(D);
+    c->tuning = defaultTuning(D, cfg->interp_order);
     c->sortInterval = c->tuning.sort_interval;
     if (s == PHX_OK)
         s = buildPatchTable(c, plo, phi, np);
@@ -2806,7 +2811,7 @@ phx_status phx_set_tuning(phx_ctx* c, phx_tuning const* t)
         return PHX_ERR_ARG;
     int const D       = c->cfg.dim;
     phx_tuning next   = *t;
-    phx_tuning const d = defaultTuning(D);
+    phx_tuning const d = defaultTuning(D, c->cfg.interp_order);
     int cells         = 1;
     for (int k = 0; k < 3; ++k)
     {

@@ -218,7 +218,7 @@ TileSpec brickTileShape32(int order);
 // of segwgTileShape (5 slabs FP64, 4 mixed precision)
 void launchPushSegWG(PushArgs const& A, PView const& in, hipStream_t s);
 void launchPush32SegWG(PushArgs const& A, PView32 const& v, hipStream_t s);
-TileSpec segwgTileShape(int dim, int order, SegGeom const& sg, int slabs);
+TileSpec segwgTileShape(int dim, int order, SegGeom const& sg, int slabs, int pad);
 // BorisPusher::move in place over A.n slots with box-list selectors; status[i] gets the slot's
 // fate for the partition (0 both selectors, 1 first only, 2 neither, 3 empty slot)
 void launchMove(PushArgs const& A, PView const& v, SelBoxes const& first, SelBoxes const& second,

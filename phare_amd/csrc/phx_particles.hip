@@ -1430,8 +1430,9 @@ void launchCompact(PView const& src, PView const& dst, uint8_t const* status, un
 }
 
 // tile of the workgroup-per-segment push: the largest segment plus the stencil and one cell of
-// motion below (2 nodes) and above (order + 2 nodes), `slabs` moment slabs
-TileSpec segwgTileShape(int dim, int order, SegGeom const& sg, int slabs)
+// motion below (2 nodes) and above (order + 2 nodes), `slabs` moment slabs, the pitches of
+// dimensions 1 and 2 padded to a multiple of `pad` nodes
+TileSpec segwgTileShape(int dim, int order, SegGeom const& sg, int slabs, int pad)
 {
     TileSpec T{};
     T.copies = 1;
@@ -1441,9 +1442,23 @@ TileSpec segwgTileShape(int dim, int order, SegGeom const& sg, int slabs)
         T.ext[d]   = d < dim ? sg.maxSeg[d] + 2 + order + 2 : 1;
         T.pitch[d] = T.ext[d];
     }
-    T.nodes  = T.ext[0] * T.ext[1] * T.ext[2];
-    T.stride = (T.nodes + 15) / 16 * 16;
-    T.bytes  = static_cast<unsigned int>(8 * slabs * T.stride);
+    auto size = [&]() {
+        T.nodes  = T.ext[0] * T.pitch[1] * T.pitch[2];
+        T.stride = (T.nodes + 15) / 16 * 16;
+        T.bytes  = static_cast<unsigned int>(8 * slabs * T.stride);
+    };
+    if (dim == 3 && pad > 1)
+    {
+        // pitches of dimensions 1 and 2 padded to a multiple of pad nodes (the padding nodes are
+        // never deposited into: the flush skips them as zeros); dropped if the tile outgrows LDS
+        for (int d = 1; d < 3; ++d)
+            T.pitch[d] = (T.ext[d] + pad - 1) / pad * pad;
+        size();
+        if (T.bytes > TILE_BYTES_MAX)
+            for (int d = 1; d < 3; ++d)
+                T.pitch[d] = T.ext[d];
+    }
+    size();
     if (T.bytes > TILE_BYTES_MAX)
     {
         // no room: every deposit goes straight to HBM
