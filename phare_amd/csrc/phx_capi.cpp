@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <limits>
 #include <random>
@@ -146,6 +147,7 @@ struct phx_ctx
     int stepsSinceSort    = 0;
     // launch shapes of the segmented push (phx_set_tuning)
     phx_tuning tuning{};
+    bool tileFallbackNoted = false; // the note that a segment tile outgrew LDS was printed
 
     // captured PPC steps, keyed on dt, the raw deposit set and the particle storage they address
     struct GraphEntry
@@ -394,6 +396,20 @@ phx_status checkDeviceError(phx_ctx* c)
 
 // arguments of the push kernel; `segOk` lets a cell-segmented domain array use the segmented
 // kernel (callers that need a fate for every slot, the compacting path, pass false)
+// a segment tile that outgrew the LDS budget (an extreme segment shape from phx_set_tuning):
+// the launch deposits every particle straight into the moment arrays, correct but slow; said
+// once per context on stderr
+void noteTileFallback(phx_ctx* c, TileSpec const& T)
+{
+    if (T.lo[0] != -0x10000000 || c->tileFallbackNoted)
+        return;
+    c->tileFallbackNoted = true;
+    std::fprintf(stderr,
+                 "phx: the LDS tile of the segment push does not fit for segments of %dx%dx%d "
+                 "cells; its deposits go straight to the moment arrays (slower)\n",
+                 c->tuning.segment_shape[0], c->tuning.segment_shape[1], c->tuning.segment_shape[2]);
+}
+
 PushArgs pushArgs(phx_ctx* c, int pop, int kind, int mode, int emSet, double dt, bool segOk)
 {
     PushArgs A{};
@@ -443,6 +459,7 @@ PushArgs pushArgs(phx_ctx* c, int pop, int kind, int mode, int emSet, double dt,
                                  : segTileShape(c->cfg.dim, c->cfg.interp_order, c->sg, PHX_SEG_COPIES);
         A.rhoFromN    = !mixed(c) && P.hasCharge && P.uniformCharge ? 1 : 0;
         A.depositWindow = c->tuning.deposit_window;
+        noteTileFallback(c, A.tile);
     }
     return A;
 }
@@ -719,6 +736,7 @@ void enqueuePushStaged3d(phx_ctx* c, int p, PushArgs& A)
         A.st   = P.seg;
         A.tile = segwgTileShape(c->cfg.dim, c->cfg.interp_order, c->sg, mixed(c) ? 4 : 5,
                                 c->tuning.tile_row_pad);
+        noteTileFallback(c, A.tile);
         A.rhoFromN = !mixed(c) && P.hasCharge && P.uniformCharge ? 1 : 0;
         if (mixed(c))
             launchPush32SegWG(A, dom.view32(), c->stream);

@@ -102,6 +102,31 @@ def test_tuning_of_a_level_of_uneven_patches():
     g.close()
 
 
+def test_segment_tile_beyond_lds_falls_back_to_direct_deposits(capfd):
+    """3D order 3 with segments of 1x1x64 cells: the segment tile (8x8x71 nodes, 5 FP64 slabs)
+    does not fit in LDS, so every deposit goes straight to the moment arrays; the run says so once
+    on stderr and still equals the oracle's at 1e-12 (3 PPC steps, sorted every step)"""
+    _say("[gpu] segment tile beyond LDS")
+    wl = W.uniform3d(ncells=(10, 10, 64), ppc=4, order=3)
+    g = phx.Level(3, 3, wl.ncells, wl.dx, masses=wl.masses, Te=wl.Te, capacity=wl.n_particles,
+                  apply_tuning_table=False)
+    g.set_tuning(segment_shape=[1, 1, 64], sort_interval=1)
+    o = O.Sim(3, 3, wl.ncells, wl.dx, masses=wl.masses, Te=wl.Te)
+    wl.setup(g, phx.F_B)
+    wl.setup(o, O.F_B)
+    g.advance(wl.dt, 3)
+    o.advance(wl.dt, 3)
+    g.sync()
+    err = capfd.readouterr().err
+    assert err.count("does not fit for segments of 1x1x64 cells") == 1, err
+    for fid in range(6):
+        U.assert_close(g.get(fid), o.get(fid), what=f"field {fid}")
+    for m in range(5):
+        U.assert_close(g.get(phx.f_pop(0, m)), o.get(O.f_pop(0, m)), what=f"moment {m}")
+    U.assert_same_particles(g.get_particles(), o.get_particles())
+    g.close()
+
+
 def test_tuning_defaults_and_refusals():
     _say("[gpu] tuning defaults and refusals")
     g = phx.Level(2, 2, [40, 24], [0.4, 0.4], capacity=1000, apply_tuning_table=False)
