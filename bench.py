@@ -39,7 +39,7 @@ HBM_PEAK_GBS = 8000.0
 # this script, FETCH_SIZE scaled x2 and WRITE_SIZE x1 by a 1 GiB device copy's readings; keyed by
 # workload and precision family (k_push_seg / k_push_seg32 in 2D, k_push_segwg / k_push32_segwg on
 # the 3D staged path)
-MEASURED_TRAFFIC = {("harris2d", "fp64"): 401.1e6, ("harris2d", "fp32"): 220.9e6,
+MEASURED_TRAFFIC = {("harris2d", "fp64"): 395.0e6, ("harris2d", "fp32"): 224.3e6,
                     ("uniform3d", "fp64"): 19.41e9, ("uniform3d", "fp32"): 11.03e9}
 
 
